@@ -1,0 +1,243 @@
+"""Input-deck writer for PIAFS cases.
+
+Writes the reference solver's own input files (``solver.inp``,
+``boundary.inp``, ``physics.inp``, ``weno.inp``, ``chemistry.inp``,
+``initial.inp``, ``imap.inp``) so that the reference binary, the CPU
+restatement and the MI355X path all read byte-identical inputs.
+
+File formats follow the reference: keyword files ``begin ... end``
+(src/Simulation/ReadInputs.c:244-429, doc/Input_Files.md:29-52), boundary
+zones (src/Simulation/InitializeBoundaries.c:60-200), binary arrays with the
+grid first and then the point-major solution (src/IOFunctions/ReadArray.c:
+235-262).  Random perturbations use SplitMix64 so the sequence is fixed by
+the seed, independent of numpy's generators.
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n uniform doubles in [0,1) from SplitMix64 (Steele, Lea, Flood 2014)."""
+    out = np.empty(n, dtype=np.float64)
+    s = seed & MASK64
+    for i in range(n):
+        s = (s + 0x9E3779B97F4A7C15) & MASK64
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        z = z ^ (z >> 31)
+        out[i] = (z >> 11) * (1.0 / (1 << 53))
+    return out
+
+
+def splitmix64_fast(seed: int, n: int) -> np.ndarray:
+    """Vectorised SplitMix64: same stream as :func:`splitmix64`."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        s = np.uint64(seed & MASK64) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = s
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def _kw_file(path: str, items: dict) -> None:
+    with open(path, "w") as f:
+        f.write("begin\n")
+        for k, v in items.items():
+            if isinstance(v, (list, tuple)):
+                v = " ".join(str(x) for x in v)
+            elif isinstance(v, float):
+                v = repr(v)
+            f.write(f"  {k:<20s} {v}\n")
+        f.write("end\n")
+
+
+@dataclass
+class Deck:
+    """A complete case: solver keywords, boundary zones, physics/weno/chemistry
+    keywords and the binary initial solution (grid + point-major u)."""
+    solver: dict
+    boundary: list
+    physics: dict
+    x: list                      # one coordinate array per dimension
+    u: np.ndarray                # interior solution, shape (prod(dim), nvars)
+    weno: dict = field(default_factory=dict)
+    chemistry: dict | None = None
+    imap: np.ndarray | None = None
+
+    def write(self, d: str) -> str:
+        os.makedirs(d, exist_ok=True)
+        _kw_file(os.path.join(d, "solver.inp"), self.solver)
+        with open(os.path.join(d, "boundary.inp"), "w") as f:
+            f.write(f"{len(self.boundary)}\n")
+            for b in self.boundary:
+                f.write(" ".join(str(t) for t in b) + "\n")
+        _kw_file(os.path.join(d, "physics.inp"), self.physics)
+        if self.weno:
+            _kw_file(os.path.join(d, "weno.inp"), self.weno)
+        if self.chemistry is not None:
+            _kw_file(os.path.join(d, "chemistry.inp"), self.chemistry)
+        with open(os.path.join(d, "initial.inp"), "wb") as f:
+            for xd in self.x:
+                np.ascontiguousarray(xd, dtype=np.float64).tofile(f)
+            np.ascontiguousarray(self.u, dtype=np.float64).tofile(f)
+        if self.imap is not None:
+            with open(os.path.join(d, "imap.inp"), "wb") as f:
+                for xd in self.x:
+                    np.ascontiguousarray(xd, dtype=np.float64).tofile(f)
+                np.ascontiguousarray(self.imap, dtype=np.float64).tofile(f)
+        return d
+
+    @property
+    def dims(self):
+        return [len(xd) for xd in self.x]
+
+    @property
+    def nvars(self):
+        return int(self.solver["nvars"])
+
+
+# --------------------------------------------------------------------------
+# physical constants needed to pick Ptot for a target Reynolds number
+# (src/PhysicalModels/Chemistry/ChemistryInitialize.c:85-135,289-314 and
+#  src/PhysicalModels/NavierStokes2D/NavierStokes2DInitialize.c:188-216)
+# --------------------------------------------------------------------------
+
+def chemistry_reynolds(Ptot: float, Ti: float = 288.0, f_CO2: float = 0.0,
+                       lambda_UV: float = 2.48e-7, theta: float = 0.17 * math.pi / 180,
+                       T0: float = 275.0, TS: float = 110.4, TA: float = 245.4, TB: float = 27.6):
+    NA, kB = 6.02214076e23, 1.380649e-23
+    f_O2 = 1.0 - f_CO2
+    R = NA * kB / (f_O2 * 0.032 + f_CO2 * 0.044)
+    Cp = f_O2 * 918.45 + f_CO2 * 842.86
+    Cv = f_O2 * 650.0 + f_CO2 * 654.5
+    gam = Cp / Cv
+    mu0 = f_O2 * 1.99e-5 + f_CO2 * 1.45e-5
+    kap0 = f_O2 * 2.70e-2 + f_CO2 * 1.58e-2
+    kg = 2 * (2 * math.pi / lambda_UV) * math.sin(theta)
+    cs = math.sqrt(gam * R * Ti)
+    L_ref = 1.0 / kg
+    rho_ref = Ptot / (R * Ti)
+    mu_ref = mu0 * math.exp(1.5 * math.log(Ti / T0)) * ((T0 + TS) / (Ti + TS))
+    kap_ref = kap0 * math.exp(1.5 * math.log(Ti / T0)) * (
+        (T0 + TA * math.exp(-TB / T0)) / (Ti + TA * math.exp(-TB / Ti)))
+    Re = rho_ref * cs * L_ref / mu_ref
+    Pr = gam * R / (gam - 1) * mu_ref / kap_ref
+    return Re, Pr, gam
+
+
+def ptot_for_reynolds(Re_target: float) -> float:
+    """Re is proportional to Ptot (rho_ref = Ptot/(R Ti)); scale from 1 atm."""
+    Re1, _, _ = chemistry_reynolds(101325.0)
+    return 101325.0 * Re_target / Re1
+
+
+# --------------------------------------------------------------------------
+# case builders
+# --------------------------------------------------------------------------
+
+def sod_1d(N: int = 1024, n_iter: int | None = None, t_final: float = 0.2, scheme: str = "weno5",
+           interp: str = "characteristic", upwinding: str = "roe", rk: str = "ssprk3",
+           mapped: int = 0, gamma: float = 1.4, x_from_generator: bool = False,
+           op_file_format: str = "none") -> Deck:
+    """Sod shock tube on [0,1]: (rho,u,p) = (1,0,1) | (0.125,0,0.1), mirror
+    ("extrapolate") BCs, fixed dt = t_final/n_iter with n_iter = ceil(0.88 (N-1))
+    (acoustic CFL about 0.5 for max|u|+c = 2.19 of the exact solution)."""
+    if n_iter is None:
+        n_iter = int(math.ceil(0.88 * (N - 1)))
+    dt = t_final / n_iter
+    x = np.arange(N, dtype=np.float64) / (N - 1)
+    rho = np.where(x < 0.5, 1.0, 0.125)
+    p = np.where(x < 0.5, 1.0, 0.1)
+    u = np.zeros((N, 3))
+    u[:, 0] = rho
+    u[:, 2] = p / (gamma - 1.0)
+    solver = dict(ndims=1, nvars=3, size=N, ghost=3, n_iter=n_iter, time_scheme="rk",
+                  time_scheme_type=rk, hyp_space_scheme=scheme, hyp_interp_type=interp,
+                  dt=dt, screen_op_iter=max(n_iter, 1), file_op_iter=n_iter * 10,
+                  op_file_format=op_file_format, ip_file_type="binary", op_overwrite="yes",
+                  model="euler1d")
+    boundary = [("extrapolate", 0, 1, 0, 0), ("extrapolate", 0, -1, 0, 0)]
+    physics = dict(gamma=gamma, upwinding=upwinding)
+    weno = dict(mapped=mapped, borges=0, yc=0, no_limiting=0, epsilon=1e-6)
+    return Deck(solver, boundary, physics, [x], u, weno)
+
+
+def ns2d_photochem(nx: int = 1024, ny: int | None = None, n_iter: int = 200, dt: float = 0.01,
+                   seed: int = 20261202, scheme: str = "crweno5", interp: str = "components",
+                   Re: float | None = 1000.0, amplitude: float = 1.0, dx: float | None = None,
+                   L: float = 10.0 * math.pi, beam_width: float = 0.1, pert: float = 1e-3,
+                   weights: str = "js", par: str = "4", rk: str = "44", z_mm: float = 0.0,
+                   upwinding: str = "roe", op_file_format: str = "none", with_imap: bool = True,
+                   include_chemistry: bool = True) -> Deck:
+    """2D Navier-Stokes + photochemistry on a periodic box.
+
+    Quiescent gas rho = 1 + pert*U(-1,1) (SplitMix64, `seed`), u = v = 0,
+    p = 1/gamma_chem; species O2 = 1, O3 = 0.01, others 0; Gaussian beam map
+    imap = amplitude*exp(-r^2/(2 (beam_width*L)^2)) centred; Re set through
+    Ptot (Re scales with Ptot when chemistry is on)."""
+    ny = nx if ny is None else ny
+    if dx is None:
+        dx = L / nx
+    Lx, Ly = nx * dx, ny * dx
+    x = np.arange(nx, dtype=np.float64) * dx
+    y = np.arange(ny, dtype=np.float64) * dx
+    _, _, gam = chemistry_reynolds(101325.0)
+    nz = int(math.ceil(z_mm * 0.001 * 20 / 0.03)) + 1 if include_chemistry else 0
+    nvars = 4 + 7 * nz if include_chemistry else 4
+    g = gam if include_chemistry else 1.4
+    npts = nx * ny
+    r = splitmix64_fast(seed, npts)
+    rho = 1.0 + pert * (2.0 * r - 1.0)
+    u = np.zeros((npts, nvars))
+    u[:, 0] = rho
+    u[:, 3] = (1.0 / g) / (g - 1.0)
+    if include_chemistry:
+        for iz in range(nz):
+            u[:, 4 + 7 * iz + 0] = 1.0
+            u[:, 4 + 7 * iz + 1] = 0.01
+    solver = dict(ndims=2, nvars=nvars, size=(nx, ny), ghost=3, n_iter=n_iter, time_scheme="rk",
+                  time_scheme_type=rk, hyp_space_scheme=scheme, hyp_interp_type=interp,
+                  par_space_scheme=par, dt=dt, screen_op_iter=max(n_iter, 1),
+                  file_op_iter=n_iter * 10, op_file_format=op_file_format, ip_file_type="binary",
+                  op_overwrite="yes", model="navierstokes2d")
+    boundary = [("periodic", 0, 1, 0, 0, -1e99, 1e99), ("periodic", 0, -1, 0, 0, -1e99, 1e99),
+                ("periodic", 1, 1, -1e99, 1e99, 0, 0), ("periodic", 1, -1, -1e99, 1e99, 0, 0)]
+    physics = dict(gamma=1.4, upwinding=upwinding, include_chemistry="yes" if include_chemistry else "no")
+    chem = None
+    if include_chemistry:
+        physics["Re"] = 1000.0 if Re is None else float(Re)
+        chem = dict(f_CO2=0.0, f_O3=0.01, z_mm=z_mm, nz=20,
+                    Ptot=ptot_for_reynolds(Re) if Re is not None else 101325.0)
+    elif Re is not None:
+        physics["Re"] = float(Re)
+        physics["Pr"] = 0.72
+    weno = dict(mapped=int(weights == "mapped"), borges=int(weights == "z"), yc=int(weights == "yc"),
+                no_limiting=int(weights == "none"), epsilon=1e-6)
+    imap = None
+    if include_chemistry and with_imap:
+        X, Y = np.meshgrid(x, y, indexing="xy")          # point-major, x fastest
+        r2 = (X - 0.5 * Lx) ** 2 + (Y - 0.5 * Ly) ** 2
+        w = beam_width * L
+        imap = (amplitude * np.exp(-r2 / (2.0 * w * w))).reshape(-1)
+    return Deck(solver, boundary, physics, [x, y], u, weno, chem, imap)
+
+
+def read_binary_op(path: str):
+    """Read a WriteBinary file (src/IOFunctions/WriteBinary.c:37-93):
+    int32 ndims, int32 nvars, int32 dim[ndims], f64 x[sum dim], f64 u[prod dim * nvars]."""
+    with open(path, "rb") as f:
+        ndims, nvars = np.fromfile(f, dtype=np.int32, count=2)
+        dims = np.fromfile(f, dtype=np.int32, count=int(ndims))
+        x = np.fromfile(f, dtype=np.float64, count=int(dims.sum()))
+        u = np.fromfile(f, dtype=np.float64, count=int(np.prod(dims)) * int(nvars))
+    return int(ndims), int(nvars), [int(d) for d in dims], x, u.reshape(-1, int(nvars))
