@@ -1,0 +1,143 @@
+/*
+ * piafs_amd.h -- C ABI of the MI355X-native PIAFS right-hand-side and
+ * explicit Runge-Kutta time-step path.
+ *
+ * Conventions: extern "C", opaque handles, plain pointers and sizes, int
+ * status (0 = ok, > 0 = input/configuration error, < 0 = device or numerical
+ * error).  No exceptions cross the ABI.  A handle is used by one host thread
+ * at a time; different handles are independent.  Host arrays use the
+ * reference's point-major layout without ghost points (x fastest, the nvars
+ * components of a point contiguous) -- the WriteBinary order of
+ * src/IOFunctions/WriteBinary.c:37-93.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the PIAFS source tree).
+ */
+#ifndef PIAFS_AMD_H
+#define PIAFS_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct piafs_case piafs_case;
+
+typedef struct {
+  int    ndims, nvars, ghosts;
+  int    dim[3];
+  int    n_iter, restart_iter, nstages;
+  double dt, t;          /* fixed step size and current simulation time */
+  double gamma, Re, Pr;  /* after the photochemistry overrides */
+  int    model;          /* 1 = euler1d, 2 = navierstokes2d */
+  int    scheme;         /* 0 = weno5, 1 = crweno5 */
+  int    characteristic; /* 1 = characteristic interpolation */
+  int    include_chem;
+  size_t device_bytes;   /* HBM held by this case */
+} piafs_info;
+
+/* ---------------------------------------------------------------------------
+ * Case lifecycle.
+ * Replaces the initialisation sequence of src/main.cpp:133-212
+ * (Simulation::define -> ReadInputs.c:185, Initialize.c:25,
+ * InitialSolution.c:24, InitializeBoundaries.c:35, InitializeSolvers.c:38,
+ * InitializePhysics.c:27 -> NavierStokes2DInitialize.c:68 /
+ * Euler1DInitialize.c:58 -> ChemistryInitialize.c:69).  Reads solver.inp,
+ * boundary.inp, physics.inp, weno.inp, chemistry.inp, initial.inp and
+ * imap.inp from `dir` (NULL = current directory) with the reference's
+ * keywords, defaults and selection strings, and uploads the state to
+ * device `device`. */
+int  piafs_create(const char *dir, int device, piafs_case **out);
+/* Replaces the cleanup of src/Simulation/Cleanup.c. */
+int  piafs_destroy(piafs_case *c);
+int  piafs_get_info(const piafs_case *c, piafs_info *info);
+/* Last error message for this case (or the last failed create). */
+const char *piafs_last_error(const piafs_case *c);
+
+/* State transfer in WriteBinary order (interior points only).
+ * Replaces ReadArray (src/IOFunctions/ReadArray.c:28) into HyPar::u and the
+ * gather of WriteArraySerial (src/IOFunctions/WriteArray.c:77). */
+int  piafs_upload_state(piafs_case *c, const double *u_host);
+int  piafs_download_state(piafs_case *c, double *u_host);
+int  piafs_set_time(piafs_case *c, double t);
+
+/* Advance nsteps time steps.  Replaces the body of the loop in
+ * src/Simulation/Solve.cpp:54-94: TimePreStep (TimePreStep.c:21) ->
+ * TimeStep -> TimeIntegration::TimeIntegrate = TimeRK (TimeRK.c:34) ->
+ * TimePostStep (TimePostStep.c:27).  Asynchronous on the case's stream. */
+int  piafs_step(piafs_case *c, int nsteps);
+/* Same, recording per-step max CFL (TimePreStep.c:54-69) and step norm
+ * (TimePostStep.c:36-64) into host arrays of length nsteps (either may be
+ * NULL).  Synchronous. */
+int  piafs_step_diag(piafs_case *c, int nsteps, double *cfl, double *norm);
+/* Wait for all queued work of this case. */
+int  piafs_sync(piafs_case *c);
+
+/* One right-hand-side evaluation F(u) at time t of an interior state given in
+ * WriteBinary order.  Replaces TimeIntegration::RHSFunction =
+ * TimeRHSFunctionExplicit (src/TimeIntegration/TimeRHSFunctionExplicit.c:29:
+ * ApplyBoundaryConditions, HyperbolicFunction, ParabolicFunction,
+ * SourceFunction). */
+int  piafs_rhs(piafs_case *c, const double *u_host, double t, double *rhs_host);
+
+/* Write the current solution in the reference's binary layout.  Replaces
+ * WriteBinary (src/IOFunctions/WriteBinary.c:37-93) as called from
+ * OutputSolution (src/Simulation/OutputSolution.cpp:25). */
+int  piafs_write_binary(piafs_case *c, const char *path);
+
+/* Whole solve as the reference's Solve (src/Simulation/Solve.cpp:29-123):
+ * n_iter steps from restart_iter, screen output every screen_op_iter, binary
+ * output files op[_NNNNN].bin as configured, final solution written.
+ * Returns the wall time of the time loop in *seconds (may be NULL). */
+int  piafs_solve(piafs_case *c, const char *dir, int verbose, double *seconds);
+
+/* Device timing of the dominant kernel: average duration (ms) of each
+ * kernel class over the last piafs_step call when profiling is enabled.
+ * names/ms arrays of length *n (in: capacity, out: count). */
+int  piafs_kernel_times(piafs_case *c, int enable, const char **names, double *ms, int *n);
+
+/* ---------------------------------------------------------------------------
+ * Per-operator entry points for unit parity (host arrays in, host arrays out;
+ * the device work runs on an internal stream and is synchronised).
+ */
+
+/* Batched tridiagonal solve, one lane per system.  Replaces tridiagLU
+ * (src/TridiagLU/tridiagLU.c:86-277, serial stages 1 and 4): a, b, c, x each
+ * [n][ns] (row-major over rows, systems contiguous, as tridiagLU.h:60-62);
+ * x is overwritten with the solution.  Returns -1 on a zero pivot. */
+int  piafs_tridiag_host(const double *a, const double *b, const double *c, double *x, int n, int ns);
+
+/* Nonlinear WENO5 weights for `count` stencils (five values each, ordered
+ * m3 m2 m1 p1 p2).  kind: 0 JS, 1 mapped, 2 borges (Z), 3 YC.  Replaces the
+ * weight macros of include/interpolation.h:255-490 as used by
+ * WENOFifthOrderCalculateWeights (src/InterpolationFunctions/
+ * WENOFifthOrderCalculateWeights.c:41-84).  w_out: count x 3. */
+int  piafs_weno_weights_host(int kind, const double *copt, const double *stencils, int count,
+                             double eps, double *w_out);
+
+/* ---------------------------------------------------------------------------
+ * Batched ensembles: many independent cases advanced together, one launch per
+ * kernel class per RK stage covering every case, the step captured in a
+ * HIP graph.  Replaces the reference's serial ensemble loop
+ * (src/Simulation/EnsembleSimulationsDefine.cpp:31-119, TimeRK.c:46-81).
+ * All member cases must share model, nvars, scheme family and RK tableau;
+ * sizes, dt, Re, beam map and WENO/CRWENO choice may differ per case.
+ */
+typedef struct piafs_batch piafs_batch;
+int  piafs_batch_create(const char *const *dirs, int ncases, int device, piafs_batch **out);
+int  piafs_batch_destroy(piafs_batch *b);
+int  piafs_batch_step(piafs_batch *b, int nsteps);
+int  piafs_batch_sync(piafs_batch *b);
+int  piafs_batch_download(piafs_batch *b, int icase, double *u_host);
+int  piafs_batch_info(piafs_batch *b, int icase, piafs_info *info);
+const char *piafs_batch_last_error(const piafs_batch *b);
+
+/* Device memory a case built from `dir` would hold (admission control for
+ * serving).  Returns 0 on success. */
+int  piafs_estimate_bytes(const char *dir, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,278 @@
+"""MI355X-native PIAFS right-hand-side / time-step path.
+
+Python host mirror of the C ABI in ``include/piafs_amd.h``.  The compute path
+is ``libpiafs_amd.so`` (hand-written HIP kernels for gfx950); this module only
+marshals arguments.  There is no CPU fallback: if the library is missing or no
+GPU is present, creating a case raises.
+
+The object model follows the reference driver (src/main.cpp:133-235): a
+:class:`Case` is created from a directory of PIAFS input decks (the
+reference's ``Simulation::define`` + ``ReadInputs`` ... ``InitializePhysics``),
+advanced with :meth:`Case.step` (``TimeStep`` = ``TimeRK``) or solved end to
+end with :meth:`Case.solve` (``Solve``), and written with
+:meth:`Case.write_binary` (``WriteBinary``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpiafs_amd.so")
+_lib = None
+
+
+class PiafsError(RuntimeError):
+    pass
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("ndims", ctypes.c_int), ("nvars", ctypes.c_int), ("ghosts", ctypes.c_int),
+                ("dim", ctypes.c_int * 3), ("n_iter", ctypes.c_int), ("restart_iter", ctypes.c_int),
+                ("nstages", ctypes.c_int), ("dt", ctypes.c_double), ("t", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("Re", ctypes.c_double), ("Pr", ctypes.c_double),
+                ("model", ctypes.c_int), ("scheme", ctypes.c_int), ("characteristic", ctypes.c_int),
+                ("include_chem", ctypes.c_int), ("device_bytes", ctypes.c_size_t)]
+
+
+EXPORTS = [
+    "piafs_create", "piafs_destroy", "piafs_get_info", "piafs_last_error", "piafs_upload_state",
+    "piafs_download_state", "piafs_set_time", "piafs_step", "piafs_step_diag", "piafs_sync", "piafs_rhs",
+    "piafs_write_binary", "piafs_solve", "piafs_kernel_times", "piafs_tridiag_host",
+    "piafs_weno_weights_host", "piafs_batch_create", "piafs_batch_destroy", "piafs_batch_step",
+    "piafs_batch_sync", "piafs_batch_download", "piafs_batch_info", "piafs_batch_last_error",
+    "piafs_estimate_bytes",
+]
+
+
+def lib():
+    """Load libpiafs_amd.so (built in place by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PiafsError(f"{LIB_PATH} not built; run __graft_entry__.build() (make -C piafs_amd)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip, cp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_char_p
+    L.piafs_create.argtypes = [cp, ip, ctypes.POINTER(vp)]
+    L.piafs_destroy.argtypes = [vp]
+    L.piafs_get_info.argtypes = [vp, ctypes.POINTER(Info)]
+    L.piafs_last_error.argtypes = [vp]
+    L.piafs_last_error.restype = cp
+    L.piafs_upload_state.argtypes = [vp, vp]
+    L.piafs_download_state.argtypes = [vp, vp]
+    L.piafs_set_time.argtypes = [vp, ctypes.c_double]
+    L.piafs_step.argtypes = [vp, ip]
+    L.piafs_step_diag.argtypes = [vp, ip, vp, vp]
+    L.piafs_sync.argtypes = [vp]
+    L.piafs_rhs.argtypes = [vp, vp, ctypes.c_double, vp]
+    L.piafs_write_binary.argtypes = [vp, cp]
+    L.piafs_solve.argtypes = [vp, cp, ip, dp]
+    L.piafs_kernel_times.argtypes = [vp, ip, ctypes.POINTER(cp), dp, ctypes.POINTER(ctypes.c_int)]
+    L.piafs_tridiag_host.argtypes = [vp, vp, vp, vp, ip, ip]
+    L.piafs_weno_weights_host.argtypes = [ip, vp, vp, ip, ctypes.c_double, vp]
+    L.piafs_batch_create.argtypes = [ctypes.POINTER(cp), ip, ip, ctypes.POINTER(vp)]
+    L.piafs_batch_destroy.argtypes = [vp]
+    L.piafs_batch_step.argtypes = [vp, ip]
+    L.piafs_batch_sync.argtypes = [vp]
+    L.piafs_batch_download.argtypes = [vp, ip, vp]
+    L.piafs_batch_info.argtypes = [vp, ip, ctypes.POINTER(Info)]
+    L.piafs_batch_last_error.argtypes = [vp]
+    L.piafs_batch_last_error.restype = cp
+    L.piafs_estimate_bytes.argtypes = [cp, ctypes.POINTER(ctypes.c_size_t)]
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Case:
+    """One PIAFS simulation resident on a GPU."""
+
+    def __init__(self, directory: str | None = None, device: int = 0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.piafs_create(directory.encode() if directory else None, int(device), ctypes.byref(h))
+        if rc:
+            raise PiafsError(f"piafs_create failed ({rc}): {L.piafs_last_error(None).decode()}")
+        self._h = h
+        self._L = L
+        self.directory = directory
+
+    # -- queries --
+    @property
+    def info(self) -> Info:
+        inf = Info()
+        self._check(self._L.piafs_get_info(self._h, ctypes.byref(inf)))
+        return inf
+
+    @property
+    def shape(self):
+        inf = self.info
+        n = 1
+        for d in range(inf.ndims):
+            n *= inf.dim[d]
+        return (n, inf.nvars)
+
+    @property
+    def time(self) -> float:
+        return self.info.t
+
+    def _check(self, rc):
+        if rc:
+            raise PiafsError(f"piafs error {rc}: {self._L.piafs_last_error(self._h).decode()}")
+
+    # -- state --
+    def solution(self) -> np.ndarray:
+        out = np.empty(self.shape)
+        self._check(self._L.piafs_download_state(self._h, _ptr(out)))
+        return out
+
+    def set_solution(self, u) -> None:
+        u = _f64(u).reshape(self.shape)
+        self._check(self._L.piafs_upload_state(self._h, _ptr(u)))
+
+    def set_time(self, t: float) -> None:
+        self._check(self._L.piafs_set_time(self._h, float(t)))
+
+    # -- time stepping --
+    def step(self, n: int = 1) -> None:
+        self._check(self._L.piafs_step(self._h, int(n)))
+
+    def step_diag(self, n: int = 1):
+        cfl = np.zeros(n)
+        norm = np.zeros(n)
+        self._check(self._L.piafs_step_diag(self._h, int(n), _ptr(cfl), _ptr(norm)))
+        return cfl, norm
+
+    def sync(self) -> None:
+        self._check(self._L.piafs_sync(self._h))
+
+    def rhs(self, u, t: float) -> np.ndarray:
+        u = _f64(u).reshape(self.shape)
+        out = np.empty(self.shape)
+        self._check(self._L.piafs_rhs(self._h, _ptr(u), float(t), _ptr(out)))
+        return out
+
+    def write_binary(self, path: str) -> None:
+        self._check(self._L.piafs_write_binary(self._h, path.encode()))
+
+    def solve(self, directory: str | None = None, verbose: bool = False) -> float:
+        secs = ctypes.c_double(0)
+        d = directory if directory is not None else self.directory
+        self._check(self._L.piafs_solve(self._h, d.encode() if d else None, int(verbose), ctypes.byref(secs)))
+        return secs.value
+
+    def profile(self, enable: bool) -> None:
+        self._L.piafs_kernel_times(self._h, 1 if enable else 0, None, None, None)
+
+    def kernel_times(self) -> dict:
+        cap = 16
+        names = (ctypes.c_char_p * cap)()
+        ms = (ctypes.c_double * cap)()
+        n = ctypes.c_int(cap)
+        self._check(self._L.piafs_kernel_times(self._h, -1, names, ms, ctypes.byref(n)))
+        return {names[k].decode(): ms[k] for k in range(n.value)}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.piafs_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent cases advanced together (piafs_batch_* in the C ABI)."""
+
+    def __init__(self, directories, device: int = 0):
+        L = lib()
+        arr = (ctypes.c_char_p * len(directories))(*[d.encode() for d in directories])
+        h = ctypes.c_void_p()
+        rc = L.piafs_batch_create(arr, len(directories), int(device), ctypes.byref(h))
+        if rc:
+            raise PiafsError(f"piafs_batch_create failed ({rc})")
+        self._h, self._L, self.n = h, L, len(directories)
+
+    def step(self, n: int = 1):
+        rc = self._L.piafs_batch_step(self._h, int(n))
+        if rc:
+            raise PiafsError(f"batch step failed ({rc}): {self._L.piafs_batch_last_error(self._h).decode()}")
+
+    def sync(self):
+        rc = self._L.piafs_batch_sync(self._h)
+        if rc:
+            raise PiafsError(f"batch sync failed ({rc}): {self._L.piafs_batch_last_error(self._h).decode()}")
+
+    def info(self, k: int) -> Info:
+        inf = Info()
+        self._L.piafs_batch_info(self._h, int(k), ctypes.byref(inf))
+        return inf
+
+    def solution(self, k: int) -> np.ndarray:
+        inf = self.info(k)
+        n = 1
+        for d in range(inf.ndims):
+            n *= inf.dim[d]
+        out = np.empty((n, inf.nvars))
+        rc = self._L.piafs_batch_download(self._h, int(k), _ptr(out))
+        if rc:
+            raise PiafsError(f"batch download failed ({rc})")
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.piafs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tridiag(a, b, c, x):
+    """Batched tridiagonal solve on the GPU; arrays shaped [n, ns] (tridiagLU layout)."""
+    a, b, c, x = (_f64(t).copy() for t in (a, b, c, x))
+    n, ns = x.shape
+    rc = lib().piafs_tridiag_host(_ptr(a), _ptr(b), _ptr(c), _ptr(x), n, ns)
+    if rc:
+        raise PiafsError(f"tridiag failed ({rc})")
+    return x
+
+
+def weno_weights(kind: int, copt, stencils, eps: float = 1e-6) -> np.ndarray:
+    st = _f64(stencils).reshape(-1, 5)
+    co = _f64(copt).reshape(3)
+    w = np.empty((st.shape[0], 3))
+    rc = lib().piafs_weno_weights_host(int(kind), _ptr(co), _ptr(st), st.shape[0], float(eps), _ptr(w))
+    if rc:
+        raise PiafsError(f"weno_weights failed ({rc})")
+    return w
+
+
+def estimate_bytes(directory: str) -> int:
+    n = ctypes.c_size_t(0)
+    rc = lib().piafs_estimate_bytes(directory.encode(), ctypes.byref(n))
+    if rc:
+        raise PiafsError(f"estimate_bytes failed ({rc})")
+    return int(n.value)

@@ -241,3 +241,77 @@ def read_binary_op(path: str):
         x = np.fromfile(f, dtype=np.float64, count=int(dims.sum()))
         u = np.fromfile(f, dtype=np.float64, count=int(np.prod(dims)) * int(nvars))
     return int(ndims), int(nvars), [int(d) for d in dims], x, u.reshape(-1, int(nvars))
+
+
+def isentropic_vortex(N: int = 60, n_iter: int = 400, dt: float = 0.05, scheme: str = "crweno5",
+                      rk: str = "ssprk3", mapped: int = 1, gamma: float = 1.4) -> Deck:
+    """Isentropic vortex convected by u_inf = 0.5 across the periodic box
+    [0,10)^2 (strength b = u_inf, centre (5,5)); after t = 20 it returns to its
+    initial position, so the exact solution equals the initial one.  Same
+    problem as Examples/2D_NavierStokes/InviscidVortexConvection (whose shipped
+    errors.dat was produced with n_iter 400, dt 0.05); inputs written in binary
+    so the grid is exact."""
+    L = 10.0
+    dx = L / N
+    x = np.arange(N) * dx
+    y = np.arange(N) * dx
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    rx, ry = X - 5.0, Y - 5.0
+    rx = np.where(rx < -5, rx + 10, np.where(rx > 5, rx - 10, rx))
+    rsq = rx * rx + ry * ry
+    b, pi = 0.5, 4.0 * math.atan(1.0)
+    rho = np.exp((1.0 / (gamma - 1.0)) * np.log(1.0 - ((gamma - 1.0) * b * b) / (8.0 * gamma * pi * pi) * np.exp(1.0 - rsq)))
+    P = np.exp(gamma * np.log(rho))
+    du = -b / (2.0 * pi) * np.exp(0.5 * (1.0 - rsq)) * ry
+    dv = b / (2.0 * pi) * np.exp(0.5 * (1.0 - rsq)) * rx
+    uu, vv = 0.5 + du, 0.0 + dv
+    u = np.stack([rho, rho * uu, rho * vv, P / (gamma - 1.0) + 0.5 * rho * (uu * uu + vv * vv)], axis=-1)
+    u = u.reshape(-1, 4)
+    solver = dict(ndims=2, nvars=4, size=(N, N), ghost=3, n_iter=n_iter, time_scheme="rk",
+                  time_scheme_type=rk, hyp_space_scheme=scheme, hyp_interp_type="components",
+                  par_space_scheme="4", dt=dt, screen_op_iter=n_iter, file_op_iter=n_iter * 10,
+                  op_file_format="none", ip_file_type="binary", op_overwrite="yes", model="navierstokes2d")
+    boundary = [("periodic", 0, 1, 0, 0, 0, 10.0), ("periodic", 0, -1, 0, 0, 0, 10.0),
+                ("periodic", 1, 1, 0, 10.0, 0, 0), ("periodic", 1, -1, 0, 10.0, 0, 0)]
+    physics = dict(gamma=gamma, upwinding="roe")
+    weno = dict(mapped=mapped, borges=0, yc=0, no_limiting=0, epsilon=1e-6)
+    return Deck(solver, boundary, physics, [x, y], u, weno)
+
+
+def density_sine_1d(N: int = 128, n_iter: int = 50, dt: float = 0.002, scheme: str = "crweno5",
+                    interp: str = "components", rk: str = "44", gamma: float = 1.4, amp: float = 0.1) -> Deck:
+    """Smooth 1D density wave rho = 1 + amp sin(2 pi x), u = 1, p = 1/gamma on a
+    periodic [0,1) grid."""
+    x = np.arange(N, dtype=np.float64) / N
+    rho = 1.0 + amp * np.sin(2 * math.pi * x)
+    u = np.zeros((N, 3))
+    u[:, 0] = rho
+    u[:, 1] = rho * 1.0
+    u[:, 2] = (1.0 / gamma) / (gamma - 1.0) + 0.5 * rho
+    solver = dict(ndims=1, nvars=3, size=N, ghost=3, n_iter=n_iter, time_scheme="rk",
+                  time_scheme_type=rk, hyp_space_scheme=scheme, hyp_interp_type=interp,
+                  dt=dt, screen_op_iter=n_iter, file_op_iter=n_iter * 10, op_file_format="none",
+                  ip_file_type="binary", op_overwrite="yes", model="euler1d")
+    boundary = [("periodic", 0, 1, 0, 0), ("periodic", 0, -1, 0, 0)]
+    physics = dict(gamma=gamma, upwinding="roe")
+    weno = dict(mapped=0, borges=0, yc=0, no_limiting=0, epsilon=1e-6)
+    return Deck(solver, boundary, physics, [x], u, weno)
+
+
+def error_norms(u: np.ndarray, uex: np.ndarray):
+    """L1/L2/Linf of u - uex relative to the norms of uex, aggregated over all
+    variables as src/HyParFunctions/CalculateError.c:67-116 does."""
+    npts = u.shape[0]
+    d = u - uex
+    n = (np.abs(uex).sum() / npts, math.sqrt((uex * uex).sum() / npts), np.abs(uex).max())
+    e = [np.abs(d).sum() / npts, math.sqrt((d * d).sum() / npts), np.abs(d).max()]
+    if min(n) > 1e-15:
+        e = [e[k] / n[k] for k in range(3)]
+    return tuple(e)
+
+
+def rel_l2_per_var(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Per-variable relative L2 difference ||a-b|| / ||b|| (absolute where ||b|| = 0)."""
+    num = np.sqrt(((a - b) ** 2).sum(axis=0))
+    den = np.sqrt((b ** 2).sum(axis=0))
+    return np.where(den > 0, num / np.where(den > 0, den, 1.0), num)

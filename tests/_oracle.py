@@ -1,0 +1,112 @@
+"""ctypes binding of oracle/liboracle.so -- the CPU restatement used as the
+checker.  Test infrastructure only."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, capture_output=True)
+        L = ctypes.CDLL(LIB)
+        vp = ctypes.c_void_p
+        L.orc_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.orc_free.argtypes = [vp]
+        for f in ("orc_ndims", "orc_nvars", "orc_ghosts", "orc_n_iter"):
+            getattr(L, f).argtypes = [vp]
+        L.orc_dim.argtypes = [vp, ctypes.c_int]
+        L.orc_time.argtypes = [vp]
+        L.orc_time.restype = ctypes.c_double
+        L.orc_param.argtypes = [vp, ctypes.c_char_p]
+        L.orc_param.restype = ctypes.c_double
+        L.orc_step.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.orc_get_solution.argtypes = [vp, vp]
+        L.orc_set_solution.argtypes = [vp, vp]
+        L.orc_rhs.argtypes = [vp, vp, ctypes.c_double, vp]
+        L.orc_tridiag.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int]
+        L.orc_weno_weights.argtypes = [ctypes.c_int] + [ctypes.c_double] * 9 + [vp]
+        L.orc_fd4_line.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OCase:
+    def __init__(self, d):
+        L = lib()
+        self.h = ctypes.c_void_p()
+        rc = L.orc_load(d.encode(), ctypes.byref(self.h))
+        if rc:
+            raise RuntimeError(f"oracle load failed: {rc}")
+        self.L = L
+        nd = L.orc_ndims(self.h)
+        n = 1
+        for k in range(nd):
+            n *= L.orc_dim(self.h, k)
+        self.shape = (n, L.orc_nvars(self.h))
+
+    def step(self, n, diag=False):
+        if diag:
+            cfl, norm = np.zeros(n), np.zeros(n)
+            self.L.orc_step(self.h, n, _p(cfl), _p(norm))
+            return cfl, norm
+        self.L.orc_step(self.h, n, None, None)
+
+    def solution(self):
+        out = np.zeros(self.shape)
+        self.L.orc_get_solution(self.h, _p(out))
+        return out
+
+    def set_solution(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.shape)
+        self.L.orc_set_solution(self.h, _p(u))
+
+    def rhs(self, u, t):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.shape)
+        out = np.zeros(self.shape)
+        self.L.orc_rhs(self.h, _p(u), float(t), _p(out))
+        return out
+
+    def param(self, name):
+        return self.L.orc_param(self.h, name.encode())
+
+    @property
+    def time(self):
+        return self.L.orc_time(self.h)
+
+    def __del__(self):
+        try:
+            self.L.orc_free(self.h)
+        except Exception:
+            pass
+
+
+def tridiag(a, b, c, x):
+    a, b, c, x = (np.ascontiguousarray(t, dtype=np.float64).copy() for t in (a, b, c, x))
+    n, ns = x.shape
+    rc = lib().orc_tridiag(_p(a), _p(b), _p(c), _p(x), n, ns)
+    return rc, x
+
+
+def weno_weights(kind, copt, st, eps=1e-6):
+    w = np.zeros(3)
+    lib().orc_weno_weights(int(kind), *[float(c) for c in copt], *[float(s) for s in st], float(eps), _p(w))
+    return w
+
+
+def fd4_line(f, N, g):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    df = np.zeros_like(f)
+    lib().orc_fd4_line(_p(f), _p(df), N, g)
+    return df

@@ -1,0 +1,137 @@
+"""Generate the golden fixtures under tests/golden/ (run in the build container).
+
+Two kinds of fixtures:
+
+1. Reference runs: small decks built by ``piafs_amd.decks`` (deterministic,
+   SplitMix64 seeds) are run through the reference PIAFS binary compiled from
+   source by ``oracle/ref.mk`` (``oracle/_ref/PIAFS``); the final solution
+   (WriteBinary layout) is stored in ``ref_runs.npz`` with the deck builder and
+   its arguments in ``ref_runs.json``.  Tests rebuild the same deck, run the
+   CPU restatement and the MI355X path, and compare.
+
+2. Data the reference tree itself ships: the input decks and the per-iteration
+   CFL / norm log rows of Examples/1D_Euler/{SodShockTube,ShuOsherProblem,
+   LaxShockTube} and the error norms of
+   Examples/2D_NavierStokes/InviscidVortexConvection/errors.dat.
+
+Usage: python tests/golden/make_golden.py [--config2-full RUN_DIR]
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from piafs_amd import decks  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "PIAFS")
+REF_TREE = "/root/reference"
+
+# name -> (builder, kwargs); every deck writes op.bin at the end
+CASES = {
+    "sod_char_roe_ssprk3_N1024": ("sod_1d", dict(N=1024)),
+    "sod_char_roe_N128": ("sod_1d", dict(N=128, n_iter=40)),
+    "sod_char_mapped_N96": ("sod_1d", dict(N=96, n_iter=30, mapped=1)),
+    "sod_rfchar_mapped_rk44_N101": ("sod_1d", dict(N=101, n_iter=30, upwinding="rf-char", mapped=1, rk="44")),
+    "sine_crweno_comp_N128": ("density_sine_1d", dict(N=128, n_iter=50)),
+    "sine_weno_comp_N100": ("density_sine_1d", dict(N=100, n_iter=40, scheme="weno5", rk="ssprk3")),
+    "ns2d_chem_crweno_js_24x20": ("ns2d_photochem", dict(nx=24, ny=20, n_iter=10)),
+    "ns2d_chem_crweno_48x40": ("ns2d_photochem", dict(nx=48, ny=40, n_iter=20)),
+    "ns2d_chem_weno_z_16x24": ("ns2d_photochem", dict(nx=16, ny=24, n_iter=10, scheme="weno5", weights="z")),
+    "ns2d_chem_par2_yc_16x16": ("ns2d_photochem", dict(nx=16, ny=16, n_iter=5, par="2", weights="yc")),
+    "ns2d_chem_mapped_nolimit_16x16": ("ns2d_photochem", dict(nx=16, ny=16, n_iter=5, weights="none")),
+    "ns2d_chem_pulse_24x20": ("ns2d_photochem", dict(nx=24, ny=20, n_iter=60)),
+    "ns2d_chem_2layer_16x16": ("ns2d_photochem", dict(nx=16, ny=16, n_iter=5, z_mm=1.5)),
+    "ns2d_nochem_mapped_20x16": ("ns2d_photochem", dict(nx=20, ny=16, n_iter=8, include_chemistry=False,
+                                                        weights="mapped")),
+    "ns2d_chem_weno_mixed_cfg3_17x31": ("ns2d_photochem", dict(nx=17, ny=31, n_iter=6, scheme="weno5",
+                                                               amplitude=1.7, Re=300.0, dx=10.0 * np.pi / 1024 * 8)),
+    "vortex_crweno_mapped_60_400": ("isentropic_vortex", dict(N=60, n_iter=400, dt=0.05)),
+}
+
+
+def build(name):
+    fn, kw = CASES[name]
+    dk = getattr(decks, fn)(**kw)
+    dk.solver["op_file_format"] = "binary"
+    dk.solver["op_overwrite"] = "yes"
+    return dk
+
+
+def run_reference(d):
+    r = subprocess.run([REF_BIN], cwd=d, capture_output=True, text=True)
+    if r.returncode:
+        raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+    return decks.read_binary_op(os.path.join(d, "op.bin"))[4]
+
+
+def ref_runs():
+    out, meta = {}, {}
+    for name in CASES:
+        with tempfile.TemporaryDirectory() as d:
+            build(name).write(d)
+            u = run_reference(d)
+        out[name] = u
+        meta[name] = {"builder": CASES[name][0], "kwargs": {k: (float(v) if isinstance(v, np.floating) else v)
+                                                            for k, v in CASES[name][1].items()}}
+        print(f"{name:40s} {u.shape} max|u|={np.abs(u).max():.4e}")
+    np.savez_compressed(os.path.join(HERE, "ref_runs.npz"), **out)
+    with open(os.path.join(HERE, "ref_runs.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+def example_logs():
+    """Copy the 1D example decks and their CFL / norm rows (output.log)."""
+    meta = {}
+    for ex in ("SodShockTube", "ShuOsherProblem", "LaxShockTube"):
+        src = os.path.join(REF_TREE, "Examples", "1D_Euler", ex)
+        dst = os.path.join(HERE, "examples", ex)
+        os.makedirs(dst, exist_ok=True)
+        for fn in ("solver.inp", "boundary.inp", "physics.inp", "weno.inp", "initial.inp"):
+            shutil.copy(os.path.join(src, fn), os.path.join(dst, fn))
+        rows = []
+        for line in open(os.path.join(src, "output.log")):
+            m = re.match(r"\s*Iteration:\s+(\d+)\s+Time:\s+(\S+)\s+Max CFL:\s+(\S+).*Norm:\s+(\S+)", line)
+            if m:
+                rows.append([int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(4))])
+        meta[ex] = {"rows": rows, "source": f"Examples/1D_Euler/{ex}/output.log",
+                    "ranks": 2 if ex == "LaxShockTube" else 1}
+    # vortex errors.dat: n, n, ?, ?, dt, L1, L2, Linf, solver time, total time
+    vals = open(os.path.join(REF_TREE, "Examples", "2D_NavierStokes", "InviscidVortexConvection",
+                             "errors.dat")).read().split()
+    meta["InviscidVortexConvection"] = {"dt": float(vals[4]), "errors": [float(v) for v in vals[5:8]],
+                                        "source": "Examples/2D_NavierStokes/InviscidVortexConvection/errors.dat",
+                                        "note": "shipped run used 2x2 MPI ranks (Jacobi reduced solve)"}
+    with open(os.path.join(HERE, "examples.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+def config2_full(run_dir):
+    """Reduce the 1024^2 / 200-step reference run to a small fixture: per
+    variable norms, row and column sums, and a stride-32 subsample."""
+    _, nv, dims, x, u = decks.read_binary_op(os.path.join(run_dir, "op.bin"))
+    nx, ny = dims
+    U = u.reshape(ny, nx, nv)
+    np.savez_compressed(
+        os.path.join(HERE, "config2_1024_ref.npz"),
+        l1=np.abs(U).sum(axis=(0, 1)), l2=np.sqrt((U * U).sum(axis=(0, 1))), linf=np.abs(U).max(axis=(0, 1)),
+        colsum=U.sum(axis=0), rowsum=U.sum(axis=1), sub=U[::32, ::32, :].copy(),
+    )
+    print("config2 fixture written", U.shape)
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 2 and sys.argv[1] == "--config2-full":
+        config2_full(sys.argv[2])
+    else:
+        ref_runs()
+        example_logs()

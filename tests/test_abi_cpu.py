@@ -1,0 +1,85 @@
+"""C-ABI library checks that need no GPU: it loads, exports every entry point
+declared in include/piafs_amd.h, and its input-deck reader accepts the
+reference's decks and rejects what the reference rejects (parsing runs on the
+host only; no device call is made here)."""
+import ctypes
+import os
+import re
+import tempfile
+
+import pytest
+
+from conftest import GOLDEN, ROOT
+import piafs_amd
+from piafs_amd import decks
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "piafs_amd.h")).read()
+    return sorted(set(re.findall(r"\b(piafs_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = piafs_amd.lib()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(piafs_amd.EXPORTS)
+
+
+def test_header_is_plain_c():
+    txt = open(os.path.join(ROOT, "include", "piafs_amd.h")).read()
+    assert 'extern "C"' in txt
+    for bad in ("torch", "std::", "hipStream_t", "namespace", "template"):
+        assert bad not in txt
+
+
+@pytest.mark.parametrize("builder,kw", [
+    ("sod_1d", dict(N=64)),
+    ("ns2d_photochem", dict(nx=32, ny=16, n_iter=2)),
+    ("ns2d_photochem", dict(nx=16, ny=16, z_mm=1.5)),
+    ("isentropic_vortex", dict(N=20)),
+])
+def test_deck_parses_and_sizes(builder, kw):
+    dk = getattr(decks, builder)(**kw)
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        n = piafs_amd.estimate_bytes(d)
+    pts = 1
+    for s in dk.dims:
+        pts *= s + 6
+    assert n >= 8 * pts * dk.nvars * 4
+
+
+@pytest.mark.parametrize("example", ["SodShockTube", "ShuOsherProblem", "LaxShockTube"])
+def test_reference_example_decks_parse(example):
+    assert piafs_amd.estimate_bytes(os.path.join(GOLDEN, "examples", example)) > 0
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda dk: dk.solver.__setitem__("hyp_space_scheme", "muscl3"), "not a supported spatial interpolation"),
+    (lambda dk: dk.solver.__setitem__("nvars", 7), "nvars has to be 11"),
+    (lambda dk: dk.solver.__setitem__("time_scheme_type", "rk5"), "not a supported multi-stage"),
+    (lambda dk: dk.physics.__setitem__("upwinding", "llf-char"), "not a valid upwinding"),
+    (lambda dk: dk.solver.__setitem__("par_space_scheme", "3"), "parabolic"),
+    (lambda dk: dk.boundary.__setitem__(0, ("slip-wall", 0, 1, 0, 0, 0, 0)), "not supported"),
+])
+def test_deck_errors_are_reported(mutate, msg):
+    dk = decks.ns2d_photochem(nx=16, ny=16, n_iter=1)
+    mutate(dk)
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        with pytest.raises(piafs_amd.PiafsError):
+            piafs_amd.estimate_bytes(d)
+        err = piafs_amd.lib().piafs_last_error(None).decode()
+    assert msg in err, err
+
+
+def test_missing_initial_solution_is_an_error():
+    dk = decks.sod_1d(N=32)
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        os.remove(os.path.join(d, "initial.inp"))
+        with pytest.raises(piafs_amd.PiafsError):
+            piafs_amd.estimate_bytes(d)

@@ -1,0 +1,179 @@
+"""Parity of the HIP path (through the C ABI) with the CPU restatement and the
+reference fixtures.  Tolerances: FP64 relative L2 per variable <= 1e-11 after
+the stated step counts; single RHS evaluations and
+operator calls <= 1e-12 (differences come only from FMA contraction)."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import _oracle as orc
+from conftest import GOLDEN, ROOT, build_deck
+import piafs_amd
+from piafs_amd import decks
+
+pytestmark = pytest.mark.gpu
+
+TOL_STEPS = 1e-11
+TOL_RHS = 1e-12
+
+
+def _rel(a, b):
+    """per-variable relative L2, guarded by the variable's scale"""
+    num = np.sqrt(((a - b) ** 2).sum(axis=0))
+    den = np.sqrt((b ** 2).sum(axis=0))
+    scale = np.maximum(den, 1e-300)
+    return np.where(den > 0, num / scale, num)
+
+
+def test_tridiag_matches_oracle():
+    rng = np.random.default_rng(11)
+    n, ns = 257, 300
+    a = rng.uniform(-1, 1, (n, ns)); c = rng.uniform(-1, 1, (n, ns))
+    b = 2.5 + rng.uniform(0, 1, (n, ns)); x = rng.uniform(-1, 1, (n, ns))
+    a[0] = 0; c[-1] = 0
+    rc, ref = orc.tridiag(a, b, c, x)
+    got = piafs_amd.tridiag(a, b, c, x)
+    np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_weno_weights_match_oracle(kind):
+    rng = np.random.default_rng(kind)
+    st = rng.uniform(-1, 1, (500, 5))
+    st[:100] = np.cumsum(rng.uniform(0, 0.01, (100, 5)), axis=1)  # smooth
+    for copt in ((0.1, 0.6, 0.3), (0.2, 0.5, 0.3)):
+        got = piafs_amd.weno_weights(kind, copt, st)
+        ref = np.array([orc.weno_weights(kind, copt, s) for s in st])
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-15)
+
+
+RHS_DECKS = {
+    "sod_char": lambda: decks.sod_1d(N=64, n_iter=1),
+    "sod_rfchar": lambda: decks.sod_1d(N=64, n_iter=1, upwinding="rf-char", mapped=1),
+    "sine_crweno": lambda: decks.density_sine_1d(N=64),
+    "ns2d_chem_crweno": lambda: decks.ns2d_photochem(nx=32, ny=24, n_iter=1),
+    "ns2d_chem_weno_z": lambda: decks.ns2d_photochem(nx=24, ny=32, scheme="weno5", weights="z"),
+    "ns2d_chem_yc_par2": lambda: decks.ns2d_photochem(nx=20, ny=20, weights="yc", par="2"),
+    "ns2d_chem_mapped": lambda: decks.ns2d_photochem(nx=20, ny=20, weights="mapped"),
+    "ns2d_chem_2layer": lambda: decks.ns2d_photochem(nx=16, ny=20, z_mm=1.5),
+    "ns2d_nochem": lambda: decks.ns2d_photochem(nx=20, ny=16, include_chemistry=False),
+    "vortex": lambda: decks.isentropic_vortex(N=32),
+}
+
+
+@pytest.mark.parametrize("name", sorted(RHS_DECKS))
+def test_rhs_matches_oracle(name):
+    dk = RHS_DECKS[name]()
+    rng = np.random.default_rng(3)
+    u = dk.u * (1.0 + 1e-3 * rng.uniform(-1, 1, dk.u.shape))
+    t = 0.45  # inside the photolysis pulse
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        ref = orc.OCase(d).rhs(u, t)
+        with piafs_amd.Case(d) as c:
+            got = c.rhs(u, t)
+    scale = np.abs(ref).max(axis=0) + 1e-300
+    err = (np.abs(got - ref).max(axis=0) / scale)
+    assert np.all(err <= TOL_RHS), err
+
+
+FIXTURES = [
+    "sod_char_roe_N128", "sod_char_mapped_N96", "sod_rfchar_mapped_rk44_N101", "sine_crweno_comp_N128",
+    "sine_weno_comp_N100", "ns2d_chem_crweno_js_24x20", "ns2d_chem_weno_z_16x24", "ns2d_chem_par2_yc_16x16",
+    "ns2d_chem_mapped_nolimit_16x16", "ns2d_chem_pulse_24x20", "ns2d_chem_2layer_16x16",
+    "ns2d_nochem_mapped_20x16", "ns2d_chem_weno_mixed_cfg3_17x31", "ns2d_chem_crweno_48x40",
+    "sod_char_roe_ssprk3_N1024",
+]
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_steps_match_reference_fixture(ref_runs, name):
+    data, meta = ref_runs
+    dk = build_deck(meta[name])
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        with piafs_amd.Case(d) as c:
+            c.step(int(dk.solver["n_iter"]))
+            got = c.solution()
+            t = c.time
+    ref = data[name]
+    err = _rel(got, ref)
+    assert np.all(err <= TOL_STEPS), err
+    assert abs(t - dk.solver["dt"] * dk.solver["n_iter"]) < 1e-12
+
+
+def _close_in_last_digit(ours, printed, fmt):
+    a = float(fmt % ours)
+    ulp = 10.0 ** (np.floor(np.log10(abs(printed))) - (3 if "3E" in fmt else 4))
+    return abs(a - printed) <= ulp * 1.01
+
+
+@pytest.mark.parametrize("example", ["SodShockTube", "ShuOsherProblem", "LaxShockTube"])
+def test_shipped_logs(examples, example):
+    rows = examples[example]["rows"]
+    with piafs_amd.Case(os.path.join(GOLDEN, "examples", example)) as c:
+        cfl, norm = c.step_diag(rows[-1][0])
+    for it, t, cf, nm in rows:
+        assert _close_in_last_digit(cfl[it - 1], cf, "%1.3E"), (it, cfl[it - 1], cf)
+        assert _close_in_last_digit(norm[it - 1], nm, "%1.4E"), (it, norm[it - 1], nm)
+
+
+def test_vortex_errors_match_errors_dat(examples):
+    ref = examples["InviscidVortexConvection"]["errors"]
+    dk = decks.isentropic_vortex(N=60, n_iter=400, dt=0.05)
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        with piafs_amd.Case(d) as c:
+            c.step(400)
+            u = c.solution()
+    np.testing.assert_allclose(decks.error_norms(u, dk.u), ref, rtol=1e-8)
+
+
+def test_write_binary_layout_and_cli(ref_runs):
+    data, meta = ref_runs
+    name = "ns2d_chem_crweno_js_24x20"
+    dk = build_deck(meta[name])
+    dk.solver["op_file_format"] = "binary"
+    dk.solver["op_overwrite"] = "yes"
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        cli = os.path.join(ROOT, "piafs_amd", "bin", "piafs_amd")
+        r = subprocess.run([cli], cwd=d, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        nd, nv, dims, x, u = decks.read_binary_op(os.path.join(d, "op.bin"))
+    assert (nd, nv, dims) == (2, 11, [24, 20])
+    np.testing.assert_array_equal(x, np.concatenate(dk.x))
+    assert np.all(_rel(u, data[name]) <= TOL_STEPS)
+
+
+def test_config1_full_size(ref_runs):
+    """1D Sod, N = 1024, WENO5 characteristic + Roe, SSPRK3, 901 steps to t = 0.2."""
+    data, meta = ref_runs
+    dk = build_deck(meta["sod_char_roe_ssprk3_N1024"])
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        with piafs_amd.Case(d) as c:
+            c.step(dk.solver["n_iter"])
+            got = c.solution()
+    assert np.all(_rel(got, data["sod_char_roe_ssprk3_N1024"]) <= TOL_STEPS)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "config2_1024_ref.npz")),
+                    reason="config-2 reference fixture not generated")
+def test_config2_full_size():
+    """2D NS + photochemistry, 1024^2, CRWENO5, RK4, 200 steps vs the reference run."""
+    z = np.load(os.path.join(GOLDEN, "config2_1024_ref.npz"))
+    dk = decks.ns2d_photochem(nx=1024, n_iter=200)
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        with piafs_amd.Case(d) as c:
+            c.step(200)
+            u = c.solution()
+    U = u.reshape(1024, 1024, -1)
+    np.testing.assert_allclose(np.sqrt((U * U).sum(axis=(0, 1))), z["l2"], rtol=1e-12)
+    assert np.all(_rel(U[::32, ::32, :].reshape(-1, U.shape[2]), z["sub"].reshape(-1, U.shape[2])) <= TOL_STEPS)
+    assert np.all(_rel(U.sum(axis=0), z["colsum"]) <= TOL_STEPS)
+    assert np.all(_rel(U.sum(axis=1), z["rowsum"]) <= TOL_STEPS)

@@ -1,0 +1,167 @@
+"""Pin the CPU restatement (oracle/) before trusting it as the checker.
+
+* bit-for-bit against reference runs (tests/golden/ref_runs.npz, produced by
+  the reference binary built from source; see tests/golden/make_golden.py);
+* against the fixtures the reference tree ships: per-iteration CFL and norm of
+  Examples/1D_Euler/{SodShockTube,ShuOsherProblem,LaxShockTube}/output.log and
+  the error norms of Examples/2D_NavierStokes/InviscidVortexConvection/errors.dat;
+* against closed forms: Thomas vs a dense solve, WENO weights on linear data,
+  fourth-order derivatives of quartics, free-stream preservation and design
+  order of accuracy of the no-limiting schemes.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import _oracle as orc
+from conftest import GOLDEN, build_deck
+from piafs_amd import decks
+
+
+def _run_oracle(dk, n=None):
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        c = orc.OCase(d)
+        c.step(int(dk.solver["n_iter"]) if n is None else n)
+        return c.solution()
+
+
+@pytest.mark.parametrize("name", [
+    "sod_char_roe_N128", "sod_char_mapped_N96", "sod_rfchar_mapped_rk44_N101", "sine_crweno_comp_N128",
+    "sine_weno_comp_N100", "ns2d_chem_crweno_js_24x20", "ns2d_chem_weno_z_16x24", "ns2d_chem_par2_yc_16x16",
+    "ns2d_chem_mapped_nolimit_16x16", "ns2d_chem_pulse_24x20", "ns2d_chem_2layer_16x16",
+    "ns2d_nochem_mapped_20x16", "ns2d_chem_weno_mixed_cfg3_17x31", "ns2d_chem_crweno_48x40",
+    "sod_char_roe_ssprk3_N1024",
+])
+def test_oracle_matches_reference_bitwise(ref_runs, name):
+    data, meta = ref_runs
+    u = _run_oracle(build_deck(meta[name]))
+    ref = data[name]
+    assert u.shape == ref.shape
+    assert np.array_equal(u, ref), f"max diff {np.abs(u - ref).max()}"
+
+
+@pytest.mark.parametrize("example", ["SodShockTube", "ShuOsherProblem", "LaxShockTube"])
+def test_oracle_matches_shipped_logs(examples, example):
+    """CFL (%1.3E) and step norm (%1.4E) rows printed by the reference run."""
+    rows = examples[example]["rows"]
+    d = os.path.join(GOLDEN, "examples", example)
+    c = orc.OCase(d)
+    last = rows[-1][0]
+    cfl, norm = c.step(last, diag=True)
+    for it, t, cf, nm in rows:
+        assert float("%1.3E" % cfl[it - 1]) == cf, (it, cfl[it - 1], cf)
+        assert float("%1.4E" % norm[it - 1]) == nm, (it, norm[it - 1], nm)
+
+
+def test_oracle_vortex_errors_match_errors_dat(examples):
+    """CRWENO5 + tridiagonal + SSPRK3 + Roe + mapped weights, 60^2, t = 20."""
+    ref = examples["InviscidVortexConvection"]["errors"]
+    dk = decks.isentropic_vortex(N=60, n_iter=400, dt=0.05)
+    u = _run_oracle(dk)
+    e = decks.error_norms(u, dk.u)
+    # the shipped run used 2x2 MPI ranks with an approximate (Jacobi) reduced
+    # solve, the serial oracle an exact one: agreement to ~10 digits
+    np.testing.assert_allclose(e, ref, rtol=1e-8)
+
+
+def test_tridiag_matches_dense_solve():
+    rng = np.random.default_rng(7)
+    n, ns = 37, 9
+    a = rng.uniform(-1, 1, (n, ns))
+    c = rng.uniform(-1, 1, (n, ns))
+    b = 3.0 + rng.uniform(0, 1, (n, ns))
+    a[0] = 0.0
+    c[-1] = 0.0
+    x = rng.uniform(-1, 1, (n, ns))
+    rc, sol = orc.tridiag(a, b, c, x)
+    assert rc == 0
+    for s in range(ns):
+        M = np.diag(b[:, s]) + np.diag(a[1:, s], -1) + np.diag(c[:-1, s], 1)
+        np.testing.assert_allclose(sol[:, s], np.linalg.solve(M, x[:, s]), rtol=1e-12, atol=1e-13)
+
+
+def test_tridiag_zero_pivot_flag():
+    a = np.zeros((3, 1)); b = np.ones((3, 1)); c = np.zeros((3, 1)); x = np.ones((3, 1))
+    b[0, 0] = 0.0
+    rc, _ = orc.tridiag(a, b, c, x)
+    assert rc == -1
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+@pytest.mark.parametrize("copt", [(0.1, 0.6, 0.3), (0.2, 0.5, 0.3)])
+def test_weights_optimal_on_linear_data(kind, copt):
+    """All three smoothness indicators equal h^2 on linear data, so every
+    weight formulation returns the optimal weights."""
+    h = 0.37
+    st = [1.0 + h * k for k in (-2, -1, 0, 1, 2)]
+    w = orc.weno_weights(kind, copt, st)
+    np.testing.assert_allclose(w, copt, rtol=1e-12)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_weights_avoid_discontinuity(kind):
+    """A jump between m1 and p1 drives the weights of the stencils crossing it
+    to ~0 (the left-biased stencil 1 stays smooth)."""
+    st = [1.0, 1.0, 1.0, 0.0, 0.0]
+    w = orc.weno_weights(kind, (0.1, 0.6, 0.3), st)
+    assert abs(w.sum() - 1.0) < 1e-14
+    assert w[0] > 0.99
+
+
+def test_fd4_exact_on_quartic():
+    N, g = 20, 3
+    x = np.arange(-g, N + g, dtype=np.float64) * 0.1
+    f = 3 * x ** 4 - 2 * x ** 3 + x - 5
+    df = orc.fd4_line(f, N, g) / 0.1
+    np.testing.assert_allclose(df, 12 * x ** 3 - 6 * x ** 2 + 1, rtol=1e-9, atol=1e-9)
+
+
+def test_free_stream_preserved():
+    """A uniform moving state is an exact steady solution: RHS == 0."""
+    dk = decks.ns2d_photochem(nx=12, ny=10, include_chemistry=False, Re=500.0)
+    dk.u[:, 0] = 1.2
+    dk.u[:, 1] = 1.2 * 0.3
+    dk.u[:, 2] = 1.2 * -0.2
+    dk.u[:, 3] = 2.0
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        c = orc.OCase(d)
+        r = c.rhs(dk.u, 0.0)
+    assert np.abs(r).max() < 1e-13
+
+
+@pytest.mark.parametrize("scheme", ["weno5", "crweno5"])
+def test_linear_schemes_fifth_order(scheme):
+    """With no_limiting the reconstructions are the linear fifth-order schemes:
+    the advection RHS of a density wave converges at order ~5 in the middle half
+    of the line (CRWENO's lines end in explicit WENO5 rows even on a periodic
+    grid, which is fourth order at the two ends and decays geometrically)."""
+    errs = []
+    for N in (32, 64):
+        dk = decks.density_sine_1d(N=N, n_iter=1, scheme=scheme, amp=1e-3)
+        dk.weno["no_limiting"] = 1
+        with tempfile.TemporaryDirectory() as d:
+            dk.write(d)
+            r = orc.OCase(d).rhs(dk.u, 0.0)
+        x = dk.x[0]
+        exact = -1.0 * 1e-3 * 2 * np.pi * np.cos(2 * np.pi * x)  # -(rho u)_x with u = 1
+        errs.append(np.abs(r[N // 4:3 * N // 4, 0] - exact[N // 4:3 * N // 4]).max())
+    order = np.log2(errs[0] / errs[1])
+    assert order > 4.5, (errs, order)
+
+
+def test_chemistry_constants_from_formulas():
+    """Re and Pr of the config-2 deck follow the reference-scale formulas
+    (ChemistryInitialize.c:289-314, NavierStokes2DInitialize.c:204-210)."""
+    dk = decks.ns2d_photochem(nx=8, ny=8, n_iter=1)
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        c = orc.OCase(d)
+        Re, Pr, gam = decks.chemistry_reynolds(dk.chemistry["Ptot"])
+        np.testing.assert_allclose(c.param("Re"), Re, rtol=1e-12)
+        np.testing.assert_allclose(c.param("Re"), 1000.0, rtol=1e-12)
+        np.testing.assert_allclose(c.param("Pr"), Pr, rtol=1e-12)
+        np.testing.assert_allclose(c.param("gamma"), 918.45 / 650.0, rtol=1e-15)
