@@ -26,6 +26,10 @@ struct Case {
   ReconParams R{};
   int device = -1;
   hipStream_t stream = nullptr;
+  // second stream for the y-sweep and viscous derivatives (joined by events)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool dual = false;
   double t = 0.0;
   std::string err;
   size_t bytes = 0;
@@ -37,6 +41,8 @@ struct Case {
   double *scr_b = nullptr, *scr_c = nullptr, *hx = nullptr, *hxT = nullptr;
   double *Qd = nullptr, *FV = nullptr, *dxinv = nullptr, *imap = nullptr;
   double *partials = nullptr, *uprev = nullptr;
+  // y-sweep buffers when the sweeps run concurrently
+  double *flux_y = nullptr, *ifcy[4] = {nullptr, nullptr, nullptr, nullptr}, *scr_by = nullptr, *scr_cy = nullptr;
   int nparts = 0;
 
   // profiling: events bracketing kernel classes
@@ -51,7 +57,7 @@ struct Case {
   double *dalloc(size_t n);
   int upload(const double *uh);
   int download(double *uh, const double *src = nullptr);
-  void mark(int cls, bool begin);
+  void mark(int cls, bool begin, hipStream_t st = nullptr);
   void rhs_from_stage(double time, double *out);
   void enqueue_step();
   int step(int n, double *cfl, double *norm);

@@ -242,7 +242,42 @@ __global__ void __launch_bounds__(TB) k_flux(SweepGeom S, int nv, double gamma, 
 //    substitution).  The forward sweep keeps b' and c per row in scratch and
 //    x' in the output array; the backward sweep overwrites it with x.
 // ---------------------------------------------------------------------------
+// Coefficients of row k of one line system: A, B, C and the right-hand side
+// (for WENO5 the right-hand side is the interface value itself).
 template <int CR, int KIND>
+__device__ __forceinline__ void row_coeffs(int k, int N, bool left, const ReconParams &R, double w0, double w1,
+                                           double w2, double w3, double w4, double w5, double &Ak, double &Bk,
+                                           double &Ck, double &Rk) {
+  double m3, m2, m1, p1, p2;
+  if (left) { m3 = w0; m2 = w1; m1 = w2; p1 = w3; p2 = w4; }
+  else { m3 = w5; m2 = w4; m1 = w3; p1 = w2; p2 = w1; }
+  const bool bnd = (k == 0) || (k == N);
+  double c1, c2, c3;
+  if (CR && !bnd) { c1 = 0.2; c2 = 0.5; c3 = 0.3; }
+  else { c1 = 0.1; c2 = 0.6; c3 = 0.3; }
+  double a1, a2, a3;
+  if (R.no_limiting) { a1 = c1; a2 = c2; a3 = c3; }
+  else weno_w<KIND>(c1, c2, c3, m3, m2, m1, p1, p2, R.eps, a1, a2, a3);
+  double f1, f2, f3;
+  if (!CR || bnd) {
+    weno5_cands(m3, m2, m1, p1, p2, f1, f2, f3);
+    Ak = 0.0; Bk = 1.0; Ck = 0.0;
+  } else {
+    const double t3 = 1.0 / 3.0;
+    crweno5_cands(m2, m1, p1, f1, f2, f3);
+    const double lo = (2 * t3) * a1 + (t3)*a2;
+    Bk = (t3)*a1 + (2 * t3) * a2 + (2 * t3) * a3;
+    const double hi = (t3)*a3;
+    if (left) { Ak = lo; Ck = hi; } else { Ak = hi; Ck = lo; }
+  }
+  Rk = a1 * f1 + a2 * f2 + a3 * f3;
+}
+
+// Rows are processed in blocks of BLK: the stencil values feeding the next
+// block and, in the backward sweep, the next block's (b', c, x') are loaded
+// one block ahead, so a memory latency is paid once per block and the
+// block's independent weight evaluations overlap.
+template <int CR, int KIND, int BLK>
 __global__ void __launch_bounds__(TB) k_recon_comp(SweepGeom S, int nv, ReconParams R,
                                                    const double *__restrict__ q, const double *__restrict__ f,
                                                    double *__restrict__ uL, double *__restrict__ uR,
@@ -258,6 +293,7 @@ __global__ void __launch_bounds__(TB) k_recon_comp(SweepGeom S, int nv, ReconPar
   const double *src = ((r < 2) ? q : f) + (long)v * S.plane + S.line_off + l;
   const long ps = S.pos_stride;
   const int N = S.N, g = S.g;
+  const int last = N + g - 1;  // last position held by a line (ghosts included)
   double *out = ((r == 0) ? uL : (r == 1) ? uR : (r == 2) ? fL : fR) + (long)v * (N + 1) * nl + l;
   double *sb = nullptr, *sc = nullptr;
   if (CR) {
@@ -265,62 +301,184 @@ __global__ void __launch_bounds__(TB) k_recon_comp(SweepGeom S, int nv, ReconPar
     sb = scr_b + so;
     sc = scr_c + so;
   }
-  // window w0..w5 = values at positions k-3 .. k+2
-  double w0 = src[(long)(g - 3) * ps], w1 = src[(long)(g - 2) * ps], w2 = src[(long)(g - 1) * ps];
-  double w3 = src[(long)g * ps], w4 = src[(long)(g + 1) * ps], w5 = src[(long)(g + 2) * ps];
+  auto ld = [&](int pos) -> double { return (pos <= last) ? src[(long)(pos + g) * ps] : 0.0; };
+  // window w0..w5 = positions k-3 .. k+2; cur[j] = position k0+3+j feeds row k0+j
+  double w0 = ld(-3), w1 = ld(-2), w2 = ld(-1), w3 = ld(0), w4 = ld(1), w5 = ld(2);
+  double cur[BLK];
+#pragma unroll
+  for (int j = 0; j < BLK; j++) cur[j] = ld(3 + j);
   double bprev = 1.0, cprev = 0.0, xprev = 0.0;
-  const double t3 = 1.0 / 3.0;
-  for (int k = 0; k <= N; k++) {
-    double m3, m2, m1, p1, p2;
-    if (left) { m3 = w0; m2 = w1; m1 = w2; p1 = w3; p2 = w4; }
-    else { m3 = w5; m2 = w4; m1 = w3; p1 = w2; p2 = w1; }
-    const bool bnd = (k == 0) || (k == N);
-    double c1, c2, c3;
-    if (CR && !bnd) { c1 = 0.2; c2 = 0.5; c3 = 0.3; }
-    else { c1 = 0.1; c2 = 0.6; c3 = 0.3; }
-    double a1, a2, a3;
-    if (R.no_limiting) { a1 = c1; a2 = c2; a3 = c3; }
-    else weno_w<KIND>(c1, c2, c3, m3, m2, m1, p1, p2, R.eps, a1, a2, a3);
-    double f1, f2, f3;
-    if (!CR) {
-      weno5_cands(m3, m2, m1, p1, p2, f1, f2, f3);
-      out[(long)k * nl] = a1 * f1 + a2 * f2 + a3 * f3;
-    } else {
-      double Ak, Bk, Ck;
-      if (bnd) {
-        weno5_cands(m3, m2, m1, p1, p2, f1, f2, f3);
-        Ak = 0.0; Bk = 1.0; Ck = 0.0;
-      } else {
-        crweno5_cands(m2, m1, p1, f1, f2, f3);
-        const double lo = (2 * t3) * a1 + (t3)*a2;
-        Bk = (t3)*a1 + (2 * t3) * a2 + (2 * t3) * a3;
-        const double hi = (t3)*a3;
-        if (left) { Ak = lo; Ck = hi; } else { Ak = hi; Ck = lo; }
-      }
-      const double Rk = a1 * f1 + a2 * f2 + a3 * f3;
-      double bk, xk;
-      if (k == 0) { bk = Bk; xk = Rk; }
-      else {
-        const double fac = Ak / bprev;
-        bk = Bk - fac * cprev;
-        xk = Rk - fac * xprev;
-      }
-      sb[(long)k * nl] = bk;
-      sc[(long)k * nl] = Ck;
-      out[(long)k * nl] = xk;
-      bprev = bk; cprev = Ck; xprev = xk;
+  for (int k0 = 0; k0 <= N; k0 += BLK) {
+    double nxt[BLK];
+#pragma unroll
+    for (int j = 0; j < BLK; j++) nxt[j] = ld(k0 + BLK + 3 + j);
+    double Ab[BLK], Bb[BLK], Cb[BLK], Rb[BLK];
+#pragma unroll
+    for (int j = 0; j < BLK; j++) {
+      row_coeffs<CR, KIND>(k0 + j, N, left, R, w0, w1, w2, w3, w4, w5, Ab[j], Bb[j], Cb[j], Rb[j]);
+      w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = cur[j];
     }
-    w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5;
-    if (k < N) w5 = src[(long)(k + 3 + g) * ps];
+#pragma unroll
+    for (int j = 0; j < BLK; j++) {
+      const int k = k0 + j;
+      if (k > N) break;
+      if (!CR) {
+        out[(long)k * nl] = Rb[j];
+      } else {
+        double bk, xk;
+        if (k == 0) { bk = Bb[j]; xk = Rb[j]; }
+        else {
+          const double fac = Ab[j] / bprev;
+          bk = Bb[j] - fac * cprev;
+          xk = Rb[j] - fac * xprev;
+        }
+        sb[(long)k * nl] = bk;
+        sc[(long)k * nl] = Cb[j];
+        out[(long)k * nl] = xk;
+        bprev = bk; cprev = Cb[j]; xprev = xk;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < BLK; j++) cur[j] = nxt[j];
   }
   if (CR) {
-    double xn = out[(long)N * nl] / sb[(long)N * nl];
+    double xn = xprev / bprev;  // row N: c_N = 0
     out[(long)N * nl] = xn;
-    for (int k = N - 1; k >= 0; k--) {
-      const double xk = (out[(long)k * nl] - sc[(long)k * nl] * xn) / sb[(long)k * nl];
+    double bb[BLK], cc[BLK], xx[BLK];
+#pragma unroll
+    for (int j = 0; j < BLK; j++) {
+      const int k = N - 1 - j;
+      bb[j] = k >= 0 ? sb[(long)k * nl] : 1.0;
+      cc[j] = k >= 0 ? sc[(long)k * nl] : 0.0;
+      xx[j] = k >= 0 ? out[(long)k * nl] : 0.0;
+    }
+    for (int k0 = N - 1; k0 >= 0; k0 -= BLK) {
+      double nb[BLK], nc[BLK], nx[BLK];
+#pragma unroll
+      for (int j = 0; j < BLK; j++) {
+        const int k = k0 - BLK - j;
+        nb[j] = k >= 0 ? sb[(long)k * nl] : 1.0;
+        nc[j] = k >= 0 ? sc[(long)k * nl] : 0.0;
+        nx[j] = k >= 0 ? out[(long)k * nl] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < BLK; j++) {
+        const int k = k0 - j;
+        if (k < 0) break;
+        const double xk = (xx[j] - cc[j] * xn) / bb[j];
+        out[(long)k * nl] = xk;
+        xn = xk;
+      }
+#pragma unroll
+      for (int j = 0; j < BLK; j++) { bb[j] = nb[j]; cc[j] = nc[j]; xx[j] = nx[j]; }
+    }
+  }
+}
+
+// CRWENO5 reconstruction as a producer/consumer workgroup: 64 systems (64
+// consecutive lines of one variable and reconstruction) per workgroup of four
+// waves.  Waves 1-3 evaluate the row coefficients (weights, A, B, C, R) of a
+// chunk of KR rows into one half of a double-buffered LDS ring while wave 0
+// runs the Thomas forward recurrence (tridiagLU.c:133-150) over the previous
+// chunk; wave 0 then back-substitutes (:243-255) with a one-block-ahead
+// prefetch.  Per system the arithmetic and its order are those of
+// k_recon_comp<1, KIND>.
+template <int KIND, int KR>
+__global__ void __launch_bounds__(256) k_crweno_pc(SweepGeom S, int nv, ReconParams R,
+                                                   const double *__restrict__ q, const double *__restrict__ f,
+                                                   double *__restrict__ uL, double *__restrict__ uR,
+                                                   double *__restrict__ fL, double *__restrict__ fR,
+                                                   double *__restrict__ scr_b, double *__restrict__ scr_c) {
+  __shared__ double buf[2][KR][4][64];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int nl = S.nlines, N = S.N, g = S.g;
+  const int l0 = blockIdx.x * 64;
+  const int v = blockIdx.y % nv, r = blockIdx.y / nv;
+  const bool left = !(r & 1);
+  const double *src = ((r < 2) ? q : f) + (long)v * S.plane + S.line_off + l0;
+  const long ps = S.pos_stride;
+  const int nchunks = (N + 1 + KR - 1) / KR;
+  const int my_l = l0 + lane;
+  const bool lane_ok = my_l < nl;
+  double *out = ((r == 0) ? uL : (r == 1) ? uR : (r == 2) ? fL : fR) + (long)v * (N + 1) * nl + my_l;
+  const long so = (long)(r * nv + v) * (N + 1) * nl + my_l;
+  double *sb = scr_b + so, *sc = scr_c + so;
+  double bprev = 1.0, cprev = 0.0, xprev = 0.0;
+  for (int c = 0; c <= nchunks; c++) {
+    if (wave > 0 && c < nchunks) {
+      const int half = c & 1;
+      for (int it = tid - 64; it < KR * 64; it += 192) {
+        const int kr = it >> 6, ln = it & 63;
+        const int k = c * KR + kr;
+        if (k <= N && l0 + ln < nl) {
+          const double *sp = src + ln;
+          const double w0 = sp[(long)(k - 3 + g) * ps], w1 = sp[(long)(k - 2 + g) * ps];
+          const double w2 = sp[(long)(k - 1 + g) * ps], w3 = sp[(long)(k + g) * ps];
+          const double w4 = sp[(long)(k + 1 + g) * ps], w5 = sp[(long)(k + 2 + g) * ps];
+          double A, B, C, Rk;
+          row_coeffs<1, KIND>(k, N, left, R, w0, w1, w2, w3, w4, w5, A, B, C, Rk);
+          buf[half][kr][0][ln] = A;
+          buf[half][kr][1][ln] = B;
+          buf[half][kr][2][ln] = C;
+          buf[half][kr][3][ln] = Rk;
+        }
+      }
+    }
+    if (wave == 0 && c > 0 && lane_ok) {
+      const int cc = c - 1, half = cc & 1;
+#pragma unroll
+      for (int kr = 0; kr < KR; kr++) {
+        const int k = cc * KR + kr;
+        if (k > N) break;
+        const double A = buf[half][kr][0][lane], B = buf[half][kr][1][lane];
+        const double C = buf[half][kr][2][lane], Rk = buf[half][kr][3][lane];
+        double bk, xk;
+        if (k == 0) { bk = B; xk = Rk; }
+        else {
+          const double fac = A / bprev;
+          bk = B - fac * cprev;
+          xk = Rk - fac * xprev;
+        }
+        sb[(long)k * nl] = bk;
+        sc[(long)k * nl] = C;
+        out[(long)k * nl] = xk;
+        bprev = bk; cprev = C; xprev = xk;
+      }
+    }
+    __syncthreads();
+  }
+  if (wave != 0 || !lane_ok) return;
+  constexpr int BLK = 8;
+  double xn = xprev / bprev;  // row N: c_N = 0
+  out[(long)N * nl] = xn;
+  double bb[BLK], cc[BLK], xx[BLK];
+#pragma unroll
+  for (int j = 0; j < BLK; j++) {
+    const int k = N - 1 - j;
+    bb[j] = k >= 0 ? sb[(long)k * nl] : 1.0;
+    cc[j] = k >= 0 ? sc[(long)k * nl] : 0.0;
+    xx[j] = k >= 0 ? out[(long)k * nl] : 0.0;
+  }
+  for (int k0 = N - 1; k0 >= 0; k0 -= BLK) {
+    double nb[BLK], nc[BLK], nx[BLK];
+#pragma unroll
+    for (int j = 0; j < BLK; j++) {
+      const int k = k0 - BLK - j;
+      nb[j] = k >= 0 ? sb[(long)k * nl] : 1.0;
+      nc[j] = k >= 0 ? sc[(long)k * nl] : 0.0;
+      nx[j] = k >= 0 ? out[(long)k * nl] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < BLK; j++) {
+      const int k = k0 - j;
+      if (k < 0) break;
+      const double xk = (xx[j] - cc[j] * xn) / bb[j];
       out[(long)k * nl] = xk;
       xn = xk;
     }
+#pragma unroll
+    for (int j = 0; j < BLK; j++) { bb[j] = nb[j]; cc[j] = nc[j]; xx[j] = nx[j]; }
   }
 }
 
@@ -1030,9 +1188,16 @@ void launch_flux(const Grid &G, const SweepGeom &S, int model, int dir, double g
 void launch_recon_components(const Grid &G, const SweepGeom &S, const ReconParams &R, const double *q,
                              const double *f, double *uL, double *uR, double *fL, double *fR, double *scr_b,
                              double *scr_c, hipStream_t s) {
+  if (R.crweno && R.kind < 0) {  // producer/consumer variant: kept for measurement, not dispatched
+    const dim3 gr((S.nlines + 63) / 64, G.nv * 4), bl(256);
+#define PC(K) hipLaunchKernelGGL((k_crweno_pc<K, 8>), gr, bl, 0, s, S, G.nv, R, q, f, uL, uR, fL, fR, scr_b, scr_c)
+    switch (R.kind) { case 1: PC(1); break; case 2: PC(2); break; case 3: PC(3); break; default: PC(0); }
+#undef PC
+    return;
+  }
   const long n = (long)S.nlines * G.nv * 4;
   const dim3 gr(nblk(n)), bl(TB);
-#define RC(CR, K) hipLaunchKernelGGL((k_recon_comp<CR, K>), gr, bl, 0, s, S, G.nv, R, q, f, uL, uR, fL, fR, scr_b, scr_c)
+#define RC(CR, K) hipLaunchKernelGGL((k_recon_comp<CR, K, 8>), gr, bl, 0, s, S, G.nv, R, q, f, uL, uR, fL, fR, scr_b, scr_c)
   if (R.crweno) {
     switch (R.kind) { case 1: RC(1, 1); break; case 2: RC(1, 2); break; case 3: RC(1, 3); break; default: RC(1, 0); }
   } else {

@@ -59,6 +59,10 @@ size_t Case::plan_bytes(const Config &c) {
   }
   n += 5 * iface;
   if (c.crweno) n += 8 * iface;
+  if (c.ndims == 2) {  // y-sweep buffers of the second stream
+    n += nv * plane + 4 * iface;
+    if (c.crweno) n += 8 * iface;
+  }
   size_t nint = (size_t)c.npts_int;
   n += 2 * nv * nint;                      // hx, hxT
   if (c.model_id == MODEL_NS2D) n += 12 * (size_t)g.nxg * g.nyg;
@@ -162,6 +166,19 @@ int Case::init(const std::string &dir, int dev) {
     scr_b = dalloc(4 * iface);
     scr_c = dalloc(4 * iface);
   }
+  if (cfg.ndims == 2) {
+    dual = true;
+    HIPCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    const size_t ify = (size_t)nv * (Sy.N + 1) * Sy.nlines;
+    flux_y = dalloc(st);
+    for (int k = 0; k < 4; k++) ifcy[k] = dalloc(ify);
+    if (cfg.crweno) {
+      scr_by = dalloc(4 * ify);
+      scr_cy = dalloc(4 * ify);
+    }
+  }
   const size_t nint = (size_t)G.nx * G.ny;
   hx = dalloc((size_t)nv * nint);
   if (cfg.ndims == 2) hxT = dalloc((size_t)nv * nint);
@@ -198,6 +215,9 @@ Case::~Case() {
     hipEventDestroy(e.second);
   }
   for (void *p : allocs) hipFree(p);
+  if (ev_fork) hipEventDestroy(ev_fork);
+  if (ev_join) hipEventDestroy(ev_join);
+  if (stream2) hipStreamDestroy(stream2);
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -228,8 +248,9 @@ int Case::download(double *uh, const double *src) {
   return 0;
 }
 
-void Case::mark(int cls, bool begin) {
+void Case::mark(int cls, bool begin, hipStream_t st) {
   if (!profiling) return;
+  if (!st) st = stream;
   if (begin) {
     if (ev_used >= ev.size()) {
       hipEvent_t a, b;
@@ -239,9 +260,9 @@ void Case::mark(int cls, bool begin) {
       ev_cls.push_back(cls);
     }
     ev_cls[ev_used] = cls;
-    hipEventRecord(ev[ev_used].first, stream);
+    hipEventRecord(ev[ev_used].first, st);
   } else {
-    hipEventRecord(ev[ev_used].second, stream);
+    hipEventRecord(ev[ev_used].second, st);
     ev_used++;
   }
 }
@@ -250,7 +271,36 @@ void Case::mark(int cls, bool begin) {
 // out: TimeRHSFunctionExplicit.c:29-77.
 void Case::rhs_from_stage(double time, double *out) {
   const int model = cfg.model_id;
-  // ---- x direction ----
+  hipStream_t sy = dual ? stream2 : stream;
+  double *fy = dual ? flux_y : flux;
+  double *const *iy = dual ? ifcy : ifc;
+  double *sby = dual ? scr_by : scr_b, *scy = dual ? scr_cy : scr_c;
+  if (dual) {
+    hipEventRecord(ev_fork, stream);
+    hipStreamWaitEvent(stream2, ev_fork, 0);
+  }
+  if (cfg.ndims == 2) {
+    // ---- y direction (natural layout) and viscous derivatives ----
+    mark(1, true, sy);
+    launch_flux(G, Sy, model, 1, cfg.gamma, U, fy, sy);
+    mark(1, false, sy);
+    mark(2, true, sy);
+    launch_recon_components(G, Sy, R, U, fy, iy[0], iy[1], iy[2], iy[3], sby, scy, sy);
+    mark(2, false, sy);
+    mark(3, true, sy);
+    launch_upwind(G, Sy, model, cfg.upwind, 1, cfg.gamma, U, iy[0], iy[1], iy[2], iy[3], fIy, sy);
+    mark(3, false, sy);
+    if (P.viscous) {
+      mark(6, true, sy);
+      launch_qderiv(G, P, U, dxinv, Qd, sy);
+      mark(6, false, sy);
+      mark(7, true, sy);
+      launch_viscflux(G, P, U, Qd, FV, sy);
+      mark(7, false, sy);
+    }
+    if (dual) hipEventRecord(ev_join, stream2);
+  }
+  // ---- x direction (transposed copy in 2D) ----
   const double *qx = (cfg.ndims == 2) ? UT : U;
   mark(1, true);
   launch_flux(G, Sx, model, 0, cfg.gamma, qx, flux, stream);
@@ -271,26 +321,8 @@ void Case::rhs_from_stage(double time, double *out) {
     mark(5, true);
     launch_transpose(cfg.nvars, G.nx, G.ny, hxT, hx, stream);
     mark(5, false);
-    // ---- y direction ----
-    mark(1, true);
-    launch_flux(G, Sy, model, 1, cfg.gamma, U, flux, stream);
-    mark(1, false);
-    mark(2, true);
-    launch_recon_components(G, Sy, R, U, flux, ifc[0], ifc[1], ifc[2], ifc[3], scr_b, scr_c, stream);
-    mark(2, false);
-    mark(3, true);
-    launch_upwind(G, Sy, model, cfg.upwind, 1, cfg.gamma, U, ifc[0], ifc[1], ifc[2], ifc[3], fIy, stream);
-    mark(3, false);
-    // ---- parabolic ----
-    if (P.viscous) {
-      mark(6, true);
-      launch_qderiv(G, P, U, dxinv, Qd, stream);
-      mark(6, false);
-      mark(7, true);
-      launch_viscflux(G, P, U, Qd, FV, stream);
-      mark(7, false);
-    }
   }
+  if (dual) hipStreamWaitEvent(stream, ev_join, 0);
   // ---- source time factor (ChemistrySolve.c:36-43), evaluated on the host ----
   double hs = 0.0;
   if (cfg.include_chem) {
