@@ -37,17 +37,18 @@ STEP_BYTES_PER_CELL = lambda nv, ns: 40 * nv * ns  # noqa: E731
 
 def kernel_algorithmic_bytes(name, nx, ny, nv, g, nstages, crweno):
     """Compulsory bytes of one launch of a kernel class (its inputs read once
-    and outputs written once), per the kernel table in DESIGN.md.  Sweeps
-    alternate x and y; the figures below are for one sweep of an nx x ny grid
-    (the two directions have the same counts on a square grid)."""
+    and outputs written once).  One launch covers both sweeps (x and y) of an
+    nx x ny grid; per sweep the counts below are for `lines` lines of N
+    interior points (the two sweeps have the same counts on a square grid)."""
     lines, N = ny, nx
     pos = N + 2 * g
+    sweeps = 2
     if name == "recon":   # u and f at every position of every line; uL uR fL fR
-        return 8 * (2 * nv * pos * lines + 4 * nv * (N + 1) * lines)
+        return sweeps * 8 * (2 * nv * pos * lines + 4 * nv * (N + 1) * lines)
     if name == "upwind":  # uL uR fL fR + 4 flow vars at the two cells; fI
-        return 8 * (4 * nv * (N + 1) * lines + 2 * 4 * (N + 1) * lines + nv * (N + 1) * lines)
+        return sweeps * 8 * (4 * nv * (N + 1) * lines + 2 * 4 * (N + 1) * lines + nv * (N + 1) * lines)
     if name == "flux":
-        return 8 * 2 * nv * pos * lines
+        return sweeps * 8 * 2 * nv * pos * lines
     if name == "stage_bc":  # u + up to one Udot read, U and UT written (ghosted)
         return 8 * nv * (nx + 2 * g) * (ny + 2 * g) * 4
     return None
@@ -132,8 +133,8 @@ def main():
         if ktimes:
             # per-class average launch time (ms) x launches per step -> share of the step
             ns = inf.nstages
-            per_step = {"stage_bc": ns, "flux": 2 * ns, "recon": 2 * ns, "upwind": 2 * ns, "divergence": ns,
-                        "transpose": ns, "qderiv": ns, "viscflux": ns, "assemble": ns, "update": 1}
+            per_step = {"stage_bc": ns, "flux": ns, "recon": ns, "upwind": ns, "qderiv": ns, "viscflux": ns,
+                        "assemble": ns, "update": 1}
             share = {k: v * per_step.get(k, 1) for k, v in ktimes.items()}
             dom = max(share.items(), key=lambda kv: kv[1])[0]
             byts = kernel_algorithmic_bytes(dom, nx, nx, nv, g, inf.nstages, True)

@@ -121,16 +121,29 @@ int  piafs_weno_weights_host(int kind, const double *copt, const double *stencil
  * kernel class per RK stage covering every case, the step captured in a
  * HIP graph.  Replaces the reference's serial ensemble loop
  * (src/Simulation/EnsembleSimulationsDefine.cpp:31-119, TimeRK.c:46-81).
- * All member cases must share model, nvars, scheme family and RK tableau;
- * sizes, dt, Re, beam map and WENO/CRWENO choice may differ per case.
+ * Every launch block belongs to one member case and reads that case's own
+ * descriptor, so members may differ in model, grid size, scheme, weights, RK
+ * tableau, dt, Re and beam map.
  */
 typedef struct piafs_batch piafs_batch;
+/* Cases are addressed by slot numbers (0..ncases-1 after create).  On a
+ * failed create *out still receives a handle whose last_error names the
+ * failing case; destroy it. */
 int  piafs_batch_create(const char *const *dirs, int ncases, int device, piafs_batch **out);
 int  piafs_batch_destroy(piafs_batch *b);
+/* Advance every member case by nsteps (asynchronous; graph replay by default). */
 int  piafs_batch_step(piafs_batch *b, int nsteps);
 int  piafs_batch_sync(piafs_batch *b);
-int  piafs_batch_download(piafs_batch *b, int icase, double *u_host);
-int  piafs_batch_info(piafs_batch *b, int icase, piafs_info *info);
+/* Replay steps from a captured HIP graph (default) or launch them eagerly. */
+int  piafs_batch_use_graph(piafs_batch *b, int on);
+/* Continuous batching: admit a case between steps (its state is uploaded and
+ * stays resident) and release a finished one.  The launches of the next step
+ * cover the new member set. */
+int  piafs_batch_add(piafs_batch *b, const char *dir, int *slot);
+int  piafs_batch_remove(piafs_batch *b, int slot);
+int  piafs_batch_steps_done(piafs_batch *b, int slot, int *steps);
+int  piafs_batch_download(piafs_batch *b, int slot, double *u_host);
+int  piafs_batch_info(piafs_batch *b, int slot, piafs_info *info);
 const char *piafs_batch_last_error(const piafs_batch *b);
 
 /* Device memory a case built from `dir` would hold (admission control for

@@ -42,7 +42,8 @@ EXPORTS = [
     "piafs_download_state", "piafs_set_time", "piafs_step", "piafs_step_diag", "piafs_sync", "piafs_rhs",
     "piafs_write_binary", "piafs_solve", "piafs_kernel_times", "piafs_tridiag_host",
     "piafs_weno_weights_host", "piafs_batch_create", "piafs_batch_destroy", "piafs_batch_step",
-    "piafs_batch_sync", "piafs_batch_download", "piafs_batch_info", "piafs_batch_last_error",
+    "piafs_batch_sync", "piafs_batch_use_graph", "piafs_batch_add", "piafs_batch_remove",
+    "piafs_batch_steps_done", "piafs_batch_download", "piafs_batch_info", "piafs_batch_last_error",
     "piafs_estimate_bytes",
 ]
 
@@ -77,6 +78,10 @@ def lib():
     L.piafs_batch_destroy.argtypes = [vp]
     L.piafs_batch_step.argtypes = [vp, ip]
     L.piafs_batch_sync.argtypes = [vp]
+    L.piafs_batch_use_graph.argtypes = [vp, ip]
+    L.piafs_batch_add.argtypes = [vp, cp, ctypes.POINTER(ctypes.c_int)]
+    L.piafs_batch_remove.argtypes = [vp, ip]
+    L.piafs_batch_steps_done.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_int)]
     L.piafs_batch_download.argtypes = [vp, ip, vp]
     L.piafs_batch_info.argtypes = [vp, ip, ctypes.POINTER(Info)]
     L.piafs_batch_last_error.argtypes = [vp]
@@ -201,30 +206,57 @@ class Case:
 
 
 class Batch:
-    """Many independent cases advanced together (piafs_batch_* in the C ABI)."""
+    """Many independent cases advanced together (piafs_batch_* in the C ABI).
 
-    def __init__(self, directories, device: int = 0):
+    Every step is one launch per kernel class and RK stage covering all member
+    cases, replayed from a captured HIP graph unless ``use_graph(False)``.
+    Members are addressed by slot numbers; :meth:`add` and :meth:`remove`
+    change the member set between steps (continuous batching)."""
+
+    def __init__(self, directories=(), device: int = 0):
         L = lib()
-        arr = (ctypes.c_char_p * len(directories))(*[d.encode() for d in directories])
+        directories = list(directories)
+        arr = (ctypes.c_char_p * max(1, len(directories)))(*[d.encode() for d in directories])
         h = ctypes.c_void_p()
         rc = L.piafs_batch_create(arr, len(directories), int(device), ctypes.byref(h))
+        self._h, self._L = h, L
         if rc:
-            raise PiafsError(f"piafs_batch_create failed ({rc})")
-        self._h, self._L, self.n = h, L, len(directories)
+            msg = L.piafs_batch_last_error(h).decode() if h else ""
+            self.close()
+            raise PiafsError(f"piafs_batch_create failed ({rc}): {msg}")
+        self.slots = list(range(len(directories)))
+
+    def _check(self, rc, what):
+        if rc:
+            raise PiafsError(f"{what} failed ({rc}): {self._L.piafs_batch_last_error(self._h).decode()}")
 
     def step(self, n: int = 1):
-        rc = self._L.piafs_batch_step(self._h, int(n))
-        if rc:
-            raise PiafsError(f"batch step failed ({rc}): {self._L.piafs_batch_last_error(self._h).decode()}")
+        self._check(self._L.piafs_batch_step(self._h, int(n)), "batch step")
 
     def sync(self):
-        rc = self._L.piafs_batch_sync(self._h)
-        if rc:
-            raise PiafsError(f"batch sync failed ({rc}): {self._L.piafs_batch_last_error(self._h).decode()}")
+        self._check(self._L.piafs_batch_sync(self._h), "batch sync")
+
+    def use_graph(self, on: bool = True):
+        self._check(self._L.piafs_batch_use_graph(self._h, int(bool(on))), "batch use_graph")
+
+    def add(self, directory: str) -> int:
+        k = ctypes.c_int(-1)
+        self._check(self._L.piafs_batch_add(self._h, directory.encode(), ctypes.byref(k)), "batch add")
+        self.slots.append(k.value)
+        return k.value
+
+    def remove(self, k: int):
+        self._check(self._L.piafs_batch_remove(self._h, int(k)), "batch remove")
+        self.slots.remove(k)
+
+    def steps_done(self, k: int) -> int:
+        n = ctypes.c_int(0)
+        self._check(self._L.piafs_batch_steps_done(self._h, int(k), ctypes.byref(n)), "batch steps_done")
+        return n.value
 
     def info(self, k: int) -> Info:
         inf = Info()
-        self._L.piafs_batch_info(self._h, int(k), ctypes.byref(inf))
+        self._check(self._L.piafs_batch_info(self._h, int(k), ctypes.byref(inf)), "batch info")
         return inf
 
     def solution(self, k: int) -> np.ndarray:
@@ -233,15 +265,19 @@ class Batch:
         for d in range(inf.ndims):
             n *= inf.dim[d]
         out = np.empty((n, inf.nvars))
-        rc = self._L.piafs_batch_download(self._h, int(k), _ptr(out))
-        if rc:
-            raise PiafsError(f"batch download failed ({rc})")
+        self._check(self._L.piafs_batch_download(self._h, int(k), _ptr(out)), "batch download")
         return out
 
     def close(self):
         if getattr(self, "_h", None):
             self._L.piafs_batch_destroy(self._h)
             self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

@@ -1,4 +1,11 @@
-// Batched ensembles (include/piafs_amd.h, piafs_batch_*).
+// Ensembles and continuous batching (include/piafs_amd.h, piafs_batch_*).
+//
+// A batch owns one engine: every member case is advanced by the same
+// launches (one per kernel class per RK stage), replayed from a captured HIP
+// graph.  Members are addressed by stable slot numbers; admitting or removing
+// a case between steps rebuilds the block tables and re-captures the graph.
+// Replaces the serial per-simulation loop of the reference's ensemble mode
+// (TimeRK.c:46-81, EnsembleSimulationsDefine.cpp:31-119).
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -7,63 +14,142 @@
 #include "../../include/piafs_amd.h"
 #include "case.hpp"
 
+using namespace piafs;
+
 struct piafs_batch {
-  std::vector<piafs_case *> cases;
+  Engine eng;
+  std::vector<Case *> slots;  // nullptr = free slot
+  std::vector<int> steps;     // steps taken by each slot since admission
   std::string err;
+
+  int rebuild() {
+    eng.cases.clear();
+    for (Case *c : slots)
+      if (c) eng.cases.push_back(c);
+    int rc = eng.build();
+    if (rc) err = eng.err;
+    return rc;
+  }
 };
+
+static Case *slot_case(piafs_batch *b, int k) {
+  if (!b || k < 0 || k >= (int)b->slots.size()) return nullptr;
+  return b->slots[k];
+}
 
 extern "C" {
 
 int piafs_batch_create(const char *const *dirs, int ncases, int device, piafs_batch **out) {
   *out = nullptr;
   piafs_batch *b = new piafs_batch();
+  b->eng.owns_cases = false;
+  b->eng.use_graph = true;
+  int rc = b->eng.init(device);
+  if (rc) {
+    delete b;
+    return rc;
+  }
   for (int k = 0; k < ncases; k++) {
-    piafs_case *c = nullptr;
-    int rc = piafs_create(dirs[k], device, &c);
+    Case *c = new Case();
+    rc = c->init(dirs[k], device, b->eng.stream);
     if (rc) {
-      b->err = std::string("case ") + std::to_string(k) + ": " + piafs_last_error(nullptr);
-      for (auto *x : b->cases) piafs_destroy(x);
-      delete b;
+      b->err = "case " + std::to_string(k) + ": " + c->err;
+      delete c;
+      for (Case *x : b->slots) delete x;
+      b->slots.clear();
+      *out = b;  // handle returned so that the error can be read; destroy it
       return rc;
     }
-    b->cases.push_back(c);
+    b->slots.push_back(c);
+    b->steps.push_back(0);
   }
+  rc = b->rebuild();
   *out = b;
-  return 0;
+  return rc;
 }
 
 int piafs_batch_destroy(piafs_batch *b) {
   if (!b) return 0;
-  for (auto *c : b->cases) piafs_destroy(c);
+  b->eng.sync();
+  b->eng.cases.clear();
+  for (Case *c : b->slots) delete c;
   delete b;
   return 0;
 }
 
+int piafs_batch_use_graph(piafs_batch *b, int on) {
+  b->eng.sync();
+  b->eng.use_graph = (on != 0);
+  return b->rebuild();
+}
+
+int piafs_batch_add(piafs_batch *b, const char *dir, int *slot) {
+  b->eng.sync();
+  Case *c = new Case();
+  int rc = c->init(dir ? dir : "", b->eng.device, b->eng.stream);
+  if (rc) {
+    b->err = c->err;
+    delete c;
+    return rc;
+  }
+  int k = 0;
+  while (k < (int)b->slots.size() && b->slots[k]) k++;
+  if (k == (int)b->slots.size()) {
+    b->slots.push_back(nullptr);
+    b->steps.push_back(0);
+  }
+  b->slots[k] = c;
+  b->steps[k] = 0;
+  if (slot) *slot = k;
+  return b->rebuild();
+}
+
+int piafs_batch_remove(piafs_batch *b, int k) {
+  Case *c = slot_case(b, k);
+  if (!c) return 1;
+  b->eng.sync();
+  b->slots[k] = nullptr;
+  int rc = b->rebuild();
+  delete c;
+  return rc;
+}
+
 int piafs_batch_step(piafs_batch *b, int nsteps) {
-  for (int s = 0; s < nsteps; s++)
-    for (auto *c : b->cases) {
-      int rc = piafs_step(c, 1);
-      if (rc) return rc;
-    }
+  int rc = b->eng.step(nsteps);
+  if (rc) {
+    b->err = b->eng.err;
+    return rc;
+  }
+  for (size_t k = 0; k < b->slots.size(); k++)
+    if (b->slots[k]) b->steps[k] += nsteps;
   return 0;
 }
 
 int piafs_batch_sync(piafs_batch *b) {
-  for (auto *c : b->cases) {
-    int rc = piafs_sync(c);
-    if (rc) return rc;
-  }
-  return 0;
+  int rc = b->eng.sync();
+  if (rc) b->err = b->eng.err;
+  return rc;
 }
 
 int piafs_batch_download(piafs_batch *b, int k, double *u) {
-  if (k < 0 || k >= (int)b->cases.size()) return 1;
-  return piafs_download_state(b->cases[k], u);
+  Case *c = slot_case(b, k);
+  if (!c) return 1;
+  int rc = c->download(u, b->eng.stream);
+  if (rc) b->err = c->err;
+  return rc;
 }
 
 int piafs_batch_info(piafs_batch *b, int k, piafs_info *info) {
-  if (k < 0 || k >= (int)b->cases.size()) return 1;
-  return piafs_get_info(b->cases[k], info);
+  Case *c = slot_case(b, k);
+  if (!c || !info) return 1;
+  fill_info(c, info);
+  return 0;
+}
+
+int piafs_batch_steps_done(piafs_batch *b, int k, int *steps) {
+  if (!slot_case(b, k) || !steps) return 1;
+  *steps = b->steps[k];
+  return 0;
 }
 
 const char *piafs_batch_last_error(const piafs_batch *b) { return b ? b->err.c_str() : ""; }

@@ -1,5 +1,4 @@
-// A PIAFS case resident on one GPU: parsed configuration, device buffers and
-// the stream its time steps are queued on.
+// Cases resident on one GPU and the engine that advances an ensemble of them.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,62 +7,77 @@
 #include <utility>
 #include <vector>
 
+#include "../../include/piafs_amd.h"
 #include "config.hpp"
 #include "kernels.hpp"
 
 namespace piafs {
 
-constexpr int NCLASS = 10;  // kernel classes timed by the profiling mode
-const char *class_name(int c);
 Grid make_grid(const Config &c);
 
+// One PIAFS simulation: parsed configuration, device buffers and the
+// descriptor the kernels read.
 struct Case {
   Config cfg;
-  Grid G{};
-  SweepGeom Sx{}, Sy{};
-  BCMap bc{};
-  PhysParams P{};
-  ReconParams R{};
+  CaseDev h{};          // host copy of the device descriptor
   int device = -1;
-  hipStream_t stream = nullptr;
-  // second stream for the y-sweep and viscous derivatives (joined by events)
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  bool dual = false;
-  double t = 0.0;
-  std::string err;
   size_t bytes = 0;
+  double t = 0.0;       // host mirror of the simulation time (t += dt per step)
+  std::string err;
   std::vector<void *> allocs;
-
-  // state (natural planar layout), stage state and its transposed copy
-  double *u = nullptr, *U = nullptr, *UT = nullptr, *Udot[4] = {nullptr, nullptr, nullptr, nullptr};
-  double *flux = nullptr, *ifc[4] = {nullptr, nullptr, nullptr, nullptr}, *fIx = nullptr, *fIy = nullptr;
-  double *scr_b = nullptr, *scr_c = nullptr, *hx = nullptr, *hxT = nullptr;
-  double *Qd = nullptr, *FV = nullptr, *dxinv = nullptr, *imap = nullptr;
-  double *partials = nullptr, *uprev = nullptr;
-  // y-sweep buffers when the sweeps run concurrently
-  double *flux_y = nullptr, *ifcy[4] = {nullptr, nullptr, nullptr, nullptr}, *scr_by = nullptr, *scr_cy = nullptr;
+  double *uprev = nullptr, *partials = nullptr;
   int nparts = 0;
 
-  // profiling: events bracketing kernel classes
+  ~Case();
+  int init(const std::string &dir, int dev, hipStream_t st);
+  int fail(int code, const std::string &msg);
+  double *dalloc(size_t n, hipStream_t st);
+  int upload(const double *uh, hipStream_t st);
+  int download(double *uh, hipStream_t st, const double *src = nullptr);
+  int write_binary(const std::string &path, hipStream_t st);
+  // lanes per line system: segments of at most max_rows rows (descriptor only;
+  // the engine's plan must be rebuilt afterwards)
+  int set_team_rows(int max_rows);
+  static size_t plan_bytes(const Config &c);
+};
+
+// An ensemble of cases advanced together: one launch per kernel class per RK
+// stage covering every case, optionally replayed from a captured HIP graph.
+struct Engine {
+  std::vector<Case *> cases;
+  bool owns_cases = true;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  CaseDev *d_cases = nullptr;
+  int *d_status = nullptr;  // one status word per member case
+  Plan plan;
+  std::vector<void *> plan_allocs;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::string err;
+
+  // profiling: events bracketing kernel classes (eager launches only)
   bool profiling = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
   std::vector<int> ev_cls;
   size_t ev_used = 0;
 
-  ~Case();
-  int init(const std::string &dir, int dev);
+  ~Engine();
+  int init(int dev);
+  int add(Case *c);
+  int build();                 // descriptor table and plans for the current member set
+  void enqueue_step();         // eager launches of one full step
+  int step(int n);             // n steps (graph replay when enabled)
+  int stage_only(int s);       // one stage (used for a single RHS evaluation)
+  int sync();
+  void mark(int cls, bool begin);
   int fail(int code, const std::string &msg);
-  double *dalloc(size_t n);
-  int upload(const double *uh);
-  int download(double *uh, const double *src = nullptr);
-  void mark(int cls, bool begin, hipStream_t st = nullptr);
-  void rhs_from_stage(double time, double *out);
-  void enqueue_step();
-  int step(int n, double *cfl, double *norm);
-  int rhs_host(const double *uh, double time, double *rh);
-  int write_binary(const std::string &path);
-  static size_t plan_bytes(const Config &c);
+  void release_plan();
 };
 
+}  // namespace piafs
+
+namespace piafs {
+void fill_info(const Case *c, piafs_info *info);
 }  // namespace piafs

@@ -2,9 +2,9 @@
 // right-hand side and the explicit Runge-Kutta stage algebra, FP64 throughout.
 //
 // Arithmetic follows the reference's association order term by term (the
-// file:line of the formula being restated is given at each device function);
-// the compiler may contract a*b+c into FMA, which changes results only at the
-// rounding level.
+// file:line of the formula being restated is given at each device function),
+// and the library is compiled without floating-point contraction so that the
+// rounding of every +, -, *, / and sqrt matches the reference build.
 #include "kernels.hpp"
 
 #include <cstdio>
@@ -13,7 +13,7 @@ namespace piafs {
 
 namespace {
 
-constexpr int TB = 256;  // threads per block for 1D-indexed kernels
+constexpr int TB = 256;  // threads per block
 
 inline unsigned nblk(long n, int tb = TB) { return (unsigned)((n + tb - 1) / tb); }
 
@@ -92,15 +92,6 @@ __device__ __forceinline__ void crweno5_cands(double m2, double m1, double p1, d
   f3 = (s6)*m1 + (5 * s6) * p1;
 }
 
-// ---------------------------------------------------------------------------
-// Stage state + boundary conditions.
-// TimeRK.c:46-57 (U = u + sum dt*A*Udot), then ApplyBoundaryConditions
-// (BCPeriodic.c:22-62: ghost <- opposite interior; BCExtrapolate.c:16-51:
-// mirror copy).  Each ghost point is evaluated at its boundary source point,
-// which equals applying the BC to the stage state.  Points outside every
-// boundary zone (the corners) are zero, as in the reference where they are
-// never written.
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ int bc_src(int i, int N, const FaceBC &lo, const FaceBC &hi, int other, bool &ok,
                                       int &other_shift) {
   other_shift = 0;
@@ -117,131 +108,6 @@ __device__ __forceinline__ int bc_src(int i, int N, const FaceBC &lo, const Face
   return (i < 0) ? -i - 1 : 2 * N - i - 1;
 }
 
-template <int TILE>
-__global__ void __launch_bounds__(TILE * 8) k_stage_bc2d(Grid G, BCMap bc, const double *__restrict__ u,
-                                                         const double *__restrict__ d0, const double *__restrict__ d1,
-                                                         const double *__restrict__ d2, double a0, double a1,
-                                                         double a2, int nprev, double *__restrict__ U,
-                                                         double *__restrict__ UT) {
-  __shared__ double tile[TILE][TILE + 1];
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int i0 = blockIdx.x * TILE - G.g, j0 = blockIdx.y * TILE - G.gy;
-  const int v = blockIdx.z;
-  const long pl = (long)v * G.plane;
-  for (int r = ty; r < TILE; r += 8) {
-    const int i = i0 + tx, j = j0 + r;
-    double val = 0.0;
-    if (i < G.nx + G.g && j < G.ny + G.gy) {
-      bool ok = true;
-      int sx, sy;
-      int si = bc_src(i, G.nx, bc.f[0][0], bc.f[0][1], j, ok, sx);
-      int sj = j;
-      if (ok) {
-        if (i >= 0 && i < G.nx) {
-          sj = bc_src(j, G.ny, bc.f[1][0], bc.f[1][1], i, ok, sy);
-          si = i + sy;
-        } else {
-          sj = j + sx;
-        }
-      }
-      if (ok) {
-        const long p = pl + (long)(sj + G.gy) * G.nxg + (si + G.g);
-        val = u[p];
-        if (nprev > 0) val += a0 * d0[p];
-        if (nprev > 1) val += a1 * d1[p];
-        if (nprev > 2) val += a2 * d2[p];
-      }
-      U[pl + (long)(j + G.gy) * G.nxg + (i + G.g)] = val;
-    }
-    tile[r][tx] = val;
-  }
-  if (!UT) return;
-  __syncthreads();
-  // transposed write: UT[v][i][j]
-  for (int r = ty; r < TILE; r += 8) {
-    const int i = i0 + r, j = j0 + tx;
-    if (i < G.nx + G.g && j < G.ny + G.gy) UT[pl + (long)(i + G.g) * G.nyg + (j + G.gy)] = tile[tx][r];
-  }
-}
-
-__global__ void k_stage_bc1d(Grid G, BCMap bc, const double *__restrict__ u, const double *__restrict__ d0,
-                             const double *__restrict__ d1, const double *__restrict__ d2, double a0, double a1,
-                             double a2, int nprev, double *__restrict__ U) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)G.nxg * G.nv) return;
-  const int v = (int)(t / G.nxg), i = (int)(t % G.nxg) - G.g;
-  bool ok = true;
-  int sh;
-  const int si = bc_src(i, G.nx, bc.f[0][0], bc.f[0][1], 0, ok, sh);
-  double val = 0.0;
-  if (ok) {
-    const long p = (long)v * G.plane + si + G.g;
-    val = u[p];
-    if (nprev > 0) val += a0 * d0[p];
-    if (nprev > 1) val += a1 * d1[p];
-    if (nprev > 2) val += a2 * d2[p];
-  }
-  U[(long)v * G.plane + i + G.g] = val;
-}
-
-// ---------------------------------------------------------------------------
-// Cell-centred flux at every position of every interior line.
-// NavierStokes2DFlux.c:24-52 (navierstokes2d.h:70-110), Euler1DFlux.c:19
-// (euler1d.h:61-85).
-// ---------------------------------------------------------------------------
-template <int MODEL, int DIR>
-__global__ void __launch_bounds__(TB) k_flux(SweepGeom S, int nv, double gamma, const double *__restrict__ q,
-                                             double *__restrict__ f) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long npos = S.N + 2 * S.g;
-  if (t >= npos * S.nlines) return;
-  const int l = (int)(t % S.nlines);
-  const long k = t / S.nlines;
-  const long b = k * S.pos_stride + S.line_off + l;
-  if (MODEL == 1) {
-    const double rho = q[b], v = q[S.plane + b] / rho, e = q[2 * S.plane + b];
-    const double P = (e - 0.5 * rho * v * v) * (gamma - 1.0);
-    f[b] = rho * v;
-    f[S.plane + b] = rho * v * v + P;
-    f[2 * S.plane + b] = (e + P) * v;
-    for (int m = 3; m < nv; m++) f[m * S.plane + b] = v * q[m * S.plane + b];
-  } else {
-    const double rho = q[b];
-    const double vx = (rho == 0) ? 0 : q[S.plane + b] / rho;
-    const double vy = (rho == 0) ? 0 : q[2 * S.plane + b] / rho;
-    const double e = q[3 * S.plane + b];
-    const double vsq = (vx * vx) + (vy * vy);
-    const double P = (e - 0.5 * rho * vsq) * (gamma - 1.0);
-    const double vn = DIR == 0 ? vx : vy;
-    f[b] = rho * vn;
-    if (DIR == 0) {
-      f[S.plane + b] = rho * vx * vx + P;
-      f[2 * S.plane + b] = rho * vx * vy;
-    } else {
-      f[S.plane + b] = rho * vy * vx;
-      f[2 * S.plane + b] = rho * vy * vy + P;
-    }
-    f[3 * S.plane + b] = (e + P) * vn;
-    for (int m = 4; m < nv; m++) f[m * S.plane + b] = vn * q[m * S.plane + b];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Component-wise WENO5 / CRWENO5 reconstruction, one lane per
-// (line, variable, reconstruction), sweeping the line with a six-value
-// register window.  Reconstructions r: 0 uL, 1 uR (of u), 2 fL, 3 fR (of the
-// flux); r even = left-biased.
-//  * weights: WENOFifthOrderCalculateWeights{JS,M,Z,YC}
-//    (WENOFifthOrderCalculateWeights.c:112-237 and the variants);
-//    optimal weights :202-220 (WENO5 values at the two end interfaces of a
-//    CRWENO line).
-//  * WENO5: Interp1PrimFifthOrderWENO.c:124-166.
-//  * CRWENO5: row assembly Interp1PrimFifthOrderCRWENO.c:132-201 (identity
-//    rows with WENO5 right-hand sides at interfaces 0 and N), then the Thomas
-//    algorithm of tridiagLU.c:133-150 (forward) and :243-255 (back
-//    substitution).  The forward sweep keeps b' and c per row in scratch and
-//    x' in the output array; the backward sweep overwrites it with x.
-// ---------------------------------------------------------------------------
 // Coefficients of row k of one line system: A, B, C and the right-hand side
 // (for WENO5 the right-hand side is the interface value itself).
 template <int CR, int KIND>
@@ -271,215 +137,6 @@ __device__ __forceinline__ void row_coeffs(int k, int N, bool left, const ReconP
     if (left) { Ak = lo; Ck = hi; } else { Ak = hi; Ck = lo; }
   }
   Rk = a1 * f1 + a2 * f2 + a3 * f3;
-}
-
-// Rows are processed in blocks of BLK: the stencil values feeding the next
-// block and, in the backward sweep, the next block's (b', c, x') are loaded
-// one block ahead, so a memory latency is paid once per block and the
-// block's independent weight evaluations overlap.
-template <int CR, int KIND, int BLK>
-__global__ void __launch_bounds__(TB) k_recon_comp(SweepGeom S, int nv, ReconParams R,
-                                                   const double *__restrict__ q, const double *__restrict__ f,
-                                                   double *__restrict__ uL, double *__restrict__ uR,
-                                                   double *__restrict__ fL, double *__restrict__ fR,
-                                                   double *__restrict__ scr_b, double *__restrict__ scr_c) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const int nl = S.nlines;
-  if (t >= (long)nl * nv * 4) return;
-  const int l = (int)(t % nl);
-  const int vr = (int)(t / nl);
-  const int v = vr % nv, r = vr / nv;
-  const bool left = !(r & 1);
-  const double *src = ((r < 2) ? q : f) + (long)v * S.plane + S.line_off + l;
-  const long ps = S.pos_stride;
-  const int N = S.N, g = S.g;
-  const int last = N + g - 1;  // last position held by a line (ghosts included)
-  double *out = ((r == 0) ? uL : (r == 1) ? uR : (r == 2) ? fL : fR) + (long)v * (N + 1) * nl + l;
-  double *sb = nullptr, *sc = nullptr;
-  if (CR) {
-    const long so = (long)(r * nv + v) * (N + 1) * nl + l;
-    sb = scr_b + so;
-    sc = scr_c + so;
-  }
-  auto ld = [&](int pos) -> double { return (pos <= last) ? src[(long)(pos + g) * ps] : 0.0; };
-  // window w0..w5 = positions k-3 .. k+2; cur[j] = position k0+3+j feeds row k0+j
-  double w0 = ld(-3), w1 = ld(-2), w2 = ld(-1), w3 = ld(0), w4 = ld(1), w5 = ld(2);
-  double cur[BLK];
-#pragma unroll
-  for (int j = 0; j < BLK; j++) cur[j] = ld(3 + j);
-  double bprev = 1.0, cprev = 0.0, xprev = 0.0;
-  for (int k0 = 0; k0 <= N; k0 += BLK) {
-    double nxt[BLK];
-#pragma unroll
-    for (int j = 0; j < BLK; j++) nxt[j] = ld(k0 + BLK + 3 + j);
-    double Ab[BLK], Bb[BLK], Cb[BLK], Rb[BLK];
-#pragma unroll
-    for (int j = 0; j < BLK; j++) {
-      row_coeffs<CR, KIND>(k0 + j, N, left, R, w0, w1, w2, w3, w4, w5, Ab[j], Bb[j], Cb[j], Rb[j]);
-      w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = w5; w5 = cur[j];
-    }
-#pragma unroll
-    for (int j = 0; j < BLK; j++) {
-      const int k = k0 + j;
-      if (k > N) break;
-      if (!CR) {
-        out[(long)k * nl] = Rb[j];
-      } else {
-        double bk, xk;
-        if (k == 0) { bk = Bb[j]; xk = Rb[j]; }
-        else {
-          const double fac = Ab[j] / bprev;
-          bk = Bb[j] - fac * cprev;
-          xk = Rb[j] - fac * xprev;
-        }
-        sb[(long)k * nl] = bk;
-        sc[(long)k * nl] = Cb[j];
-        out[(long)k * nl] = xk;
-        bprev = bk; cprev = Cb[j]; xprev = xk;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < BLK; j++) cur[j] = nxt[j];
-  }
-  if (CR) {
-    double xn = xprev / bprev;  // row N: c_N = 0
-    out[(long)N * nl] = xn;
-    double bb[BLK], cc[BLK], xx[BLK];
-#pragma unroll
-    for (int j = 0; j < BLK; j++) {
-      const int k = N - 1 - j;
-      bb[j] = k >= 0 ? sb[(long)k * nl] : 1.0;
-      cc[j] = k >= 0 ? sc[(long)k * nl] : 0.0;
-      xx[j] = k >= 0 ? out[(long)k * nl] : 0.0;
-    }
-    for (int k0 = N - 1; k0 >= 0; k0 -= BLK) {
-      double nb[BLK], nc[BLK], nx[BLK];
-#pragma unroll
-      for (int j = 0; j < BLK; j++) {
-        const int k = k0 - BLK - j;
-        nb[j] = k >= 0 ? sb[(long)k * nl] : 1.0;
-        nc[j] = k >= 0 ? sc[(long)k * nl] : 0.0;
-        nx[j] = k >= 0 ? out[(long)k * nl] : 0.0;
-      }
-#pragma unroll
-      for (int j = 0; j < BLK; j++) {
-        const int k = k0 - j;
-        if (k < 0) break;
-        const double xk = (xx[j] - cc[j] * xn) / bb[j];
-        out[(long)k * nl] = xk;
-        xn = xk;
-      }
-#pragma unroll
-      for (int j = 0; j < BLK; j++) { bb[j] = nb[j]; cc[j] = nc[j]; xx[j] = nx[j]; }
-    }
-  }
-}
-
-// CRWENO5 reconstruction as a producer/consumer workgroup: 64 systems (64
-// consecutive lines of one variable and reconstruction) per workgroup of four
-// waves.  Waves 1-3 evaluate the row coefficients (weights, A, B, C, R) of a
-// chunk of KR rows into one half of a double-buffered LDS ring while wave 0
-// runs the Thomas forward recurrence (tridiagLU.c:133-150) over the previous
-// chunk; wave 0 then back-substitutes (:243-255) with a one-block-ahead
-// prefetch.  Per system the arithmetic and its order are those of
-// k_recon_comp<1, KIND>.
-template <int KIND, int KR>
-__global__ void __launch_bounds__(256) k_crweno_pc(SweepGeom S, int nv, ReconParams R,
-                                                   const double *__restrict__ q, const double *__restrict__ f,
-                                                   double *__restrict__ uL, double *__restrict__ uR,
-                                                   double *__restrict__ fL, double *__restrict__ fR,
-                                                   double *__restrict__ scr_b, double *__restrict__ scr_c) {
-  __shared__ double buf[2][KR][4][64];
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int nl = S.nlines, N = S.N, g = S.g;
-  const int l0 = blockIdx.x * 64;
-  const int v = blockIdx.y % nv, r = blockIdx.y / nv;
-  const bool left = !(r & 1);
-  const double *src = ((r < 2) ? q : f) + (long)v * S.plane + S.line_off + l0;
-  const long ps = S.pos_stride;
-  const int nchunks = (N + 1 + KR - 1) / KR;
-  const int my_l = l0 + lane;
-  const bool lane_ok = my_l < nl;
-  double *out = ((r == 0) ? uL : (r == 1) ? uR : (r == 2) ? fL : fR) + (long)v * (N + 1) * nl + my_l;
-  const long so = (long)(r * nv + v) * (N + 1) * nl + my_l;
-  double *sb = scr_b + so, *sc = scr_c + so;
-  double bprev = 1.0, cprev = 0.0, xprev = 0.0;
-  for (int c = 0; c <= nchunks; c++) {
-    if (wave > 0 && c < nchunks) {
-      const int half = c & 1;
-      for (int it = tid - 64; it < KR * 64; it += 192) {
-        const int kr = it >> 6, ln = it & 63;
-        const int k = c * KR + kr;
-        if (k <= N && l0 + ln < nl) {
-          const double *sp = src + ln;
-          const double w0 = sp[(long)(k - 3 + g) * ps], w1 = sp[(long)(k - 2 + g) * ps];
-          const double w2 = sp[(long)(k - 1 + g) * ps], w3 = sp[(long)(k + g) * ps];
-          const double w4 = sp[(long)(k + 1 + g) * ps], w5 = sp[(long)(k + 2 + g) * ps];
-          double A, B, C, Rk;
-          row_coeffs<1, KIND>(k, N, left, R, w0, w1, w2, w3, w4, w5, A, B, C, Rk);
-          buf[half][kr][0][ln] = A;
-          buf[half][kr][1][ln] = B;
-          buf[half][kr][2][ln] = C;
-          buf[half][kr][3][ln] = Rk;
-        }
-      }
-    }
-    if (wave == 0 && c > 0 && lane_ok) {
-      const int cc = c - 1, half = cc & 1;
-#pragma unroll
-      for (int kr = 0; kr < KR; kr++) {
-        const int k = cc * KR + kr;
-        if (k > N) break;
-        const double A = buf[half][kr][0][lane], B = buf[half][kr][1][lane];
-        const double C = buf[half][kr][2][lane], Rk = buf[half][kr][3][lane];
-        double bk, xk;
-        if (k == 0) { bk = B; xk = Rk; }
-        else {
-          const double fac = A / bprev;
-          bk = B - fac * cprev;
-          xk = Rk - fac * xprev;
-        }
-        sb[(long)k * nl] = bk;
-        sc[(long)k * nl] = C;
-        out[(long)k * nl] = xk;
-        bprev = bk; cprev = C; xprev = xk;
-      }
-    }
-    __syncthreads();
-  }
-  if (wave != 0 || !lane_ok) return;
-  constexpr int BLK = 8;
-  double xn = xprev / bprev;  // row N: c_N = 0
-  out[(long)N * nl] = xn;
-  double bb[BLK], cc[BLK], xx[BLK];
-#pragma unroll
-  for (int j = 0; j < BLK; j++) {
-    const int k = N - 1 - j;
-    bb[j] = k >= 0 ? sb[(long)k * nl] : 1.0;
-    cc[j] = k >= 0 ? sc[(long)k * nl] : 0.0;
-    xx[j] = k >= 0 ? out[(long)k * nl] : 0.0;
-  }
-  for (int k0 = N - 1; k0 >= 0; k0 -= BLK) {
-    double nb[BLK], nc[BLK], nx[BLK];
-#pragma unroll
-    for (int j = 0; j < BLK; j++) {
-      const int k = k0 - BLK - j;
-      nb[j] = k >= 0 ? sb[(long)k * nl] : 1.0;
-      nc[j] = k >= 0 ? sc[(long)k * nl] : 0.0;
-      nx[j] = k >= 0 ? out[(long)k * nl] : 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < BLK; j++) {
-      const int k = k0 - j;
-      if (k < 0) break;
-      const double xk = (xx[j] - cc[j] * xn) / bb[j];
-      out[(long)k * nl] = xk;
-      xn = xk;
-    }
-#pragma unroll
-    for (int j = 0; j < BLK; j++) { bb[j] = nb[j]; cc[j] = nc[j]; xx[j] = nx[j]; }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -526,23 +183,518 @@ __device__ __forceinline__ void euler1d_eig(double g, double u0, double u1, doub
   E.R[2][2] = rho * ((v * v) / 2 + (c * c) / (g - 1) + c * v) / (2 * c);
 }
 
+// ---------------------------------------------------------------------------
+// Ensemble indexing: block -> case, local work item
+// ---------------------------------------------------------------------------
+#define CASE_ARGS const CaseDev *__restrict__ cases, const int *__restrict__ blockcase, \
+                  const int *__restrict__ firstblock, int stage
+#define CASE_PREAMBLE                                                      \
+  const int cidx = blockcase[blockIdx.x];                                  \
+  const CaseDev &C = cases[cidx];                                          \
+  const long t = (long)(blockIdx.x - firstblock[cidx]) * blockDim.x + threadIdx.x; \
+  (void)stage;
+
+// ---------------------------------------------------------------------------
+// Stage state + boundary conditions.
+// TimeRK.c:46-57 (U = u + sum_j dt*A[s][j]*Udot_j, zero coefficients skipped:
+// adding 0*Udot leaves U unchanged), then ApplyBoundaryConditions
+// (BCPeriodic.c:22-62: ghost <- opposite interior; BCExtrapolate.c:16-51:
+// mirror copy).  Each ghost point is evaluated at its boundary source point,
+// which equals applying the BC to the stage state.  Points outside every
+// boundary zone (the corners) are zero, as in the reference where they are
+// never written.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double stage_value(const CaseDev &C, int s, long p) {
+  double val = C.u[p];
+  const int ns = C.nstages;
+  for (int j = 0; j < s; j++) {
+    const double a = C.dt * C.A[s * ns + j];
+    if (a != 0.0) val += a * C.Udot[j][p];
+  }
+  return val;
+}
+
+__global__ void __launch_bounds__(256) k_stage(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const Grid &G = C.G;
+  const int s = stage;
+  if (G.ndims == 1) {
+    if (t >= (long)G.nxg * G.nv) return;
+    const int v = (int)(t / G.nxg), i = (int)(t % G.nxg) - G.g;
+    bool ok = true;
+    int sh;
+    const int si = bc_src(i, G.nx, C.bc.f[0][0], C.bc.f[0][1], 0, ok, sh);
+    C.U[(long)v * G.plane + i + G.g] = ok ? stage_value(C, s, (long)v * G.plane + si + G.g) : 0.0;
+    return;
+  }
+  // 2D: 32x32 tiles, 256 threads; natural write, then transposed write via LDS
+  __shared__ double tile[32][33];
+  const int ntx = (G.nxg + 31) / 32, nty = (G.nyg + 31) / 32;
+  const int id = (int)(blockIdx.x - firstblock[cidx]);
+  const int v = id / (ntx * nty), rem = id % (ntx * nty);
+  const int i0 = (rem % ntx) * 32 - G.g, j0 = (rem / ntx) * 32 - G.gy;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const long pl = (long)v * G.plane;
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + tx, j = j0 + r;
+    double val = 0.0;
+    if (i < G.nx + G.g && j < G.ny + G.gy) {
+      bool ok = true;
+      int sx, sy;
+      int si = bc_src(i, G.nx, C.bc.f[0][0], C.bc.f[0][1], j, ok, sx);
+      int sj = j;
+      if (ok) {
+        if (i >= 0 && i < G.nx) {
+          sj = bc_src(j, G.ny, C.bc.f[1][0], C.bc.f[1][1], i, ok, sy);
+          si = i + sy;
+        } else {
+          sj = j + sx;
+        }
+      }
+      if (ok) val = stage_value(C, s, pl + (long)(sj + G.gy) * G.nxg + (si + G.g));
+      C.U[pl + (long)(j + G.gy) * G.nxg + (i + G.g)] = val;
+    }
+    tile[r][tx] = val;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + r, j = j0 + tx;
+    if (i < G.nx + G.g && j < G.ny + G.gy) C.UT[pl + (long)(i + G.g) * G.nyg + (j + G.gy)] = tile[tx][r];
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// Cell-centred flux at every position of every interior line, both sweeps:
+// the x-sweep from the natural state, the y-sweep from the transposed copy.
+// NavierStokes2DFlux.c:24-52 (navierstokes2d.h:70-110), Euler1DFlux.c:19
+// (euler1d.h:61-85).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_flux(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const long nX = (long)(C.Sx.N + 2 * C.Sx.g) * C.Sx.nlines;
+  const long nY = (C.G.ndims == 2) ? (long)(C.Sy.N + 2 * C.Sy.g) * C.Sy.nlines : 0;
+  if (t >= nX + nY) return;
+  const int dir = t < nX ? 0 : 1;
+  const SweepGeom &S = dir == 0 ? C.Sx : C.Sy;
+  const long tt = dir == 0 ? t : t - nX;
+  const double *q = dir == 0 ? C.U : C.UT;
+  double *f = dir == 0 ? C.flux_x : C.flux_y;
+  const int P = S.N + 2 * S.g;
+  const int pos = (int)(tt % P) - S.g, l = (int)(tt / P);
+  const long b = S.base + (long)l * S.line_stride + pos;
+  const double gamma = C.P.gamma;
+  const int nv = C.G.nv;
+  if (C.model == 1) {
+    const double rho = q[b], v = q[S.plane + b] / rho, e = q[2 * S.plane + b];
+    const double P_ = (e - 0.5 * rho * v * v) * (gamma - 1.0);
+    f[b] = rho * v;
+    f[S.plane + b] = rho * v * v + P_;
+    f[2 * S.plane + b] = (e + P_) * v;
+    for (int m = 3; m < nv; m++) f[m * S.plane + b] = v * q[m * S.plane + b];
+  } else {
+    const double rho = q[b];
+    const double vx = (rho == 0) ? 0 : q[S.plane + b] / rho;
+    const double vy = (rho == 0) ? 0 : q[2 * S.plane + b] / rho;
+    const double e = q[3 * S.plane + b];
+    const double vsq = (vx * vx) + (vy * vy);
+    const double P_ = (e - 0.5 * rho * vsq) * (gamma - 1.0);
+    const double vn = dir == 0 ? vx : vy;
+    f[b] = rho * vn;
+    if (dir == 0) {
+      f[S.plane + b] = rho * vx * vx + P_;
+      f[2 * S.plane + b] = rho * vx * vy;
+    } else {
+      f[S.plane + b] = rho * vy * vx;
+      f[2 * S.plane + b] = rho * vy * vy + P_;
+    }
+    f[3 * S.plane + b] = (e + P_) * vn;
+    for (int m = 4; m < nv; m++) f[m * S.plane + b] = vn * q[m * S.plane + b];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tridiagonal line systems solved by teams of lanes, with the reference's
+// arithmetic.
+//
+// A team of T lanes (a power of two; teams of more than 64 lanes span whole
+// waves of one block) owns one system, and each lane keeps a segment of at
+// most kMaxRows consecutive rows in registers.  The result is the one of the
+// sequential Thomas sweep of tridiagLU.c:133-150 (fac = a_k/b'_{k-1},
+// b'_k = b_k - fac c_{k-1}, x'_k = x_k - fac x'_{k-1}) and :243-255
+// (x_k = (x'_k - c_k x_{k+1}) / b'_k), bit for bit:
+//  1. each segment gets an estimate of the state entering it -- the pivot
+//     b' from a scan of the segments' Moebius maps b'_{k-1} -> b'_k (2x2
+//     matrices), x' from a scan of their affine maps;
+//  2. every lane runs the sweep over its rows from its entry state, and each
+//     lane's entry becomes the state its predecessor ended with; this repeats
+//     until no entry changes.  Lane 0 starts from the exact state, so the
+//     fixed point is reached after at most T passes and is exactly the
+//     sequential sweep; with the estimates (and the sweep's contraction) the
+//     ends are already exact after the first pass, so two passes are typical;
+//  3. the back substitution is treated the same way from the top.
+// The systems are never written to memory; only the solution is.
+// ---------------------------------------------------------------------------
+struct Mob {
+  double a, b, c, d;  // [[a, b], [c, d]]
+};
+struct Aff {
+  double a, b;  // x -> a*x + b
+};
+
+// Rescale by a power of two (exact) so that products of many segment maps
+// neither overflow nor underflow; a Moebius map is unchanged by scaling.
+__device__ __forceinline__ Mob mob_norm(Mob r) {
+  const double mx = fmax(fmax(fabs(r.a), fabs(r.b)), fmax(fabs(r.c), fabs(r.d)));
+  if (mx > 0.0 && mx < INFINITY) {
+    const int e = ilogb(mx);
+    r.a = ldexp(r.a, -e);
+    r.b = ldexp(r.b, -e);
+    r.c = ldexp(r.c, -e);
+    r.d = ldexp(r.d, -e);
+  }
+  return r;
+}
+__device__ __forceinline__ Mob mob_mul(const Mob &x, const Mob &y) {
+  Mob r;
+  r.a = x.a * y.a + x.b * y.c;
+  r.b = x.a * y.b + x.b * y.d;
+  r.c = x.c * y.a + x.d * y.c;
+  r.d = x.c * y.b + x.d * y.d;
+  return mob_norm(r);
+}
+// outer o inner
+__device__ __forceinline__ Aff aff_cmp(const Aff &o, const Aff &i) { return Aff{o.a * i.a, o.a * i.b + o.b}; }
+
+// Exclusive scans over the lanes of a team.  `lane` is the lane within the
+// team; `red` is LDS for one Mob per wave of the block.
+__device__ Mob team_scan_mob(Mob x, int lane, int T, double *red) {
+  const int W = T < 64 ? T : 64;
+  const int wl = lane & (W - 1);
+  for (int d = 1; d < W; d <<= 1) {
+    Mob o;
+    o.a = __shfl_up(x.a, d, W);
+    o.b = __shfl_up(x.b, d, W);
+    o.c = __shfl_up(x.c, d, W);
+    o.d = __shfl_up(x.d, d, W);
+    if (wl >= d) x = mob_mul(x, o);
+  }
+  Mob pre = {1.0, 0.0, 0.0, 1.0};
+  if (T > 64) {
+    const int w = lane >> 6, wave = threadIdx.x >> 6;
+    if (wl == 63) {
+      double *s = red + 4 * wave;
+      s[0] = x.a; s[1] = x.b; s[2] = x.c; s[3] = x.d;
+    }
+    __syncthreads();
+    for (int k = 0; k < w; k++) {
+      const double *s = red + 4 * (wave - w + k);
+      pre = mob_mul(Mob{s[0], s[1], s[2], s[3]}, pre);
+    }
+    __syncthreads();
+  }
+  Mob e;
+  e.a = __shfl_up(x.a, 1, W);
+  e.b = __shfl_up(x.b, 1, W);
+  e.c = __shfl_up(x.c, 1, W);
+  e.d = __shfl_up(x.d, 1, W);
+  if (wl == 0) e = pre;
+  else if (T > 64) e = mob_mul(e, pre);
+  return e;
+}
+
+__device__ Aff team_scan_aff_fwd(Aff x, int lane, int T, double *red) {
+  const int W = T < 64 ? T : 64;
+  const int wl = lane & (W - 1);
+  for (int d = 1; d < W; d <<= 1) {
+    const Aff o = {__shfl_up(x.a, d, W), __shfl_up(x.b, d, W)};
+    if (wl >= d) x = aff_cmp(x, o);
+  }
+  Aff pre = {1.0, 0.0};
+  if (T > 64) {
+    const int w = lane >> 6, wave = threadIdx.x >> 6;
+    if (wl == 63) {
+      red[2 * wave] = x.a;
+      red[2 * wave + 1] = x.b;
+    }
+    __syncthreads();
+    for (int k = 0; k < w; k++) pre = aff_cmp(Aff{red[2 * (wave - w + k)], red[2 * (wave - w + k) + 1]}, pre);
+    __syncthreads();
+  }
+  Aff e = {__shfl_up(x.a, 1, W), __shfl_up(x.b, 1, W)};
+  if (wl == 0) e = pre;
+  else if (T > 64) e = aff_cmp(e, pre);
+  return e;
+}
+
+__device__ Aff team_scan_aff_bwd(Aff x, int lane, int T, double *red) {
+  const int W = T < 64 ? T : 64;
+  const int wl = lane & (W - 1);
+  for (int d = 1; d < W; d <<= 1) {
+    const Aff o = {__shfl_down(x.a, d, W), __shfl_down(x.b, d, W)};
+    if (wl + d < W) x = aff_cmp(x, o);
+  }
+  Aff suf = {1.0, 0.0};
+  if (T > 64) {
+    const int w = lane >> 6, nw = T >> 6, wave = threadIdx.x >> 6;
+    if (wl == 0) {
+      red[2 * wave] = x.a;
+      red[2 * wave + 1] = x.b;
+    }
+    __syncthreads();
+    for (int k = nw - 1; k > w; k--) suf = aff_cmp(Aff{red[2 * (wave - w + k)], red[2 * (wave - w + k) + 1]}, suf);
+    __syncthreads();
+  }
+  Aff e = {__shfl_down(x.a, 1, W), __shfl_down(x.b, 1, W)};
+  if (wl == W - 1) e = suf;
+  else if (T > 64) e = aff_cmp(e, suf);
+  return e;
+}
+
+// (a, b) of the previous lane of the team (dir = +1) or of the next one
+// (dir = -1); the first (last) lane of the team keeps its own values.
+__device__ __forceinline__ void team_shift2(double &a, double &b, int lane, int T, int dir, double *red) {
+  const int W = T < 64 ? T : 64;
+  const int wl = lane & (W - 1);
+  double pa, pb;
+  if (dir > 0) {
+    pa = __shfl_up(a, 1, W);
+    pb = __shfl_up(b, 1, W);
+  } else {
+    pa = __shfl_down(a, 1, W);
+    pb = __shfl_down(b, 1, W);
+  }
+  if (T > 64) {
+    const int wave = threadIdx.x >> 6;
+    const bool edge_out = dir > 0 ? (wl == 63) : (wl == 0);
+    if (edge_out) {
+      red[2 * wave] = a;
+      red[2 * wave + 1] = b;
+    }
+    __syncthreads();
+    const bool edge_in = dir > 0 ? (wl == 0 && lane != 0) : (wl == 63 && lane != T - 1);
+    if (edge_in) {
+      pa = red[2 * (wave - dir)];
+      pb = red[2 * (wave - dir) + 1];
+    }
+    __syncthreads();
+  }
+  a = pa;
+  b = pb;
+}
+
+// true on every lane of the wave (T <= 64) or block (T > 64) when `p` holds
+// on any of them
+__device__ __forceinline__ bool team_any(bool p, int T) {
+  if (T > 64) return __syncthreads_or(p) != 0;
+  return __any(p) != 0;
+}
+
+// Solve one team's system.  On entry the n rows of this lane hold
+// (A, B, Cc, X) = (sub-, main, super-diagonal, right-hand side); on exit X
+// holds the solution and B the pivots.  The row before the team's first row
+// is an identity row carrying xfirst; the row after the team's last row (held
+// by the lane with is_last) is one carrying xlast.  Every lane of every team
+// of the block must call this (it may synchronise the block).
+template <int MR>
+__device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], double (&Cc)[MR], double (&X)[MR],
+                                           int n, int lane, int T, bool is_last, double xfirst, double xlast,
+                                           double *red, int *status) {
+  const bool has = n > 0;
+  // c of the row before this lane's first row (the identity row's c = 0 for lane 0)
+  double cin = 0.0, unused = 0.0;
+#pragma unroll
+  for (int i = 0; i < MR; i++)
+    if (i == n - 1) cin = Cc[i];
+  team_shift2(cin, unused, lane, T, +1, red);
+  if (lane == 0) cin = 0.0;
+  // 1a. pivot entering the segment: M_k = [[b_k, -a_k c_{k-1}], [1, 0]] on (b'_{k-1}, 1)
+  Mob P = {1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+  for (int i = 0; i < MR; i++) {
+    if (i < n) {
+      const double ac = A[i] * (i == 0 ? cin : Cc[i > 0 ? i - 1 : 0]);
+      const Mob R = {B[i] * P.a - ac * P.c, B[i] * P.b - ac * P.d, P.a, P.b};
+      P = R;
+    }
+  }
+  P = mob_norm(P);
+  const Mob E = team_scan_mob(P, lane, T, red);
+  double bin = lane == 0 ? 1.0 : (E.a + E.b) / (E.c + E.d);
+  // 1b. x' entering the segment (affine maps along the estimated pivots)
+  Aff F = {lane == 0 ? 0.0 : 1.0, lane == 0 ? xfirst : 0.0};
+  {
+    double b = bin, c = cin;
+#pragma unroll
+    for (int i = 0; i < MR; i++) {
+      if (i < n) {
+        const double fac = A[i] / b;
+        b = B[i] - fac * c;
+        c = Cc[i];
+        F = Aff{-(fac * F.a), X[i] - fac * F.b};
+      }
+    }
+  }
+  const Aff Ef = team_scan_aff_fwd(F, lane, T, red);
+  double xin = lane == 0 ? xfirst : Ef.a * xfirst + Ef.b;
+  // 2. forward sweep: pass until every entry state equals its predecessor's end
+  for (int it = 0;; it++) {
+    double b = bin, x = xin, c = cin;
+#pragma unroll
+    for (int i = 0; i < MR; i++) {
+      if (i < n) {
+        const double fac = A[i] / b;
+        b = B[i] - fac * c;
+        x = X[i] - fac * x;
+        c = Cc[i];
+      }
+    }
+    team_shift2(b, x, lane, T, +1, red);
+    const bool changed = has && lane != 0 && !(b == bin && x == xin);
+    if (lane != 0) {
+      bin = b;
+      xin = x;
+    }
+    if (!team_any(changed, T) || it >= T) break;
+  }
+  bool zp = false;
+  {
+    double b = bin, x = xin, c = cin;
+#pragma unroll
+    for (int i = 0; i < MR; i++) {
+      if (i < n) {
+        zp = zp || (b == 0.0);
+        const double fac = A[i] / b;
+        b = B[i] - fac * c;
+        x = X[i] - fac * x;
+        c = Cc[i];
+        B[i] = b;
+        X[i] = x;
+      }
+    }
+  }
+  // 3. back substitution from the top: estimate of x entering from above
+  Aff Gm = {is_last ? 0.0 : 1.0, is_last ? xlast : 0.0};
+#pragma unroll
+  for (int i = MR - 1; i >= 0; i--) {
+    if (i < n) {
+      const double rb = 1.0 / B[i];
+      Gm = Aff{-(Cc[i] * rb) * Gm.a, (X[i] - Cc[i] * Gm.b) * rb};
+    }
+  }
+  const Aff Eb = team_scan_aff_bwd(Gm, lane, T, red);
+  double yin = is_last ? xlast : Eb.a * xlast + Eb.b;
+  for (int it = 0;; it++) {
+    double x = yin;
+#pragma unroll
+    for (int i = MR - 1; i >= 0; i--)
+      if (i < n) x = (X[i] - Cc[i] * x) / B[i];
+    double dummy = 0.0;
+    team_shift2(x, dummy, lane, T, -1, red);
+    const bool changed = has && !is_last && !(x == yin);
+    if (!is_last) yin = x;
+    if (!team_any(changed, T) || it >= T) break;
+  }
+  {
+    double x = yin;
+#pragma unroll
+    for (int i = MR - 1; i >= 0; i--) {
+      if (i < n) {
+        zp = zp || (B[i] == 0.0);
+        x = (X[i] - Cc[i] * x) / B[i];
+        X[i] = x;
+      }
+    }
+  }
+  if (zp && has) atomicOr(status, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Component-wise WENO5 / CRWENO5 reconstruction.  One system per (line,
+// variable, reconstruction r): r = 0 uL, 1 uR (of the state), 2 fL, 3 fR (of
+// the flux); r even = left-biased.  A team of lanes covers the interfaces of
+// one system: lane j computes rows 1 + j*m .. of the interior rows 1..N-1,
+// lane 0 also interface 0 and the lane holding row N-1 also interface N.
+//  * weights: WENOFifthOrderCalculateWeights{JS,M,Z,YC}
+//    (WENOFifthOrderCalculateWeights.c:112-237 and the variants);
+//    optimal weights :202-220 (WENO5 values at the two end interfaces of a
+//    CRWENO line).
+//  * WENO5: Interp1PrimFifthOrderWENO.c:124-166.
+//  * CRWENO5: row assembly Interp1PrimFifthOrderCRWENO.c:132-201 (identity
+//    rows with WENO5 right-hand sides at interfaces 0 and N), then the line
+//    solve (team_solve above).
+// ---------------------------------------------------------------------------
+template <int CR, int KIND>
+__device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, int lane, double *red) {
+  const SweepGeom &S = dir ? C.Sy : C.Sx;
+  const int nv = C.G.nv, N = S.N, nl = S.nlines, m = S.rows, g = S.g;
+  const bool active = sys < (long)nl * nv * 4;
+  const long sy = active ? sys : 0;
+  const int r = (int)(sy & 3), v = (int)((sy >> 2) % nv), l = (int)((sy >> 2) / nv);
+  const bool left = !(r & 1);
+  const double *src = ((r < 2) ? (dir ? C.UT : C.U) : (dir ? C.flux_y : C.flux_x)) + (long)v * S.plane + S.base +
+                      (long)l * S.line_stride;
+  double *out = (dir ? C.iy[r] : C.ix[r]) + ((long)v * nl + l) * (N + 1);
+  const int lo = 1 + lane * m;
+  int n = N - lo;
+  n = n < 0 ? 0 : (n > m ? m : n);
+  if (!active) n = 0;
+  const int lastlane = (N - 2) / m;
+  const int hi_pos = N + g - 1;
+  // stencil window: positions lo-3 .. lo+n+1
+  double win[kMaxRows + 5];
+#pragma unroll
+  for (int j = 0; j < kMaxRows + 5; j++) {
+    const int pos = lo - 3 + j;
+    win[j] = (n > 0 && j < n + 5 && pos <= hi_pos) ? src[pos] : 0.0;
+  }
+  double A[kMaxRows], B[kMaxRows], Cc[kMaxRows], X[kMaxRows];
+#pragma unroll
+  for (int i = 0; i < kMaxRows; i++) {
+    if (i < n) {
+      row_coeffs<CR, KIND>(lo + i, N, left, C.R, win[i], win[i + 1], win[i + 2], win[i + 3], win[i + 4], win[i + 5],
+                           A[i], B[i], Cc[i], X[i]);
+    } else {
+      A[i] = 0.0; B[i] = 1.0; Cc[i] = 0.0; X[i] = 0.0;
+    }
+  }
+  // the end interfaces 0 and N (identity rows carrying WENO5 values)
+  double x0 = 0.0, xN = 0.0;
+  for (int e = 0; e < 2; e++) {
+    const bool mine = active && (e == 0 ? lane == 0 : lane == lastlane);
+    if (mine) {
+      const int k = e == 0 ? 0 : N;
+      double w[6];
+#pragma unroll
+      for (int j = 0; j < 6; j++) w[j] = src[k - 3 + j];
+      double a_, b_, c_, x_;
+      row_coeffs<CR, KIND>(k, N, left, C.R, w[0], w[1], w[2], w[3], w[4], w[5], a_, b_, c_, x_);
+      if (e == 0) x0 = x_; else xN = x_;
+    }
+  }
+  if (CR) team_solve<kMaxRows>(A, B, Cc, X, n, lane, S.team, lane == lastlane, x0, xN, red, C.status);
+#pragma unroll
+  for (int i = 0; i < kMaxRows; i++)
+    if (i < n) out[lo + i] = X[i];
+  if (active && lane == 0) out[0] = x0;
+  if (active && lane == lastlane) out[N] = xN;
+}
+
 // Characteristic WENO5 for euler1d, one thread per (interface, line).
 // WENOFifthOrderCalculateWeightsChar* (WENOFifthOrderCalculateWeights.c:761-930)
 // and Interp1PrimFifthOrderWENOChar.c:127-195: L and R from the Roe average of
 // the two cells adjacent to the interface; species rows are the identity, so
 // species reconstruct component-wise.
 template <int KIND>
-__global__ void __launch_bounds__(TB) k_recon_char_e1d(SweepGeom S, int nv, ReconParams R, double gam,
-                                                       const double *__restrict__ q, const double *__restrict__ f,
-                                                       double *__restrict__ uL, double *__restrict__ uR,
-                                                       double *__restrict__ fL, double *__restrict__ fR) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void recon_char_e1d(const SweepGeom &S, int nv, const ReconParams &R, double gam, int l,
+                                               int k, const double *__restrict__ q, const double *__restrict__ f,
+                                               double *const *io) {
   const int nl = S.nlines, N = S.N;
-  if (t >= (long)(N + 1) * nl) return;
-  const int l = (int)(t % nl), k = (int)(t / nl);
-  const long ps = S.pos_stride, pl = S.plane;
-  const long base = S.line_off + l;
-  auto at = [&](const double *a, int v, int pos) { return a[v * pl + (long)(pos + S.g) * ps + base]; };
+  const long pl = S.plane;
+  const long base = S.base + (long)l * S.line_stride;
+  auto at = [&](const double *a, int v, int pos) { return a[v * pl + base + pos]; };
   double a0, a1, a2;
   euler1d_roeavg(gam, at(q, 0, k - 1), at(q, 1, k - 1), at(q, 2, k - 1), at(q, 0, k), at(q, 1, k), at(q, 2, k),
                  a0, a1, a2);
@@ -552,7 +704,7 @@ __global__ void __launch_bounds__(TB) k_recon_char_e1d(SweepGeom S, int nv, Reco
   for (int r = 0; r < 4; r++) {
     const double *src = (r < 2) ? q : f;
     const bool left = !(r & 1);
-    double *out = (r == 0) ? uL : (r == 1) ? uR : (r == 2) ? fL : fR;
+    double *out = io[r];
     int pos[5];
     if (left) { pos[0] = k - 3; pos[1] = k - 2; pos[2] = k - 1; pos[3] = k; pos[4] = k + 1; }
     else { pos[0] = k + 2; pos[1] = k + 1; pos[2] = k; pos[3] = k - 1; pos[4] = k - 2; }
@@ -572,7 +724,7 @@ __global__ void __launch_bounds__(TB) k_recon_char_e1d(SweepGeom S, int nv, Reco
       fch[vv] = w1 * f1 + w2 * f2 + w3 * f3;
     }
     for (int vv = 0; vv < 3; vv++)
-      out[((long)vv * (N + 1) + k) * nl + l] = E.R[vv][0] * fch[0] + E.R[vv][1] * fch[1] + E.R[vv][2] * fch[2];
+      out[((long)vv * nl + l) * (N + 1) + k] = E.R[vv][0] * fch[0] + E.R[vv][1] * fch[1] + E.R[vv][2] * fch[2];
     for (int m = 3; m < nv; m++) {
       const double m3 = at(src, m, pos[0]), m2 = at(src, m, pos[1]), m1 = at(src, m, pos[2]);
       const double p1 = at(src, m, pos[3]), p2 = at(src, m, pos[4]);
@@ -580,8 +732,48 @@ __global__ void __launch_bounds__(TB) k_recon_char_e1d(SweepGeom S, int nv, Reco
       if (R.no_limiting) { w1 = 0.1; w2 = 0.6; w3 = 0.3; }
       else weno_w<KIND>(0.1, 0.6, 0.3, m3, m2, m1, p1, p2, R.eps, w1, w2, w3);
       weno5_cands(m3, m2, m1, p1, p2, f1, f2, f3);
-      out[((long)m * (N + 1) + k) * nl + l] = w1 * f1 + w2 * f2 + w3 * f3;
+      out[((long)m * nl + l) * (N + 1) + k] = w1 * f1 + w2 * f2 + w3 * f3;
     }
+  }
+}
+
+// Blocks of a sweep for the team-based reconstruction (each block holds
+// 256/team whole teams; a block never mixes sweeps).
+__host__ __device__ inline int recon_sweep_blocks(const SweepGeom &S, int nv) {
+  const long teams_per_block = 256 / S.team;
+  const long nsys = (long)S.nlines * nv * 4;
+  return (int)((nsys + teams_per_block - 1) / teams_per_block);
+}
+
+__global__ void __launch_bounds__(256) k_recon(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  if (C.R.characteristic) {
+    const SweepGeom &S = C.Sx;
+    if (t >= (long)(S.N + 1) * S.nlines) return;
+    const int k = (int)(t % (S.N + 1)), l = (int)(t / (S.N + 1));
+    switch (C.R.kind) {
+      case 1: recon_char_e1d<1>(S, C.G.nv, C.R, C.P.gamma, l, k, C.U, C.flux_x, C.ix); break;
+      case 2: recon_char_e1d<2>(S, C.G.nv, C.R, C.P.gamma, l, k, C.U, C.flux_x, C.ix); break;
+      case 3: recon_char_e1d<3>(S, C.G.nv, C.R, C.P.gamma, l, k, C.U, C.flux_x, C.ix); break;
+      default: recon_char_e1d<0>(S, C.G.nv, C.R, C.P.gamma, l, k, C.U, C.flux_x, C.ix); break;
+    }
+    return;
+  }
+  __shared__ double red[16];
+  const int bid = (int)(blockIdx.x - firstblock[cidx]);
+  const int nbx = recon_sweep_blocks(C.Sx, C.G.nv);
+  const int dir = bid < nbx ? 0 : 1;
+  const int lb = dir ? bid - nbx : bid;
+  const int T = dir ? C.Sy.team : C.Sx.team;
+  const int lane = threadIdx.x & (T - 1);
+  const long sys = (long)lb * (256 / T) + threadIdx.x / T;
+  const int cr = C.R.crweno;
+  switch (C.R.kind) {
+    case 1: cr ? recon_team<1, 1>(C, dir, sys, lane, red) : recon_team<0, 1>(C, dir, sys, lane, red); break;
+    case 2: cr ? recon_team<1, 2>(C, dir, sys, lane, red) : recon_team<0, 2>(C, dir, sys, lane, red); break;
+    case 3: cr ? recon_team<1, 3>(C, dir, sys, lane, red) : recon_team<0, 3>(C, dir, sys, lane, red); break;
+    default: cr ? recon_team<1, 0>(C, dir, sys, lane, red) : recon_team<0, 0>(C, dir, sys, lane, red); break;
   }
 }
 
@@ -594,16 +786,14 @@ __global__ void __launch_bounds__(TB) k_recon_char_e1d(SweepGeom S, int nv, Reco
 //  Euler1D Roe: Euler1DUpwind.c:27-93 (no entropy fix); rf-char :107-190.
 // ---------------------------------------------------------------------------
 template <int DIR>
-__global__ void __launch_bounds__(TB) k_upwind_ns2d_roe(SweepGeom S, int nv, double g, const double *__restrict__ q,
-                                                        const double *__restrict__ uL, const double *__restrict__ uR,
-                                                        const double *__restrict__ fL, const double *__restrict__ fR,
-                                                        double *__restrict__ fI) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void upwind_ns2d_roe(const SweepGeom &S, int nv, double g, long t,
+                                                const double *__restrict__ q, double *const *io,
+                                                double *__restrict__ fI) {
+  const double *uL = io[0], *uR = io[1], *fL = io[2], *fR = io[3];
   const int nl = S.nlines, N = S.N;
-  if (t >= (long)(N + 1) * nl) return;
-  const int l = (int)(t % nl), k = (int)(t / nl);
-  const long ps = S.pos_stride, pl = S.plane;
-  const long cL = (long)(k - 1 + S.g) * ps + S.line_off + l, cR = cL + ps;
+  const int k = (int)(t % (N + 1)), l = (int)(t / (N + 1));
+  const long pl = S.plane;
+  const long cL = S.base + (long)l * S.line_stride + (k - 1), cR = cL + 1;
   // Roe average of the cell-centred neighbours (navierstokes2d.h:115-160)
   const double rhoL = q[cL], vxL = q[pl + cL] / rhoL, vyL = q[2 * pl + cL] / rhoL, eL = q[3 * pl + cL];
   const double vsqL = (vxL * vxL) + (vyL * vyL), PL = (eL - 0.5 * rhoL * vsqL) * (g - 1.0);
@@ -630,7 +820,7 @@ __global__ void __launch_bounds__(TB) k_upwind_ns2d_roe(SweepGeom S, int nv, dou
   else { D[1] = vn; D[2] = vn + a; }
   D[3] = vn;
   const double delta = 0.000001, delta2 = delta * delta;
-  double Dsp = (fabs(vn) < delta ? (vn * vn + delta2) / (2 * delta) : fabs(vn));
+  const double Dsp = (fabs(vn) < delta ? (vn * vn + delta2) / (2 * delta) : fabs(vn));
 #pragma unroll
   for (int m = 0; m < 4; m++) D[m] = (fabs(D[m]) < delta ? (D[m] * D[m] + delta2) / (2 * delta) : fabs(D[m]));
   double L[4][4], R[4][4];
@@ -681,8 +871,7 @@ __global__ void __launch_bounds__(TB) k_upwind_ns2d_roe(SweepGeom S, int nv, dou
     R[0][cp] = 1.0; R[1][cp] = vx + a * nx; R[2][cp] = vy + a * ny; R[3][cp] = h0 + a * un;
     R[0][cz] = 0.0; R[1][cz] = ny; R[2][cz] = -nx; R[3][cz] = vx * ny - vy * nx;
   }
-  // interface data, flow block
-  const long ii = (long)k * nl + l, vstride = (long)(N + 1) * nl;
+  const long ii = (long)l * (N + 1) + k, vstride = (long)nl * (N + 1);
   double ud[4], fo[4];
 #pragma unroll
   for (int m = 0; m < 4; m++) ud[m] = 0.5 * (uR[m * vstride + ii] - uL[m * vstride + ii]);
@@ -710,23 +899,19 @@ __global__ void __launch_bounds__(TB) k_upwind_ns2d_roe(SweepGeom S, int nv, dou
   }
 }
 
-template <int RF>
-__global__ void __launch_bounds__(TB) k_upwind_e1d(SweepGeom S, int nv, double g, const double *__restrict__ q,
-                                                   const double *__restrict__ uL, const double *__restrict__ uR,
-                                                   const double *__restrict__ fL, const double *__restrict__ fR,
-                                                   double *__restrict__ fI) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g, int rf, long t,
+                                           const double *__restrict__ q, double *const *io, double *__restrict__ fI) {
+  const double *uL = io[0], *uR = io[1], *fL = io[2], *fR = io[3];
   const int nl = S.nlines, N = S.N;
-  if (t >= (long)(N + 1) * nl) return;
-  const int l = (int)(t % nl), k = (int)(t / nl);
-  const long ps = S.pos_stride, pl = S.plane;
-  const long cL = (long)(k - 1 + S.g) * ps + S.line_off + l, cR = cL + ps;
+  const int k = (int)(t % (N + 1)), l = (int)(t / (N + 1));
+  const long pl = S.plane;
+  const long cL = S.base + (long)l * S.line_stride + (k - 1), cR = cL + 1;
   double a0, a1, a2;
   euler1d_roeavg(g, q[cL], q[pl + cL], q[2 * pl + cL], q[cR], q[pl + cR], q[2 * pl + cR], a0, a1, a2);
   Eig3 E;
   euler1d_eig(g, a0, a1, a2, E);
-  const long ii = (long)k * nl + l, vs = (long)(N + 1) * nl;
-  if (!RF) {
+  const long ii = (long)l * (N + 1) + k, vs = (long)nl * (N + 1);
+  if (!rf) {
     double ud[3], DL[3][3];
     for (int m = 0; m < 3; m++) ud[m] = 0.5 * (uR[m * vs + ii] - uL[m * vs + ii]);
     for (int i = 0; i < 3; i++)
@@ -744,80 +929,65 @@ __global__ void __launch_bounds__(TB) k_upwind_e1d(SweepGeom S, int nv, double g
       const double udm = 0.5 * (uR[m * vs + ii] - uL[m * vs + ii]);
       fI[m * vs + ii] = 0.5 * (fL[m * vs + ii] + fR[m * vs + ii]) - Dsp * udm;
     }
+    return;
+  }
+  // cell eigenvalues (euler1d.h:131-147) of the two neighbours
+  Eig3 EL, ER;
+  euler1d_eig(g, q[cL], q[pl + cL], q[2 * pl + cL], EL);
+  euler1d_eig(g, q[cR], q[pl + cR], q[2 * pl + cR], ER);
+  double fc[3];
+  for (int m = 0; m < 3; m++) {
+    double ucL = 0, ucR = 0, fcL = 0, fcR = 0;
+    for (int j = 0; j < 3; j++) {
+      ucL += E.L[m][j] * uL[j * vs + ii];
+      ucR += E.L[m][j] * uR[j * vs + ii];
+      fcL += E.L[m][j] * fL[j * vs + ii];
+      fcR += E.L[m][j] * fR[j * vs + ii];
+    }
+    const double eL = EL.D[m], eC = E.D[m], eR = ER.D[m];
+    if (eL > 0 && eC > 0 && eR > 0) fc[m] = fcL;
+    else if (eL < 0 && eC < 0 && eR < 0) fc[m] = fcR;
+    else {
+      const double x1 = fabs(eL), x2 = fabs(eC), x3 = fabs(eR);
+      const double mx1 = x1 > x2 ? x1 : x2, mx2 = x2 > x3 ? x2 : x3;
+      const double alpha = mx1 > mx2 ? mx1 : mx2;
+      fc[m] = 0.5 * (fcL + fcR + alpha * (ucL - ucR));
+    }
+  }
+  for (int i = 0; i < 3; i++) fI[i * vs + ii] = E.R[i][0] * fc[0] + E.R[i][1] * fc[1] + E.R[i][2] * fc[2];
+  // species rows: identity eigenvectors, eigenvalue v
+  const double eLs = EL.D[0], eCs = E.D[0], eRs = ER.D[0];
+  for (int m = 3; m < nv; m++) {
+    const double ucL = uL[m * vs + ii], ucR = uR[m * vs + ii], fcL = fL[m * vs + ii], fcR = fR[m * vs + ii];
+    double r;
+    if (eLs > 0 && eCs > 0 && eRs > 0) r = fcL;
+    else if (eLs < 0 && eCs < 0 && eRs < 0) r = fcR;
+    else {
+      const double x1 = fabs(eLs), x2 = fabs(eCs), x3 = fabs(eRs);
+      const double mx1 = x1 > x2 ? x1 : x2, mx2 = x2 > x3 ? x2 : x3;
+      r = 0.5 * (fcL + fcR + (mx1 > mx2 ? mx1 : mx2) * (ucL - ucR));
+    }
+    fI[m * vs + ii] = r;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_upwind(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const long nX = (long)(C.Sx.N + 1) * C.Sx.nlines;
+  const long nY = (C.G.ndims == 2) ? (long)(C.Sy.N + 1) * C.Sy.nlines : 0;
+  if (t >= nX + nY) return;
+  if (C.model == 1) {
+    upwind_e1d(C.Sx, C.G.nv, C.P.gamma, C.upwind, t, C.U, C.ix, C.fIx);
+  } else if (t < nX) {
+    upwind_ns2d_roe<0>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
   } else {
-    // cell eigenvalues (euler1d.h:131-147) of the two neighbours
-    Eig3 EL, ER;
-    euler1d_eig(g, q[cL], q[pl + cL], q[2 * pl + cL], EL);
-    euler1d_eig(g, q[cR], q[pl + cR], q[2 * pl + cR], ER);
-    double fc[3];
-    for (int m = 0; m < 3; m++) {
-      double ucL = 0, ucR = 0, fcL = 0, fcR = 0;
-      for (int j = 0; j < 3; j++) {
-        ucL += E.L[m][j] * uL[j * vs + ii];
-        ucR += E.L[m][j] * uR[j * vs + ii];
-        fcL += E.L[m][j] * fL[j * vs + ii];
-        fcR += E.L[m][j] * fR[j * vs + ii];
-      }
-      const double eL = EL.D[m], eC = E.D[m], eR = ER.D[m];
-      if (eL > 0 && eC > 0 && eR > 0) fc[m] = fcL;
-      else if (eL < 0 && eC < 0 && eR < 0) fc[m] = fcR;
-      else {
-        const double x1 = fabs(eL), x2 = fabs(eC), x3 = fabs(eR);
-        const double mx1 = x1 > x2 ? x1 : x2, mx2 = x2 > x3 ? x2 : x3;
-        const double alpha = mx1 > mx2 ? mx1 : mx2;
-        fc[m] = 0.5 * (fcL + fcR + alpha * (ucL - ucR));
-      }
-    }
-    for (int i = 0; i < 3; i++) fI[i * vs + ii] = E.R[i][0] * fc[0] + E.R[i][1] * fc[1] + E.R[i][2] * fc[2];
-    // species rows: identity eigenvectors, eigenvalue v
-    const double eLs = EL.D[0], eCs = E.D[0], eRs = ER.D[0];
-    for (int m = 3; m < nv; m++) {
-      const double ucL = uL[m * vs + ii], ucR = uR[m * vs + ii], fcL = fL[m * vs + ii], fcR = fR[m * vs + ii];
-      double r;
-      if (eLs > 0 && eCs > 0 && eRs > 0) r = fcL;
-      else if (eLs < 0 && eCs < 0 && eRs < 0) r = fcR;
-      else {
-        const double x1 = fabs(eLs), x2 = fabs(eCs), x3 = fabs(eRs);
-        const double mx1 = x1 > x2 ? x1 : x2, mx2 = x2 > x3 ? x2 : x3;
-        r = 0.5 * (fcL + fcR + (mx1 > mx2 ? mx1 : mx2) * (ucL - ucR));
-      }
-      fI[m * vs + ii] = r;
-    }
+    upwind_ns2d_roe<1>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.UT, C.iy, C.fIy);
   }
 }
 
-// hyp = dxinv * (fI[k+1] - fI[k]): HyperbolicFunction.c:97-112.  Output is the
-// compact interior array h[(v*N + k)*nlines + l].
-__global__ void __launch_bounds__(TB) k_divergence(SweepGeom S, int nv, const double *__restrict__ dxinv,
-                                                   const double *__restrict__ fI, double *__restrict__ h) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const int nl = S.nlines, N = S.N;
-  if (t >= (long)N * nl) return;
-  const int l = (int)(t % nl), k = (int)(t / nl);
-  const double di = dxinv[S.g + k];
-  for (int v = 0; v < nv; v++) {
-    const long a = ((long)v * (N + 1) + k) * nl + l;
-    h[((long)v * N + k) * nl + l] = di * (fI[a + nl] - fI[a]);
-  }
-}
-
-// [v][rows][cols] -> [v][cols][rows]
-__global__ void __launch_bounds__(256) k_transpose(int rows, int cols, const double *__restrict__ src,
-                                                   double *__restrict__ dst) {
-  __shared__ double tile[32][33];
-  const int v = blockIdx.z;
-  const long off = (long)v * rows * cols;
-  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
-  for (int y = threadIdx.y; y < 32; y += 8) {
-    const int r = r0 + y, c = c0 + threadIdx.x;
-    if (r < rows && c < cols) tile[y][threadIdx.x] = src[off + (long)r * cols + c];
-  }
-  __syncthreads();
-  for (int y = threadIdx.y; y < 32; y += 8) {
-    const int c = c0 + y, r = r0 + threadIdx.x;
-    if (r < rows && c < cols) dst[off + (long)c * rows + r] = tile[threadIdx.x][y];
-  }
-}
+// hyp = dxinv * (fI[k+1] - fI[k]) along x: HyperbolicFunction.c:97-112.
+// Output is the compact interior array h[(v*N + k)*nlines + l] (transposed in 2D).
 
 // ---------------------------------------------------------------------------
 // Parabolic term (navierstokes2d): NavierStokes2DParabolicFunction.c:41-204.
@@ -828,10 +998,10 @@ __global__ void __launch_bounds__(256) k_transpose(int rows, int cols, const dou
 struct Prim {
   double vx, vy, T;
 };
-__device__ __forceinline__ Prim prim_at(const Grid &G, double gam, const double *U, long p) {
+__device__ __forceinline__ Prim prim_at(long plane, double gam, const double *U, long p) {
   const double rho = U[p];
-  const double vx = (rho == 0) ? 0 : U[G.plane + p] / rho, vy = (rho == 0) ? 0 : U[2 * G.plane + p] / rho;
-  const double e = U[3 * G.plane + p];
+  const double vx = (rho == 0) ? 0 : U[plane + p] / rho, vy = (rho == 0) ? 0 : U[2 * plane + p] / rho;
+  const double e = U[3 * plane + p];
   const double vsq = (vx * vx) + (vy * vy);
   const double P = (e - 0.5 * rho * vsq) * (gam - 1.0);
   Prim r;
@@ -844,10 +1014,9 @@ __device__ __forceinline__ Prim prim_at(const Grid &G, double gam, const double 
 // FirstDerivativeFourthOrder.c:39-132 (PAR4) / FirstDerivativeFirstOrder.c (bias +1)
 // of the three primitives at position i of a line of N interior points.
 template <int PAR4>
-__device__ __forceinline__ void deriv_prim(const Grid &G, double gam, const double *U, long p0, long st, int i,
+__device__ __forceinline__ void deriv_prim(long plane, int g, double gam, const double *U, long p0, long st, int i,
                                            int N, double out[3]) {
-  const int g = G.g;
-  auto P = [&](int k) { return prim_at(G, gam, U, p0 + (long)k * st); };
+  auto P = [&](int k) { return prim_at(plane, gam, U, p0 + (long)k * st); };
   if (PAR4) {
     const double t12 = 1.0 / 12.0;
     if (i == -g) {
@@ -894,44 +1063,51 @@ __device__ __forceinline__ void deriv_prim(const Grid &G, double gam, const doub
 
 // Qd planes: 0 ux, 1 vx, 2 Tx (d/dx), 3 uy, 4 vy, 5 Ty (d/dy), each scaled by
 // dxinv (NavierStokes2DParabolicFunction.c:96-106).
-template <int PAR4>
-__global__ void __launch_bounds__(TB) k_qderiv(Grid G, double gam, const double *__restrict__ U,
-                                               const double *__restrict__ dxinv, double *__restrict__ Qd) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_qderiv(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const Grid &G = C.G;
   if (t >= (long)G.nxg * G.nyg) return;
   const int i = (int)(t % G.nxg) - G.g, j = (int)(t / G.nxg) - G.g;
-  const long p = t;
   const long pl = (long)G.nxg * G.nyg;
+  const double gam = C.P.gamma;
   double dx[3] = {0, 0, 0}, dy[3] = {0, 0, 0};
   if (j >= 0 && j < G.ny) {
     const long p0 = (long)(j + G.g) * G.nxg + G.g;  // (0, j)
-    deriv_prim<PAR4>(G, gam, U, p0, 1, i, G.nx, dx);
-    const double s = dxinv[G.g + i];
+    if (C.P.par4) deriv_prim<1>(G.plane, G.g, gam, C.U, p0, 1, i, G.nx, dx);
+    else deriv_prim<0>(G.plane, G.g, gam, C.U, p0, 1, i, G.nx, dx);
+    const double s = C.dxinv[G.g + i];
     dx[0] *= s; dx[1] *= s; dx[2] *= s;
   }
   if (i >= 0 && i < G.nx) {
     const long p0 = (long)G.g * G.nxg + (i + G.g);  // (i, 0)
-    deriv_prim<PAR4>(G, gam, U, p0, G.nxg, j, G.ny, dy);
-    const double s = dxinv[(G.nx + 2 * G.g) + G.g + j];
+    if (C.P.par4) deriv_prim<1>(G.plane, G.g, gam, C.U, p0, G.nxg, j, G.ny, dy);
+    else deriv_prim<0>(G.plane, G.g, gam, C.U, p0, G.nxg, j, G.ny, dy);
+    const double s = C.dxinv[(G.nx + 2 * G.g) + G.g + j];
     dy[0] *= s; dy[1] *= s; dy[2] *= s;
   }
-  Qd[p] = dx[0]; Qd[pl + p] = dx[1]; Qd[2 * pl + p] = dx[2];
-  Qd[3 * pl + p] = dy[0]; Qd[4 * pl + p] = dy[1]; Qd[5 * pl + p] = dy[2];
+  double *Qd = C.Qd;
+  Qd[t] = dx[0]; Qd[pl + t] = dx[1]; Qd[2 * pl + t] = dx[2];
+  Qd[3 * pl + t] = dy[0]; Qd[4 * pl + t] = dy[1]; Qd[5 * pl + t] = dy[2];
 }
 
 // Viscous fluxes (NavierStokes2DParabolicFunction.c:112-143 and :155-186),
 // Sutherland viscosity and the conductivity law of navierstokes2d.h:315-331
 // (raiseto(x,1.5) = exp(1.5 log x), include/math_ops.h:40).
 // FV planes: 0..2 = x-flux components 1..3, 3..5 = y-flux components 1..3.
-__global__ void __launch_bounds__(TB) k_viscflux(Grid G, PhysParams P, const double *__restrict__ U,
-                                                 const double *__restrict__ Qd, double *__restrict__ FV) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_viscflux(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const Grid &G = C.G;
+  const PhysParams &P = C.P;
   if (t >= (long)G.nxg * G.nyg) return;
   const int i = (int)(t % G.nxg) - G.g, j = (int)(t / G.nxg) - G.g;
   const bool ix = (i >= 0 && i < G.nx), jy = (j >= 0 && j < G.ny);
   if (!ix && !jy) return;  // corners are never read
   const long pl = (long)G.nxg * G.nyg;
-  const Prim q = prim_at(G, P.gamma, U, t);
+  const double *Qd = C.Qd;
+  double *FV = C.FV;
+  const Prim q = prim_at(G.plane, P.gamma, C.U, t);
   const double ux = Qd[t], vx = Qd[pl + t], Tx = Qd[2 * pl + t];
   const double uy = Qd[3 * pl + t], vy = Qd[4 * pl + t], Ty = Qd[5 * pl + t];
   const double Td = q.T * P.Tref;
@@ -954,115 +1130,167 @@ __global__ void __launch_bounds__(TB) k_viscflux(Grid G, PhysParams P, const dou
 
 // ---------------------------------------------------------------------------
 // RHS assembly at interior points: rhs = -hyp + par + source
-// (TimeRHSFunctionExplicit.c:55-74).  hyp: x part from the x-sweep (compact
-// interior, natural order) plus the y divergence; par: second derivative of
-// the viscous fluxes (central stencil at interior points); source:
-// ChemistrySource (ChemistrySolve.c:18-115, chemistry.h:30-127).
+// (TimeRHSFunctionExplicit.c:55-74).  hyp = dxinv (fI[k+1] - fI[k]) summed
+// over the dimensions in order (HyperbolicFunction.c:97-112): the x part from
+// the x-sweep interfaces (contiguous along x), the y part from the y-sweep
+// interfaces, which are contiguous along y and pass through an LDS tile;
+// par: derivatives of the viscous fluxes; source: ChemistrySource
+// (ChemistrySolve.c:18-115, chemistry.h:30-127) with the photon density of
+// the Gaussian pulse at the stage time.
 // ---------------------------------------------------------------------------
-template <int ND, int VISC, int PAR4, int CHEM>
-__global__ void __launch_bounds__(TB) k_assemble(Grid G, PhysParams P, const double *__restrict__ U,
-                                                 const double *__restrict__ hx, const double *__restrict__ fIy,
-                                                 const double *__restrict__ FV, const double *__restrict__ dxinv,
-                                                 const double *__restrict__ imap, double hnu_scale,
-                                                 double *__restrict__ udot) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long nint = (long)G.nx * G.ny;
-  if (t >= nint) return;
-  const int i = (int)(t % G.nx), j = (int)(t / G.nx);
-  const long p = (long)(j + G.gy) * G.nxg + (i + G.g);
-  const int nv = G.nv;
-  double dyi = 0.0;
-  if (ND == 2) dyi = dxinv[(G.nx + 2 * G.g) + G.g + j];
-  // parabolic part for the momentum and energy components
-  double par[4] = {0, 0, 0, 0};
-  if (VISC) {
-    const long pl = (long)G.nxg * G.nyg;
-    const double dxi = dxinv[G.g + i];
-    for (int c = 0; c < 3; c++) {
-      const double *Fx = FV + c * pl, *Fy = FV + (3 + c) * pl;
-      double dfx, dfy;
-      if (PAR4) {
-        const double t12 = 1.0 / 12.0;
-        dfx = (Fx[p - 2] - 8 * Fx[p - 1] + 8 * Fx[p + 1] - Fx[p + 2]) * t12;
-        dfy = (Fy[p - 2 * G.nxg] - 8 * Fy[p - G.nxg] + 8 * Fy[p + G.nxg] - Fy[p + 2 * G.nxg]) * t12;
-      } else {
-        dfx = 0.0 * Fx[p + 1] - (-1.0) * Fx[p] + (-1.0) * Fx[p - 1];
-        dfy = 0.0 * Fy[p + G.nxg] - (-1.0) * Fy[p] + (-1.0) * Fy[p - G.nxg];
-      }
-      par[c + 1] = dxi * dfx;
-      par[c + 1] = par[c + 1] + dyi * dfy;
-    }
-  }
-  // chemistry source: photon density per z-layer, species rates, heating
-  double src_e = 0.0;
-  const int ie = (ND == 2) ? 3 : 2;
-  for (int v = 0; v < nv; v++) {
-    double hyp = hx[(long)v * nint + t];
-    if (ND == 2) {
-      const long a = ((long)v * (G.ny + 1) + j) * G.nx + i;
-      hyp = hyp + dyi * (fIy[a + G.nx] - fIy[a]);
-    }
-    double r = -hyp;
-    if (VISC && v >= 1 && v <= 3) r = r + par[v];
-    else if (ND == 2) r = r + 0.0;
-    if (!CHEM || v < P.nflow) {
-      udot[(long)v * G.plane + p] = r;
+__device__ __forceinline__ void viscous_par(const CaseDev &C, int i, int j, long p, double par[3]) {
+  const Grid &G = C.G;
+  const PhysParams &P = C.P;
+  const long pl = (long)G.nxg * G.nyg;
+  const double dxi = C.dxinv[G.g + i], dyi = C.dxinv[(G.nx + 2 * G.g) + G.g + j];
+  for (int c = 0; c < 3; c++) {
+    const double *Fx = C.FV + c * pl, *Fy = C.FV + (3 + c) * pl;
+    double dfx, dfy;
+    if (P.par4) {
+      const double t12 = 1.0 / 12.0;
+      dfx = (Fx[p - 2] - 8 * Fx[p - 1] + 8 * Fx[p + 1] - Fx[p + 2]) * t12;
+      dfy = (Fy[p - 2 * G.nxg] - 8 * Fy[p - G.nxg] + 8 * Fy[p + G.nxg] - Fy[p + 2 * G.nxg]) * t12;
     } else {
-      udot[(long)v * G.plane + p] = r;  // species source added below
+      dfx = 0.0 * Fx[p + 1] - (-1.0) * Fx[p] + (-1.0) * Fx[p - 1];
+      dfy = 0.0 * Fy[p + G.nxg] - (-1.0) * Fy[p] + (-1.0) * Fy[p - G.nxg];
     }
-  }
-  if (CHEM) {
-    const long pl = G.plane;
-    double nh = (hnu_scale * imap[p]) / P.hnu_den;
-    for (int iz = 0; iz < P.nlayers; iz++) {
-      const int nfv = P.nflow + P.z_stride * iz;
-      if (iz > 0) nh = nh * (1.0 - P.damp_coef * U[(long)(P.nflow + P.z_stride * (iz - 1) + 1) * pl + p]);
-      const double nO2 = U[(long)(nfv + 0) * pl + p], nO3 = U[(long)(nfv + 1) * pl + p];
-      const double n1D = U[(long)(nfv + 2) * pl + p], n1Dg = U[(long)(nfv + 3) * pl + p];
-      const double n3Su = U[(long)(nfv + 4) * pl + p], n1Sg = U[(long)(nfv + 5) * pl + p];
-      const double nCO2 = U[(long)(nfv + 6) * pl + p];
-      double s[7];
-      s[0] = 0.0;
-      s[1] = -(P.k0a + P.k0b) * nh * nO3 - (P.k2a + P.k2b) * nO3 * n1D - (P.k3a + P.k3b) * nO3 * n1Sg -
-             P.k5 * nO3 * n3Su - P.k6 * n1Dg * nO3;
-      s[2] = P.k0a * nh * nO3 - (P.k1a + P.k1b) * n1D * nO2 - (P.k2a + P.k2b) * n1D * nO3 - P.k4 * n1D * nCO2;
-      s[3] = P.k0a * nh * nO3 + P.k5 * nO3 * n3Su - P.k6 * n1Dg * nO3;
-      s[4] = P.k2a * nO3 * n1D - P.k5 * nO3 * n3Su;
-      s[5] = P.k1a * n1D * nO2 - (P.k3a + P.k3b) * nO3 * n1Sg;
-      s[6] = 0.0;
-      for (int m = 0; m < 7; m++) {
-        double *o = udot + (long)(nfv + m) * pl + p;
-        *o = *o + s[m];
-      }
-      src_e = ((P.q0a * P.k0a + P.q0b * P.k0b) * nh * nO3 + (P.q1a * P.k1a + P.q1b * P.k1b) * nO2 * n1D +
-               (P.q2a * P.k2a + P.q2b * P.k2b) * nO3 * n1D + (P.q3a * P.k3a + P.q3b * P.k3b) * nO3 * n1Sg +
-               P.q4 * P.k4 * n1D * nCO2 + P.q5 * P.k5 * nO3 * n3Su + P.q6 * P.k6 * n1Dg * nO3) /
-              P.ch_gm1;
-    }
-    double *o = udot + (long)ie * pl + p;
-    *o = *o + src_e;
+    par[c] = dxi * dfx;
+    par[c] = par[c] + dyi * dfy;
   }
 }
 
-// u += (dt*b_s) * Udot_s, stage by stage (TimeRK.c:86-99), interior points.
-__global__ void __launch_bounds__(TB) k_update(Grid G, double *__restrict__ u, const double *__restrict__ d0,
-                                               const double *__restrict__ d1, const double *__restrict__ d2,
-                                               const double *__restrict__ d3, double b0, double b1, double b2,
-                                               double b3, int nst) {
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+__device__ __forceinline__ void chem_source(const CaseDev &C, int stage, long p, double *udot) {
+  const Grid &G = C.G;
+  const PhysParams &P = C.P;
+  const double *U = C.U;
+  // photon density of the pulse at the stage time (ChemistrySolve.c:36-43)
+  const double time = *C.time + C.c[stage] * C.dt;
+  double hs = 0.0;
+  if (time > P.t_start_norm) {
+    const double sigma = P.t_pulse_norm / 2.35482;
+    const double tp = time - (P.t_start_norm + P.t_pulse_norm);
+    hs = P.I0 * exp(-(tp * tp) / (2 * sigma * sigma));
+  }
+  const long pl = G.plane;
+  double nh = (hs * C.imap[p]) / P.hnu_den;
+  double src_e = 0.0;
+  for (int iz = 0; iz < P.nlayers; iz++) {
+    const int nfv = P.nflow + P.z_stride * iz;
+    if (iz > 0) nh = nh * (1.0 - P.damp_coef * U[(long)(P.nflow + P.z_stride * (iz - 1) + 1) * pl + p]);
+    const double nO2 = U[(long)(nfv + 0) * pl + p], nO3 = U[(long)(nfv + 1) * pl + p];
+    const double n1D = U[(long)(nfv + 2) * pl + p], n1Dg = U[(long)(nfv + 3) * pl + p];
+    const double n3Su = U[(long)(nfv + 4) * pl + p], n1Sg = U[(long)(nfv + 5) * pl + p];
+    const double nCO2 = U[(long)(nfv + 6) * pl + p];
+    double s[7];
+    s[0] = 0.0;
+    s[1] = -(P.k0a + P.k0b) * nh * nO3 - (P.k2a + P.k2b) * nO3 * n1D - (P.k3a + P.k3b) * nO3 * n1Sg -
+           P.k5 * nO3 * n3Su - P.k6 * n1Dg * nO3;
+    s[2] = P.k0a * nh * nO3 - (P.k1a + P.k1b) * n1D * nO2 - (P.k2a + P.k2b) * n1D * nO3 - P.k4 * n1D * nCO2;
+    s[3] = P.k0a * nh * nO3 + P.k5 * nO3 * n3Su - P.k6 * n1Dg * nO3;
+    s[4] = P.k2a * nO3 * n1D - P.k5 * nO3 * n3Su;
+    s[5] = P.k1a * n1D * nO2 - (P.k3a + P.k3b) * nO3 * n1Sg;
+    s[6] = 0.0;
+    for (int m = 0; m < 7; m++) {
+      double *o = udot + (long)(nfv + m) * pl + p;
+      *o = *o + s[m];
+    }
+    src_e = ((P.q0a * P.k0a + P.q0b * P.k0b) * nh * nO3 + (P.q1a * P.k1a + P.q1b * P.k1b) * nO2 * n1D +
+             (P.q2a * P.k2a + P.q2b * P.k2b) * nO3 * n1D + (P.q3a * P.k3a + P.q3b * P.k3b) * nO3 * n1Sg +
+             P.q4 * P.k4 * n1D * nCO2 + P.q5 * P.k5 * nO3 * n3Su + P.q6 * P.k6 * n1Dg * nO3) /
+            P.ch_gm1;
+  }
+  double *o = udot + (long)(P.nflow - 1) * pl + p;
+  *o = *o + src_e;
+}
+
+__global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const Grid &G = C.G;
+  const int nv = G.nv;
+  double *udot = C.Udot[stage];
+  if (G.ndims == 1) {
+    if (t >= G.nx) return;
+    const int i = (int)t;
+    const long p = i + G.g;
+    const double dxi = C.dxinv[G.g + i];
+    for (int v = 0; v < nv; v++) {
+      const long a = (long)v * (G.nx + 1) + i;
+      const double hyp = dxi * (C.fIx[a + 1] - C.fIx[a]);
+      udot[(long)v * G.plane + p] = -hyp;
+    }
+    if (C.P.chem) chem_source(C, stage, p, udot);
+    return;
+  }
+  // 2D: a 32 x 32 tile of points, four per thread
+  __shared__ double tl[32][33];
+  const int ntx = (G.nx + 31) / 32;
+  const int id = (int)(blockIdx.x - firstblock[cidx]);
+  const int i0 = (id % ntx) * 32, j0 = (id / ntx) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int i = i0 + tx;
+  const bool visc = C.P.viscous;
+  double par[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int j = j0 + ty + 8 * q;
+    par[q][0] = par[q][1] = par[q][2] = 0.0;
+    if (visc && i < G.nx && j < G.ny) viscous_par(C, i, j, (long)(j + G.gy) * G.nxg + (i + G.g), par[q]);
+  }
+  const double dxi = i < G.nx ? C.dxinv[G.g + i] : 0.0;
+  for (int v = 0; v < nv; v++) {
+    // y-sweep interfaces of lines i0..i0+31, interfaces j0..j0+32
+    for (int b = ty; b < 32; b += 8) {
+      const int ii = i0 + b;
+      if (ii < G.nx) {
+        const double *row = C.fIy + ((long)v * G.nx + ii) * (G.ny + 1);
+        for (int a = tx; a < 33; a += 32) {
+          const int j = j0 + a;
+          if (j <= G.ny) tl[b][a] = row[j];
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int jl = ty + 8 * q, j = j0 + jl;
+      if (i < G.nx && j < G.ny) {
+        const long a = ((long)v * G.ny + j) * (G.nx + 1) + i;
+        double hyp = dxi * (C.fIx[a + 1] - C.fIx[a]);
+        hyp = hyp + C.dxinv[(G.nx + 2 * G.g) + G.g + j] * (tl[tx][jl + 1] - tl[tx][jl]);
+        double r = -hyp;
+        if (visc && v >= 1 && v <= 3) r = r + par[q][v - 1];
+        udot[(long)v * G.plane + (long)(j + G.gy) * G.nxg + (i + G.g)] = r;
+      }
+    }
+    __syncthreads();
+  }
+  if (C.P.chem) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int j = j0 + ty + 8 * q;
+      if (i < G.nx && j < G.ny) chem_source(C, stage, (long)(j + G.gy) * G.nxg + (i + G.g), udot);
+    }
+  }
+}
+
+// u += (dt*b_s) * Udot_s stage by stage (TimeRK.c:86-99) at interior points;
+// time += dt (TimePostStep.c:33).
+__global__ void __launch_bounds__(256) k_update(CASE_ARGS) {
+  CASE_PREAMBLE
+  const Grid &G = C.G;
   const long nint = (long)G.nx * G.ny;
   if (t >= nint * G.nv) return;
+  if (t == 0) *C.time = *C.time + C.dt;
   const int v = (int)(t / nint);
   const long q = t % nint;
   const int i = (int)(q % G.nx), j = (int)(q / G.nx);
   const long p = (long)v * G.plane + (long)(j + G.gy) * G.nxg + (i + G.g);
-  double x = u[p];
-  x += b0 * d0[p];
-  if (nst > 1) x += b1 * d1[p];
-  if (nst > 2) x += b2 * d2[p];
-  if (nst > 3) x += b3 * d3[p];
-  u[p] = x;
+  double x = C.u[p];
+  for (int s = 0; s < C.nstages; s++) x += (C.dt * C.b[s]) * C.Udot[s][p];
+  C.u[p] = x;
 }
+
 
 // Per-block partial max CFL (Euler1DComputeCFL.c / NavierStokes2DComputeCFL.c)
 // and sum of squared step differences (TimePostStep.c:41-53).
@@ -1113,34 +1341,39 @@ __global__ void __launch_bounds__(TB) k_cfl_norm(Grid G, int model, double gam, 
   }
 }
 
-// Batched Thomas solve, one lane per system: tridiagLU.c:133-150, :243-255.
+
+// Batched tridiagonal solve for the unit-level entry point, one team per
+// system (tridiagLU.c:86-277 semantics with a_0 = c_{n-1} = 0, as in the
+// reference's serial use): systems stored [n][ns], rows distributed over the
+// team's lanes as in the reconstruction.
 __global__ void __launch_bounds__(TB) k_tridiag(const double *__restrict__ a, const double *__restrict__ b,
                                                 const double *__restrict__ c, double *__restrict__ x, int n, int ns,
-                                                int *__restrict__ flag, double *__restrict__ sb) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= ns) return;
-  double bprev = b[d], xprev = x[d], cprev = c[d];
-  sb[d] = bprev;
-  if (bprev == 0) { atomicOr(flag, 1); return; }
-  for (int i = 1; i < n; i++) {
-    const long q = (long)i * ns + d;
-    if (bprev == 0) { atomicOr(flag, 1); return; }
-    const double fac = a[q] / bprev;
-    const double bk = b[q] - fac * cprev, xk = x[q] - fac * xprev;
-    sb[q] = bk;
-    x[q] = xk;
-    bprev = bk; xprev = xk; cprev = c[q];
+                                                int T, int m, int *__restrict__ flag) {
+  __shared__ double red[16];
+  const int lane = threadIdx.x & (T - 1);
+  const long d = (long)blockIdx.x * (TB / T) + threadIdx.x / T;
+  const bool active = d < ns;
+  const int lo = lane * m;
+  int nr = n - lo;
+  nr = nr < 0 ? 0 : (nr > m ? m : nr);
+  if (!active) nr = 0;
+  double A[kMaxRows], B[kMaxRows], Cc[kMaxRows], X[kMaxRows];
+#pragma unroll
+  for (int i = 0; i < kMaxRows; i++) {
+    if (i < nr) {
+      const long q = (long)(lo + i) * ns + d;
+      A[i] = (lo + i == 0) ? 0.0 : a[q];
+      B[i] = b[q];
+      Cc[i] = (lo + i == n - 1) ? 0.0 : c[q];
+      X[i] = x[q];
+    } else {
+      A[i] = 0.0; B[i] = 1.0; Cc[i] = 0.0; X[i] = 0.0;
+    }
   }
-  {
-    const long q = (long)(n - 1) * ns + d;
-    if (sb[q] == 0) { atomicOr(flag, 1); return; }
-    x[q] = x[q] / sb[q];
-  }
-  for (int i = n - 2; i >= 0; i--) {
-    const long q = (long)i * ns + d;
-    if (sb[q] == 0) { atomicOr(flag, 1); return; }
-    x[q] = (x[q] - c[q] * x[q + ns]) / sb[q];
-  }
+  team_solve<kMaxRows>(A, B, Cc, X, nr, lane, T, lane == (n - 1) / m, 0.0, 0.0, red, flag);
+#pragma unroll
+  for (int i = 0; i < kMaxRows; i++)
+    if (i < nr) x[(long)(lo + i) * ns + d] = X[i];
 }
 
 __global__ void __launch_bounds__(TB) k_weno_weights(int kind, double c1, double c2, double c3,
@@ -1156,129 +1389,64 @@ __global__ void __launch_bounds__(TB) k_weno_weights(int kind, double c1, double
   w[3 * (long)t + 2] = w3;
 }
 
+
 }  // namespace
 
 // ===========================================================================
-// launchers
+// host side: work decomposition and launchers
 // ===========================================================================
 
-void launch_stage_bc(const Grid &G, const BCMap &bc, const double *u, const double *const *udot, const double *a,
-                     int nprev, double *U, double *UT, hipStream_t s) {
-  const double *d0 = nprev > 0 ? udot[0] : nullptr, *d1 = nprev > 1 ? udot[1] : nullptr,
-               *d2 = nprev > 2 ? udot[2] : nullptr;
-  const double a0 = nprev > 0 ? a[0] : 0, a1 = nprev > 1 ? a[1] : 0, a2 = nprev > 2 ? a[2] : 0;
-  if (G.ndims == 1) {
-    hipLaunchKernelGGL(k_stage_bc1d, dim3(nblk((long)G.nxg * G.nv)), dim3(TB), 0, s, G, bc, u, d0, d1, d2, a0, a1,
-                       a2, nprev, U);
-  } else {
-    constexpr int T = 32;
-    dim3 grid((G.nxg + T - 1) / T, (G.nyg + T - 1) / T, G.nv);
-    hipLaunchKernelGGL(k_stage_bc2d<T>, grid, dim3(T, 8), 0, s, G, bc, u, d0, d1, d2, a0, a1, a2, nprev, U, UT);
-  }
+static const char *kKNames[K_NCLASS] = {"stage_bc", "flux", "recon", "upwind", "qderiv", "viscflux",
+                                        "assemble", "update"};
+const char *kclass_name(int k) { return kKNames[k]; }
+
+bool team_shape(int N, int max_rows, int &team, int &rows) {
+  if (max_rows < 1 || max_rows > kMaxRows) max_rows = kMaxRows;
+  const int nrows = N - 1 > 1 ? N - 1 : 1;
+  team = 1;
+  while ((long)team * max_rows < nrows) team <<= 1;
+  rows = (nrows + team - 1) / team;
+  return team <= 256;
 }
 
-void launch_flux(const Grid &G, const SweepGeom &S, int model, int dir, double gamma, const double *q, double *f,
-                 hipStream_t s) {
-  const long n = (long)(S.N + 2 * S.g) * S.nlines;
-  if (model == 1) hipLaunchKernelGGL((k_flux<1, 0>), dim3(nblk(n)), dim3(TB), 0, s, S, G.nv, gamma, q, f);
-  else if (dir == 0) hipLaunchKernelGGL((k_flux<2, 0>), dim3(nblk(n)), dim3(TB), 0, s, S, G.nv, gamma, q, f);
-  else hipLaunchKernelGGL((k_flux<2, 1>), dim3(nblk(n)), dim3(TB), 0, s, S, G.nv, gamma, q, f);
+void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
+  const Grid &G = c.G;
+  const bool two = (G.ndims == 2);
+  auto nb = [](long n) { return (int)((n + TB - 1) / TB); };
+  blocks[K_STAGE] = two ? ((G.nxg + 31) / 32) * ((G.nyg + 31) / 32) * G.nv : nb((long)G.nxg * G.nv);
+  blocks[K_FLUX] = nb((long)(c.Sx.N + 2 * c.Sx.g) * c.Sx.nlines + (two ? (long)(c.Sy.N + 2 * c.Sy.g) * c.Sy.nlines : 0));
+  if (c.R.characteristic) blocks[K_RECON] = nb((long)(c.Sx.N + 1) * c.Sx.nlines);
+  else blocks[K_RECON] = recon_sweep_blocks(c.Sx, G.nv) + (two ? recon_sweep_blocks(c.Sy, G.nv) : 0);
+  blocks[K_UPWIND] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
+  const bool visc = two && c.P.viscous;
+  blocks[K_QDERIV] = visc ? nb((long)G.nxg * G.nyg) : 0;
+  blocks[K_VISCFLUX] = visc ? nb((long)G.nxg * G.nyg) : 0;
+  blocks[K_ASSEMBLE] = two ? ((G.nx + 31) / 32) * ((G.ny + 31) / 32) : nb(G.nx);
+  blocks[K_UPDATE] = nb((long)G.nx * G.ny * G.nv);
 }
 
-void launch_recon_components(const Grid &G, const SweepGeom &S, const ReconParams &R, const double *q,
-                             const double *f, double *uL, double *uR, double *fL, double *fR, double *scr_b,
-                             double *scr_c, hipStream_t s) {
-  if (R.crweno && R.kind < 0) {  // producer/consumer variant: kept for measurement, not dispatched
-    const dim3 gr((S.nlines + 63) / 64, G.nv * 4), bl(256);
-#define PC(K) hipLaunchKernelGGL((k_crweno_pc<K, 8>), gr, bl, 0, s, S, G.nv, R, q, f, uL, uR, fL, fR, scr_b, scr_c)
-    switch (R.kind) { case 1: PC(1); break; case 2: PC(2); break; case 3: PC(3); break; default: PC(0); }
-#undef PC
-    return;
-  }
-  const long n = (long)S.nlines * G.nv * 4;
-  const dim3 gr(nblk(n)), bl(TB);
-#define RC(CR, K) hipLaunchKernelGGL((k_recon_comp<CR, K, 8>), gr, bl, 0, s, S, G.nv, R, q, f, uL, uR, fL, fR, scr_b, scr_c)
-  if (R.crweno) {
-    switch (R.kind) { case 1: RC(1, 1); break; case 2: RC(1, 2); break; case 3: RC(1, 3); break; default: RC(1, 0); }
-  } else {
-    switch (R.kind) { case 1: RC(0, 1); break; case 2: RC(0, 2); break; case 3: RC(0, 3); break; default: RC(0, 0); }
-  }
-#undef RC
+typedef void (*KernFn)(const CaseDev *, const int *, const int *, int);
+
+static void run(KernFn k, int cls, const CaseDev *d, const Plan &pl, int s, hipStream_t st,
+                void (*mark)(void *, int, bool), void *ctx) {
+  if (pl.nblocks[cls] <= 0) return;
+  if (mark) mark(ctx, cls, true);
+  hipLaunchKernelGGL(k, dim3(pl.nblocks[cls]), dim3(TB), 0, st, d, pl.blockcase[cls], pl.firstblock[cls], s);
+  if (mark) mark(ctx, cls, false);
 }
 
-void launch_recon_char_euler1d(const Grid &G, const SweepGeom &S, const ReconParams &R, double gamma,
-                               const double *q, const double *f, double *uL, double *uR, double *fL, double *fR,
-                               hipStream_t s) {
-  const long n = (long)(S.N + 1) * S.nlines;
-  const dim3 gr(nblk(n)), bl(TB);
-#define RCH(K) hipLaunchKernelGGL((k_recon_char_e1d<K>), gr, bl, 0, s, S, G.nv, R, gamma, q, f, uL, uR, fL, fR)
-  switch (R.kind) { case 1: RCH(1); break; case 2: RCH(2); break; case 3: RCH(3); break; default: RCH(0); }
-#undef RCH
+void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, void (*mark)(void *, int, bool),
+                  void *ctx) {
+  run(k_stage, K_STAGE, d, pl, s, st, mark, ctx);
+  run(k_flux, K_FLUX, d, pl, s, st, mark, ctx);
+  run(k_recon, K_RECON, d, pl, s, st, mark, ctx);
+  run(k_upwind, K_UPWIND, d, pl, s, st, mark, ctx);
+  run(k_qderiv, K_QDERIV, d, pl, s, st, mark, ctx);
+  run(k_viscflux, K_VISCFLUX, d, pl, s, st, mark, ctx);
+  run(k_assemble, K_ASSEMBLE, d, pl, s, st, mark, ctx);
 }
 
-void launch_upwind(const Grid &G, const SweepGeom &S, int model, int upw, int dir, double gamma, const double *q,
-                   const double *uL, const double *uR, const double *fL, const double *fR, double *fI,
-                   hipStream_t s) {
-  const long n = (long)(S.N + 1) * S.nlines;
-  const dim3 gr(nblk(n)), bl(TB);
-  if (model == 1) {
-    if (upw == 1) hipLaunchKernelGGL((k_upwind_e1d<1>), gr, bl, 0, s, S, G.nv, gamma, q, uL, uR, fL, fR, fI);
-    else hipLaunchKernelGGL((k_upwind_e1d<0>), gr, bl, 0, s, S, G.nv, gamma, q, uL, uR, fL, fR, fI);
-  } else if (dir == 0) {
-    hipLaunchKernelGGL((k_upwind_ns2d_roe<0>), gr, bl, 0, s, S, G.nv, gamma, q, uL, uR, fL, fR, fI);
-  } else {
-    hipLaunchKernelGGL((k_upwind_ns2d_roe<1>), gr, bl, 0, s, S, G.nv, gamma, q, uL, uR, fL, fR, fI);
-  }
-}
-
-void launch_divergence(const Grid &G, const SweepGeom &S, const double *dxinv_dir, const double *fI, double *h,
-                       hipStream_t s) {
-  const long n = (long)S.N * S.nlines;
-  hipLaunchKernelGGL(k_divergence, dim3(nblk(n)), dim3(TB), 0, s, S, G.nv, dxinv_dir, fI, h);
-}
-
-void launch_transpose(int nv, int rows, int cols, const double *src, double *dst, hipStream_t s) {
-  dim3 grid((cols + 31) / 32, (rows + 31) / 32, nv);
-  hipLaunchKernelGGL(k_transpose, grid, dim3(32, 8), 0, s, rows, cols, src, dst);
-}
-
-void launch_qderiv(const Grid &G, const PhysParams &P, const double *U, const double *dxinv, double *Qd,
-                   hipStream_t s) {
-  const long n = (long)G.nxg * G.nyg;
-  if (P.par4) hipLaunchKernelGGL((k_qderiv<1>), dim3(nblk(n)), dim3(TB), 0, s, G, P.gamma, U, dxinv, Qd);
-  else hipLaunchKernelGGL((k_qderiv<0>), dim3(nblk(n)), dim3(TB), 0, s, G, P.gamma, U, dxinv, Qd);
-}
-
-void launch_viscflux(const Grid &G, const PhysParams &P, const double *U, const double *Qd, double *FV,
-                     hipStream_t s) {
-  const long n = (long)G.nxg * G.nyg;
-  hipLaunchKernelGGL(k_viscflux, dim3(nblk(n)), dim3(TB), 0, s, G, P, U, Qd, FV);
-}
-
-void launch_assemble(const Grid &G, const PhysParams &P, const double *U, const double *hx, const double *fIy,
-                     const double *FV, const double *dxinv, const double *imap, double hnu_scale, double *udot,
-                     hipStream_t s) {
-  const long n = (long)G.nx * G.ny;
-  const dim3 gr(nblk(n)), bl(TB);
-#define AS(ND, VI, P4, CH) \
-  hipLaunchKernelGGL((k_assemble<ND, VI, P4, CH>), gr, bl, 0, s, G, P, U, hx, fIy, FV, dxinv, imap, hnu_scale, udot)
-  if (G.ndims == 1) {
-    if (P.chem) AS(1, 0, 0, 1); else AS(1, 0, 0, 0);
-  } else if (P.viscous) {
-    if (P.par4) { if (P.chem) AS(2, 1, 1, 1); else AS(2, 1, 1, 0); }
-    else { if (P.chem) AS(2, 1, 0, 1); else AS(2, 1, 0, 0); }
-  } else {
-    if (P.chem) AS(2, 0, 0, 1); else AS(2, 0, 0, 0);
-  }
-#undef AS
-}
-
-void launch_update(const Grid &G, double *u, const double *const *udot, const double *bcoef, int nst, hipStream_t s) {
-  const long n = (long)G.nx * G.ny * G.nv;
-  hipLaunchKernelGGL(k_update, dim3(nblk(n)), dim3(TB), 0, s, G, u, udot[0], nst > 1 ? udot[1] : udot[0],
-                     nst > 2 ? udot[2] : udot[0], nst > 3 ? udot[3] : udot[0], bcoef[0], nst > 1 ? bcoef[1] : 0.0,
-                     nst > 2 ? bcoef[2] : 0.0, nst > 3 ? bcoef[3] : 0.0, nst);
-}
+void launch_update(const CaseDev *d, const Plan &pl, hipStream_t st) { run(k_update, K_UPDATE, d, pl, 0, st, nullptr, nullptr); }
 
 int cfl_norm_blocks(const Grid &G) {
   const long n = (long)G.nx * G.ny;
@@ -1291,9 +1459,14 @@ void launch_cfl_norm(const Grid &G, int model, double gamma, double dt, const do
   hipLaunchKernelGGL(k_cfl_norm, dim3(nblocks), dim3(TB), 0, s, G, model, gamma, dt, u, u_prev, dxinv, partials);
 }
 
-void launch_tridiag(const double *a, const double *b, const double *c, double *x, int n, int ns, int *flag,
-                    double *scr_b, hipStream_t s) {
-  hipLaunchKernelGGL(k_tridiag, dim3(nblk(ns)), dim3(TB), 0, s, a, b, c, x, n, ns, flag, scr_b);
+bool launch_tridiag(const double *a, const double *b, const double *c, double *x, int n, int ns, int *flag,
+                    hipStream_t s) {
+  int T, m;
+  if (!team_shape(n + 1, kMaxRows, T, m)) return false;
+  const long teams_per_block = TB / T;
+  const long nblocks = (ns + teams_per_block - 1) / teams_per_block;
+  hipLaunchKernelGGL(k_tridiag, dim3((unsigned)nblocks), dim3(TB), 0, s, a, b, c, x, n, ns, T, m, flag);
+  return true;
 }
 
 void launch_weno_weights(int kind, const double *copt, const double *st, int count, double eps, double *w,

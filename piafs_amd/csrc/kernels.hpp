@@ -1,20 +1,25 @@
 // Device kernels of the PIAFS RHS / RK path and their host launchers.
 //
+// Every kernel covers a whole ensemble of cases in one launch.  Each case's
+// work items are padded to whole blocks so that a block belongs to exactly
+// one case: block b runs case blockcase[b], local item
+// (b - firstblock[case]) * blockDim + threadIdx.  A single simulation is an
+// ensemble of one.
+//
 // Layouts (all FP64, one plane per variable):
-//   natural  : u[v*plane + (j+gy)*nxg + (i+g)]      x fastest (2D) / u[v*plane + i+g] (1D)
-//   x-sweep  : uT[v*plane + (i+g)*nyg + (j+gy)]     transposed copy, y fastest
-//   interface: a[(v*(N+1) + k)*nlines + l]          k = interface along the sweep,
-//                                                    l = interior line index (fast)
-// A "sweep" is one direction's hyperbolic term over all interior lines.  In
-// both sweep layouts the sweep coordinate is the slow index and the line index
-// the fast one, so one lane per line reads coalesced rows at every position.
+//   natural   : u[v*plane + (j+gy)*nxg + (i+g)]     x fastest (2D) / u[v*plane + i+g] (1D)
+//   transposed: uT[v*plane + (i+g)*nyg + (j+gy)]    y fastest (2D only)
+//   interface : a[(v*nlines + l)*(N+1) + k]         k = interface along the sweep (fast),
+//                                                    l = interior line index
+// Every sweep reads lines that are contiguous in memory: the x-sweep the
+// natural state, the y-sweep the transposed copy written by the stage kernel.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
-namespace piafs {
+#include <vector>
 
-constexpr int MAXV = 64;
+namespace piafs {
 
 struct Grid {
   int ndims, nv, g;
@@ -27,11 +32,20 @@ struct Grid {
 struct SweepGeom {
   int N;             // interior points along the sweep
   int nlines;        // interior lines
-  int pos_stride;    // elements between consecutive positions along the sweep
-  int line_off;      // element offset of interior line 0 within a row
   int g;
   long plane;
+  long base;         // element offset of (line 0, position 0); positions run -g .. N+g-1
+  int line_stride;   // elements between consecutive lines
+  int team;          // lanes sharing one line system (power of two, <= 256)
+  int rows;          // interior rows (1 .. N-1) held by each lane, <= kMaxRows
 };
+
+// Longest segment of a line system one lane keeps in registers.
+constexpr int kMaxRows = 16;
+// Team shape for lines of N points: the fewest lanes (a power of two) whose
+// segments of at most max_rows rows cover interior rows 1..N-1.  Returns
+// false when a line is too long for one block.
+bool team_shape(int N, int max_rows, int &team, int &rows);
 
 // Boundary map for one face: type (0 periodic, 1 extrapolate, -1 none) and the
 // covered interior range [lo, hi) in the other dimension.
@@ -53,49 +67,65 @@ struct PhysParams {
   double k0a, k0b, k1a, k1b, k2a, k2b, k3a, k3b, k4, k5, k6;
   double q0a, q0b, q1a, q1b, q2a, q2b, q3a, q3b, q4, q5, q6;
   double ch_gm1, hnu_den, damp_coef;   // (gamma_chem - 1), c*h*nu*n_O2, dz*sO3*n_O2
+  double I0, t_start_norm, t_pulse_norm;
 };
 
 struct ReconParams {
-  int crweno, kind, no_limiting;
+  int crweno, kind, no_limiting, characteristic;
   double eps;
 };
 
-// ---- launchers (stream-ordered, no synchronisation) ----
-// Stage state with boundary conditions: U = u + sum_t a[t]*Udot[t] at every
-// ghosted point, each ghost taken from its boundary source point.  Writes the
-// natural layout and, in 2D, the transposed copy for the x-sweep.
-void launch_stage_bc(const Grid &G, const BCMap &bc, const double *u, const double *const *udot,
-                     const double *a, int nprev, double *U, double *UT, hipStream_t s);
-void launch_flux(const Grid &G, const SweepGeom &S, int model, int dir, double gamma, const double *q,
-                 double *f, hipStream_t s);
-void launch_recon_components(const Grid &G, const SweepGeom &S, const ReconParams &R, const double *q,
-                             const double *f, double *uL, double *uR, double *fL, double *fR,
-                             double *scr_b, double *scr_c, hipStream_t s);
-void launch_recon_char_euler1d(const Grid &G, const SweepGeom &S, const ReconParams &R, double gamma,
-                               const double *q, const double *f, double *uL, double *uR, double *fL,
-                               double *fR, hipStream_t s);
-void launch_upwind(const Grid &G, const SweepGeom &S, int model, int upw, int dir, double gamma,
-                   const double *q, const double *uL, const double *uR, const double *fL, const double *fR,
-                   double *fI, hipStream_t s);
-void launch_divergence(const Grid &G, const SweepGeom &S, const double *dxinv_dir, const double *fI,
-                       double *h, hipStream_t s);
-void launch_transpose(int nv, int rows, int cols, const double *src, double *dst, hipStream_t s);
-void launch_qderiv(const Grid &G, const PhysParams &P, const double *U, const double *dxinv, double *Qd,
-                   hipStream_t s);
-void launch_viscflux(const Grid &G, const PhysParams &P, const double *U, const double *Qd, double *FV,
-                     hipStream_t s);
-void launch_assemble(const Grid &G, const PhysParams &P, const double *U, const double *hx,
-                     const double *fIy, const double *FV, const double *dxinv, const double *imap,
-                     double hnu_scale, double *udot, hipStream_t s);
-void launch_update(const Grid &G, double *u, const double *const *udot, const double *bcoef, int nst,
-                   hipStream_t s);
+// Everything a kernel needs to know about one case (device-resident table).
+struct CaseDev {
+  Grid G;
+  SweepGeom Sx, Sy;
+  BCMap bc;
+  PhysParams P;
+  ReconParams R;
+  int model, upwind, nstages;
+  double dt, A[16], b[4], c[4];
+  double *u, *U, *UT, *Udot[4];
+  double *flux_x, *flux_y, *ix[4], *iy[4], *fIx, *fIy;
+  double *Qd, *FV, *dxinv, *imap;
+  double *time;  // current simulation time of the case (one double)
+  int *status;   // bit 0: a zero pivot met in a line solve (tridiagLU.c:139-142 returns -1)
+};
+
+// Kernel classes launched per RK stage (plus the final update).
+enum KClass {
+  K_STAGE = 0, K_FLUX, K_RECON, K_UPWIND, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE, K_NCLASS
+};
+const char *kclass_name(int k);
+
+// Block -> case tables of every kernel class for one ensemble.
+struct Plan {
+  int ncases = 0;
+  int nblocks[K_NCLASS] = {0};
+  int *blockcase[K_NCLASS] = {nullptr};   // device arrays, one int per block
+  int *firstblock[K_NCLASS] = {nullptr};  // device arrays, one int per case
+};
+
+// Host side: blocks each case needs for each kernel class (0 = class unused).
+void case_blocks(const CaseDev &c, int blocks[K_NCLASS]);
+
+// Stage s of the RK step for every case in the plan: stage state + BCs,
+// fluxes, reconstruction, upwinding, divergence, viscous terms, assembly of
+// Udot[s].  Times: stage time = *time + c[s]*dt of each case.
+void launch_stage(const CaseDev *d_cases, const Plan &plan, int s, hipStream_t st, void (*mark)(void *, int, bool),
+                  void *mark_ctx);
+// u += dt*b_s*Udot_s for every stage, time += dt.
+void launch_update(const CaseDev *d_cases, const Plan &plan, hipStream_t st);
+
+// diagnostics (single case): per-block partial max CFL and sum of squared
+// step differences of the interior state
 void launch_cfl_norm(const Grid &G, int model, double gamma, double dt, const double *u, const double *u_prev,
                      const double *dxinv, double *partials, int nblocks, hipStream_t s);
 int cfl_norm_blocks(const Grid &G);
 
 // per-operator kernels used by the unit-level entry points
-void launch_tridiag(const double *a, const double *b, const double *c, double *x, int n, int ns, int *flag,
-                    double *scr_b, hipStream_t s);
+// false when n is too long for one team
+bool launch_tridiag(const double *a, const double *b, const double *c, double *x, int n, int ns, int *flag,
+                    hipStream_t s);
 void launch_weno_weights(int kind, const double *copt, const double *st, int count, double eps, double *w,
                          hipStream_t s);
 

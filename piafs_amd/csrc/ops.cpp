@@ -23,15 +23,18 @@ struct DevBuf {
 extern "C" int piafs_tridiag_host(const double *a, const double *b, const double *c, double *x, int n, int ns) {
   if (n <= 0 || ns <= 0) return 0;
   const size_t m = (size_t)n * ns;
-  DevBuf da(m), db(m), dc(m), dx(m), ds(m);
+  DevBuf da(m), db(m), dc(m), dx(m);
   int *flag = nullptr;
-  if (!da.p || !db.p || !dc.p || !dx.p || !ds.p || hipMalloc(&flag, sizeof(int)) != hipSuccess) return -2;
+  if (!da.p || !db.p || !dc.p || !dx.p || hipMalloc(&flag, sizeof(int)) != hipSuccess) return -2;
   hipMemcpy(da.p, a, m * sizeof(double), hipMemcpyHostToDevice);
   hipMemcpy(db.p, b, m * sizeof(double), hipMemcpyHostToDevice);
   hipMemcpy(dc.p, c, m * sizeof(double), hipMemcpyHostToDevice);
   hipMemcpy(dx.p, x, m * sizeof(double), hipMemcpyHostToDevice);
   hipMemset(flag, 0, sizeof(int));
-  piafs::launch_tridiag(da.p, db.p, dc.p, dx.p, n, ns, flag, ds.p, 0);
+  if (!piafs::launch_tridiag(da.p, db.p, dc.p, dx.p, n, ns, flag, 0)) {
+    hipFree(flag);
+    return 1;  // systems longer than one team covers
+  }
   int hflag = 0;
   hipMemcpy(x, dx.p, m * sizeof(double), hipMemcpyDeviceToHost);
   hipMemcpy(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost);

@@ -1,13 +1,14 @@
-// Case runtime and the C ABI of include/piafs_amd.h.
+// Case runtime, ensemble engine and the single-case C ABI of
+// include/piafs_amd.h.
 //
-// One case = one PIAFS simulation resident in HBM.  A time step is a fixed
-// sequence of stream-ordered kernel launches (no host synchronisation):
-// per RK stage  stage+BC -> x-sweep (flux, reconstruction, upwind,
-// divergence) -> y-sweep -> viscous derivatives -> assembly of Udot; after
-// the last stage the solution update.  See TimeRK.c:34-102 for the reference
-// control flow that this replaces.
+// A time step is a fixed sequence of stream-ordered launches with no host
+// synchronisation: per RK stage  stage+BC -> fluxes -> reconstruction (x and
+// y sweeps in one launch) -> upwinding -> x divergence -> transpose ->
+// viscous derivatives -> assembly of Udot; after the last stage the update.
+// TimeRK.c:34-102 is the reference control flow this replaces.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -18,57 +19,13 @@
 #include "../../include/piafs_amd.h"
 #include "case.hpp"
 
-using namespace piafs;
-
 namespace piafs {
 
-static const char *kClassNames[NCLASS] = {"stage_bc", "flux", "recon", "upwind", "divergence",
-                                          "transpose", "qderiv", "viscflux", "assemble", "update"};
-
-const char *class_name(int c) { return kClassNames[c]; }
-
-int Case::fail(int code, const std::string &msg) {
-  err = msg;
-  return code;
-}
-
-#define HIPCHK(call)                                                                    \
-  do {                                                                                  \
-    hipError_t e_ = (call);                                                             \
+#define HIPCHK(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
     if (e_ != hipSuccess) return fail(-1, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
-
-double *Case::dalloc(size_t n) {
-  void *p = nullptr;
-  if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) return nullptr;
-  bytes += n * sizeof(double);
-  allocs.push_back(p);
-  hipMemsetAsync(p, 0, n * sizeof(double), stream);
-  return (double *)p;
-}
-
-size_t Case::plan_bytes(const Config &c) {
-  Grid g = make_grid(c);
-  size_t plane = g.plane, nv = c.nvars;
-  size_t n = 0;
-  n += nv * plane * (3 + c.nstages + 1);  // u, U, UT, Udot[s], flux
-  size_t iface = 0;
-  for (int d = 0; d < c.ndims; d++) {
-    size_t lines = (c.ndims == 2) ? (size_t)c.dim[1 - d] : 1;
-    iface = std::max(iface, nv * (size_t)(c.dim[d] + 1) * lines);
-  }
-  n += 5 * iface;
-  if (c.crweno) n += 8 * iface;
-  if (c.ndims == 2) {  // y-sweep buffers of the second stream
-    n += nv * plane + 4 * iface;
-    if (c.crweno) n += 8 * iface;
-  }
-  size_t nint = (size_t)c.npts_int;
-  n += 2 * nv * nint;                      // hx, hxT
-  if (c.model_id == MODEL_NS2D) n += 12 * (size_t)g.nxg * g.nyg;
-  n += plane + c.x.size() + 4096;
-  return n * sizeof(double);
-}
 
 Grid make_grid(const Config &c) {
   Grid G{};
@@ -85,25 +42,59 @@ Grid make_grid(const Config &c) {
   return G;
 }
 
-int Case::init(const std::string &dir, int dev) {
+// ---------------------------------------------------------------------------
+// Case
+// ---------------------------------------------------------------------------
+int Case::fail(int code, const std::string &msg) {
+  err = msg;
+  return code;
+}
+
+double *Case::dalloc(size_t n, hipStream_t st) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) return nullptr;
+  bytes += n * sizeof(double);
+  allocs.push_back(p);
+  if (hipMemsetAsync(p, 0, n * sizeof(double), st) != hipSuccess) return nullptr;
+  return (double *)p;
+}
+
+size_t Case::plan_bytes(const Config &c) {
+  Grid g = make_grid(c);
+  size_t plane = g.plane, nv = c.nvars;
+  size_t n = nv * plane * (3 + c.nstages + 3);  // u, U, UT, Udot[s], flux_x, flux_y, uprev
+  size_t ix = nv * (size_t)(c.dim[0] + 1) * (c.ndims == 2 ? c.dim[1] : 1);
+  size_t iy = c.ndims == 2 ? nv * (size_t)(c.dim[1] + 1) * c.dim[0] : 0;
+  n += 5 * (ix + iy);
+  if (c.model_id == MODEL_NS2D) n += 12 * (size_t)g.nxg * g.nyg;
+  n += plane + c.x.size() + 4096;
+  return n * sizeof(double);
+}
+
+int Case::init(const std::string &dir, int dev, hipStream_t st) {
   if (cfg.parse(dir, err)) return 1;
   device = dev;
   HIPCHK(hipSetDevice(device));
-  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  G = make_grid(cfg);
+  CaseDev &D = h;
+  memset(&D, 0, sizeof(D));
+  D.G = make_grid(cfg);
+  const Grid &G = D.G;
   const int nv = cfg.nvars;
-  // sweeps
-  Sx.N = G.nx; Sx.g = G.g; Sx.plane = G.plane;
+  // sweep geometry: x lines are rows of the natural layout, y lines rows of
+  // the transposed copy
+  D.Sx.N = G.nx; D.Sx.g = G.g; D.Sx.plane = G.plane;
   if (cfg.ndims == 1) {
-    Sx.nlines = 1; Sx.pos_stride = 1; Sx.line_off = 0;
+    D.Sx.nlines = 1; D.Sx.base = G.g; D.Sx.line_stride = G.nxg;
   } else {
-    Sx.nlines = G.ny; Sx.pos_stride = G.nyg; Sx.line_off = G.gy;   // transposed layout
-    Sy.N = G.ny; Sy.g = G.g; Sy.plane = G.plane;
-    Sy.nlines = G.nx; Sy.pos_stride = G.nxg; Sy.line_off = G.g;    // natural layout
+    D.Sx.nlines = G.ny; D.Sx.base = (long)G.gy * G.nxg + G.g; D.Sx.line_stride = G.nxg;
+    D.Sy.N = G.ny; D.Sy.g = G.g; D.Sy.plane = G.plane;
+    D.Sy.nlines = G.nx; D.Sy.base = (long)G.g * G.nyg + G.gy; D.Sy.line_stride = G.nyg;
   }
-  // boundary map (one zone per face; later zones on the same face replace earlier ones)
+  if (set_team_rows(kMaxRows)) return 1;
+  // boundary map: one zone per face (a later zone on the same face replaces an earlier one)
   for (int d = 0; d < 2; d++)
-    for (int f = 0; f < 2; f++) bc.f[d][f] = FaceBC{-1, 0, 0};
+    for (int f = 0; f < 2; f++) D.bc.f[d][f] = FaceBC{-1, 0, 0};
   for (const Boundary &b : cfg.bcs) {
     if (!b.on) continue;
     FaceBC fb;
@@ -116,10 +107,9 @@ int Case::init(const std::string &dir, int dev) {
       fb.lo = 0;
       fb.hi = 1;
     }
-    bc.f[b.dim][b.face == 1 ? 0 : 1] = fb;
+    D.bc.f[b.dim][b.face == 1 ? 0 : 1] = fb;
   }
-  // physics parameters
-  memset(&P, 0, sizeof(P));
+  PhysParams &P = D.P;
   P.gamma = cfg.gamma;
   P.viscous = (cfg.model_id == MODEL_NS2D && cfg.Re > 0) ? 1 : 0;
   P.par4 = cfg.par4 ? 1 : 0;
@@ -144,85 +134,85 @@ int Case::init(const std::string &dir, int dev) {
     P.ch_gm1 = ch.gamma - 1.0;
     P.hnu_den = ch.c_light * ch.h_planck * ch.nu * ch.n_O2;
     P.damp_coef = ch.dz * (ch.sO3 * ch.n_O2);
+    P.I0 = ch.I0;
+    P.t_start_norm = ch.t_start_norm;
+    P.t_pulse_norm = ch.t_pulse_norm;
   }
-  R.crweno = cfg.crweno ? 1 : 0;
-  R.kind = cfg.weight_kind;
-  R.no_limiting = cfg.no_limiting;
-  R.eps = cfg.eps;
+  D.R.crweno = cfg.crweno ? 1 : 0;
+  D.R.kind = cfg.weight_kind;
+  D.R.no_limiting = cfg.no_limiting;
+  D.R.characteristic = cfg.characteristic ? 1 : 0;
+  D.R.eps = cfg.eps;
+  D.model = cfg.model_id;
+  D.upwind = cfg.upwind;
+  D.nstages = cfg.nstages;
+  D.dt = cfg.dt;
+  for (int k = 0; k < 16; k++) D.A[k] = cfg.A[k];
+  for (int k = 0; k < 4; k++) { D.b[k] = cfg.b[k]; D.c[k] = cfg.c[k]; }
   // device buffers
-  const size_t st = (size_t)nv * G.plane;
-  u = dalloc(st);
-  U = dalloc(st);
-  if (cfg.ndims == 2) UT = dalloc(st);
-  for (int s = 0; s < cfg.nstages; s++) Udot[s] = dalloc(st);
-  flux = dalloc(st);
-  size_t iface = 0;
-  iface = std::max(iface, (size_t)nv * (Sx.N + 1) * Sx.nlines);
-  if (cfg.ndims == 2) iface = std::max(iface, (size_t)nv * (Sy.N + 1) * Sy.nlines);
-  for (int k = 0; k < 4; k++) ifc[k] = dalloc(iface);
-  fIx = dalloc(iface);
-  if (cfg.ndims == 2) fIy = dalloc((size_t)nv * (Sy.N + 1) * Sy.nlines);
-  if (cfg.crweno) {
-    scr_b = dalloc(4 * iface);
-    scr_c = dalloc(4 * iface);
+  const size_t st_sz = (size_t)nv * G.plane;
+  const bool two = (cfg.ndims == 2);
+  D.u = dalloc(st_sz, st);
+  D.U = dalloc(st_sz, st);
+  if (two) D.UT = dalloc(st_sz, st);
+  for (int s = 0; s < cfg.nstages; s++) D.Udot[s] = dalloc(st_sz, st);
+  D.flux_x = dalloc(st_sz, st);
+  const size_t ix = (size_t)nv * (D.Sx.N + 1) * D.Sx.nlines;
+  for (int k = 0; k < 4; k++) D.ix[k] = dalloc(ix, st);
+  D.fIx = dalloc(ix, st);
+  if (two) {
+    const size_t iy = (size_t)nv * (D.Sy.N + 1) * D.Sy.nlines;
+    D.flux_y = dalloc(st_sz, st);
+    for (int k = 0; k < 4; k++) D.iy[k] = dalloc(iy, st);
+    D.fIy = dalloc(iy, st);
   }
-  if (cfg.ndims == 2) {
-    dual = true;
-    HIPCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-    const size_t ify = (size_t)nv * (Sy.N + 1) * Sy.nlines;
-    flux_y = dalloc(st);
-    for (int k = 0; k < 4; k++) ifcy[k] = dalloc(ify);
-    if (cfg.crweno) {
-      scr_by = dalloc(4 * ify);
-      scr_cy = dalloc(4 * ify);
-    }
-  }
-  const size_t nint = (size_t)G.nx * G.ny;
-  hx = dalloc((size_t)nv * nint);
-  if (cfg.ndims == 2) hxT = dalloc((size_t)nv * nint);
   if (P.viscous) {
-    Qd = dalloc(6 * (size_t)G.nxg * G.nyg);
-    FV = dalloc(6 * (size_t)G.nxg * G.nyg);
+    D.Qd = dalloc(6 * (size_t)G.nxg * G.nyg, st);
+    D.FV = dalloc(6 * (size_t)G.nxg * G.nyg, st);
   }
-  dxinv = dalloc(cfg.dxinv.size());
-  imap = dalloc(G.plane);
+  D.dxinv = dalloc(cfg.dxinv.size(), st);
+  D.imap = dalloc(G.plane, st);
+  D.time = dalloc(1, st);
   nparts = cfl_norm_blocks(G);
-  partials = dalloc(2 * (size_t)nparts);
-  uprev = dalloc(st);
-  if (!u || !U || !flux || !dxinv || !imap || !hx || !partials || !uprev)
+  partials = dalloc(2 * (size_t)nparts, st);
+  uprev = dalloc(st_sz, st);
+  for (void *p : allocs)
+    if (!p) return fail(-2, "device allocation failed");
+  if (!D.u || !D.U || !D.flux_x || !D.dxinv || !D.imap || !D.time || !partials || !uprev ||
+      (two && (!D.UT || !D.flux_y || !D.fIy)))
     return fail(-2, "device allocation failed");
-  HIPCHK(hipMemcpyAsync(dxinv, cfg.dxinv.data(), cfg.dxinv.size() * sizeof(double), hipMemcpyHostToDevice,
-                        stream));
+  HIPCHK(hipMemcpyAsync(D.dxinv, cfg.dxinv.data(), cfg.dxinv.size() * sizeof(double), hipMemcpyHostToDevice, st));
   if (cfg.include_chem) {
     std::vector<double> im(G.plane, 0.0);
     for (long p = 0; p < (long)cfg.imap.size(); p++) im[p] = cfg.imap[p];
-    HIPCHK(hipMemcpyAsync(imap, im.data(), G.plane * sizeof(double), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpyAsync(D.imap, im.data(), G.plane * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
   }
   t = (double)cfg.restart_iter * cfg.dt;
-  int rc = upload(cfg.u0.data());
+  HIPCHK(hipMemcpyAsync(D.time, &t, sizeof(double), hipMemcpyHostToDevice, st));
+  int rc = upload(cfg.u0.data(), st);
   if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int Case::set_team_rows(int max_rows) {
+  CaseDev &D = h;
+  if (!team_shape(D.Sx.N, max_rows, D.Sx.team, D.Sx.rows) ||
+      (cfg.ndims == 2 && !team_shape(D.Sy.N, max_rows, D.Sy.team, D.Sy.rows)))
+    return fail(1, "grid lines longer than " + std::to_string(256 * kMaxRows + 1) +
+                       " points are not supported by the line solver");
   return 0;
 }
 
 Case::~Case() {
   if (device >= 0) hipSetDevice(device);
-  for (auto &e : ev) {
-    hipEventDestroy(e.first);
-    hipEventDestroy(e.second);
-  }
   for (void *p : allocs) hipFree(p);
-  if (ev_fork) hipEventDestroy(ev_fork);
-  if (ev_join) hipEventDestroy(ev_join);
-  if (stream2) hipStreamDestroy(stream2);
-  if (stream) hipStreamDestroy(stream);
 }
 
 // point-major interior (WriteBinary order) <-> natural planar ghosted
-int Case::upload(const double *uh) {
+int Case::upload(const double *uh, hipStream_t st) {
+  const Grid &G = h.G;
   const int nv = cfg.nvars;
   std::vector<double> buf((size_t)nv * G.plane, 0.0);
   for (long q = 0; q < cfg.npts_int; q++) {
@@ -230,16 +220,17 @@ int Case::upload(const double *uh) {
     long p = (long)(j + G.gy) * G.nxg + (i + G.g);
     for (int v = 0; v < nv; v++) buf[(size_t)v * G.plane + p] = uh[(size_t)q * nv + v];
   }
-  HIPCHK(hipMemcpyAsync(u, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpyAsync(h.u, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
 
-int Case::download(double *uh, const double *src) {
+int Case::download(double *uh, hipStream_t st, const double *src) {
+  const Grid &G = h.G;
   const int nv = cfg.nvars;
   std::vector<double> buf((size_t)nv * G.plane);
-  HIPCHK(hipMemcpyAsync(buf.data(), src ? src : u, buf.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipStreamSynchronize(stream));
+  HIPCHK(hipMemcpyAsync(buf.data(), src ? src : h.u, buf.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   for (long q = 0; q < cfg.npts_int; q++) {
     int i = (int)(q % G.nx), j = (int)(q / G.nx);
     long p = (long)(j + G.gy) * G.nxg + (i + G.g);
@@ -248,9 +239,106 @@ int Case::download(double *uh, const double *src) {
   return 0;
 }
 
-void Case::mark(int cls, bool begin, hipStream_t st) {
+// WriteBinary layout (WriteBinary.c:37-93)
+int Case::write_binary(const std::string &path, hipStream_t st) {
+  std::vector<double> uh((size_t)cfg.npts_int * cfg.nvars);
+  int rc = download(uh.data(), st);
+  if (rc) return rc;
+  FILE *f = fopen(path.c_str(), "wb");
+  if (!f) return fail(2, "could not open " + path + " for writing");
+  int nd = cfg.ndims, nv = cfg.nvars;
+  bool ok = fwrite(&nd, sizeof(int), 1, f) == 1 && fwrite(&nv, sizeof(int), 1, f) == 1 &&
+            fwrite(cfg.dim, sizeof(int), nd, f) == (size_t)nd;
+  long off = 0;
+  for (int d = 0; d < nd && ok; d++) {
+    ok = fwrite(&cfg.x[off + cfg.ghosts], sizeof(double), cfg.dim[d], f) == (size_t)cfg.dim[d];
+    off += cfg.dim[d] + 2 * cfg.ghosts;
+  }
+  ok = ok && fwrite(uh.data(), sizeof(double), uh.size(), f) == uh.size();
+  fclose(f);
+  if (!ok) return fail(2, "short write to " + path);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+int Engine::fail(int code, const std::string &msg) {
+  err = msg;
+  return code;
+}
+
+int Engine::init(int dev) {
+  device = dev;
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  return 0;
+}
+
+int Engine::add(Case *c) {
+  cases.push_back(c);
+  return 0;
+}
+
+void Engine::release_plan() {
+  if (gexec) hipGraphExecDestroy(gexec);
+  if (graph) hipGraphDestroy(graph);
+  gexec = nullptr;
+  graph = nullptr;
+  for (void *p : plan_allocs) hipFree(p);
+  plan_allocs.clear();
+  d_cases = nullptr;
+  d_status = nullptr;
+  plan = Plan();
+}
+
+int Engine::build() {
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamSynchronize(stream));
+  release_plan();
+  const int nc = (int)cases.size();
+  plan.ncases = nc;
+  if (nc == 0) return 0;
+  void *p = nullptr;
+  HIPCHK(hipMalloc(&p, nc * sizeof(int)));
+  plan_allocs.push_back(p);
+  d_status = (int *)p;
+  HIPCHK(hipMemset(d_status, 0, nc * sizeof(int)));
+  std::vector<CaseDev> hd(nc);
+  for (int k = 0; k < nc; k++) {
+    hd[k] = cases[k]->h;
+    hd[k].status = d_status + k;
+  }
+  HIPCHK(hipMalloc(&p, nc * sizeof(CaseDev)));
+  plan_allocs.push_back(p);
+  d_cases = (CaseDev *)p;
+  HIPCHK(hipMemcpy(d_cases, hd.data(), nc * sizeof(CaseDev), hipMemcpyHostToDevice));
+  for (int cls = 0; cls < K_NCLASS; cls++) {
+    std::vector<int> bc, first(nc);
+    for (int k = 0; k < nc; k++) {
+      int b[K_NCLASS];
+      case_blocks(hd[k], b);
+      first[k] = (int)bc.size();
+      for (int j = 0; j < b[cls]; j++) bc.push_back(k);
+    }
+    plan.nblocks[cls] = (int)bc.size();
+    if (bc.empty()) continue;
+    HIPCHK(hipMalloc(&p, bc.size() * sizeof(int)));
+    plan_allocs.push_back(p);
+    plan.blockcase[cls] = (int *)p;
+    HIPCHK(hipMemcpy(p, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&p, nc * sizeof(int)));
+    plan_allocs.push_back(p);
+    plan.firstblock[cls] = (int *)p;
+    HIPCHK(hipMemcpy(p, first.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+static void mark_cb(void *ctx, int cls, bool begin) { ((Engine *)ctx)->mark(cls, begin); }
+
+void Engine::mark(int cls, bool begin) {
   if (!profiling) return;
-  if (!st) st = stream;
   if (begin) {
     if (ev_used >= ev.size()) {
       hipEvent_t a, b;
@@ -260,207 +348,140 @@ void Case::mark(int cls, bool begin, hipStream_t st) {
       ev_cls.push_back(cls);
     }
     ev_cls[ev_used] = cls;
-    hipEventRecord(ev[ev_used].first, st);
+    hipEventRecord(ev[ev_used].first, stream);
   } else {
-    hipEventRecord(ev[ev_used].second, st);
+    hipEventRecord(ev[ev_used].second, stream);
     ev_used++;
   }
 }
 
-// Right-hand side of stage state U (already holding the BC-filled state) into
-// out: TimeRHSFunctionExplicit.c:29-77.
-void Case::rhs_from_stage(double time, double *out) {
-  const int model = cfg.model_id;
-  hipStream_t sy = dual ? stream2 : stream;
-  double *fy = dual ? flux_y : flux;
-  double *const *iy = dual ? ifcy : ifc;
-  double *sby = dual ? scr_by : scr_b, *scy = dual ? scr_cy : scr_c;
-  if (dual) {
-    hipEventRecord(ev_fork, stream);
-    hipStreamWaitEvent(stream2, ev_fork, 0);
-  }
-  if (cfg.ndims == 2) {
-    // ---- y direction (natural layout) and viscous derivatives ----
-    mark(1, true, sy);
-    launch_flux(G, Sy, model, 1, cfg.gamma, U, fy, sy);
-    mark(1, false, sy);
-    mark(2, true, sy);
-    launch_recon_components(G, Sy, R, U, fy, iy[0], iy[1], iy[2], iy[3], sby, scy, sy);
-    mark(2, false, sy);
-    mark(3, true, sy);
-    launch_upwind(G, Sy, model, cfg.upwind, 1, cfg.gamma, U, iy[0], iy[1], iy[2], iy[3], fIy, sy);
-    mark(3, false, sy);
-    if (P.viscous) {
-      mark(6, true, sy);
-      launch_qderiv(G, P, U, dxinv, Qd, sy);
-      mark(6, false, sy);
-      mark(7, true, sy);
-      launch_viscflux(G, P, U, Qd, FV, sy);
-      mark(7, false, sy);
-    }
-    if (dual) hipEventRecord(ev_join, stream2);
-  }
-  // ---- x direction (transposed copy in 2D) ----
-  const double *qx = (cfg.ndims == 2) ? UT : U;
-  mark(1, true);
-  launch_flux(G, Sx, model, 0, cfg.gamma, qx, flux, stream);
-  mark(1, false);
-  mark(2, true);
-  if (cfg.characteristic)
-    launch_recon_char_euler1d(G, Sx, R, cfg.gamma, qx, flux, ifc[0], ifc[1], ifc[2], ifc[3], stream);
-  else
-    launch_recon_components(G, Sx, R, qx, flux, ifc[0], ifc[1], ifc[2], ifc[3], scr_b, scr_c, stream);
-  mark(2, false);
-  mark(3, true);
-  launch_upwind(G, Sx, model, cfg.upwind, 0, cfg.gamma, qx, ifc[0], ifc[1], ifc[2], ifc[3], fIx, stream);
-  mark(3, false);
-  mark(4, true);
-  launch_divergence(G, Sx, dxinv, fIx, (cfg.ndims == 2) ? hxT : hx, stream);
-  mark(4, false);
-  if (cfg.ndims == 2) {
-    mark(5, true);
-    launch_transpose(cfg.nvars, G.nx, G.ny, hxT, hx, stream);
-    mark(5, false);
-  }
-  if (dual) hipStreamWaitEvent(stream, ev_join, 0);
-  // ---- source time factor (ChemistrySolve.c:36-43), evaluated on the host ----
-  double hs = 0.0;
-  if (cfg.include_chem) {
-    const Chem &ch = cfg.chem;
-    if (time > ch.t_start_norm) {
-      double sigma = ch.t_pulse_norm / 2.35482;
-      double tp = time - (ch.t_start_norm + ch.t_pulse_norm);
-      hs = ch.I0 * exp(-(tp * tp) / (2 * sigma * sigma));
-    }
-  }
-  mark(8, true);
-  launch_assemble(G, P, U, hx, fIy, FV, dxinv, imap, hs, out, stream);
-  mark(8, false);
+void Engine::enqueue_step() {
+  int ns = 0;
+  for (Case *c : cases) ns = std::max(ns, c->cfg.nstages);
+  for (int s = 0; s < ns; s++) launch_stage(d_cases, plan, s, stream, profiling ? mark_cb : nullptr, this);
+  if (profiling) mark(K_UPDATE, true);
+  launch_update(d_cases, plan, stream);
+  if (profiling) mark(K_UPDATE, false);
 }
 
-void Case::enqueue_step() {
-  const int ns = cfg.nstages;
-  for (int s = 0; s < ns; s++) {
-    const double stage_time = t + cfg.c[s] * cfg.dt;
-    const double *dd[3] = {nullptr, nullptr, nullptr};
-    double aa[3] = {0, 0, 0};
-    int np = 0;
-    for (int j = 0; j < s; j++) {
-      const double a = cfg.dt * cfg.A[s * ns + j];
-      if (a == 0.0) continue;  // U += 0*Udot leaves U unchanged
-      dd[np] = Udot[j];
-      aa[np] = a;
-      np++;
-    }
-    mark(0, true);
-    launch_stage_bc(G, bc, u, dd, aa, np, U, UT, stream);
-    mark(0, false);
-    rhs_from_stage(stage_time, Udot[s]);
-  }
-  double bb[4];
-  const double *dd[4];
-  for (int s = 0; s < ns; s++) {
-    bb[s] = cfg.dt * cfg.b[s];
-    dd[s] = Udot[s];
-  }
-  mark(9, true);
-  launch_update(G, u, dd, bb, ns, stream);
-  mark(9, false);
-  t += cfg.dt;  // TimePostStep.c:33
-}
-
-int Case::step(int n, double *cfl, double *norm) {
+int Engine::step(int n) {
   HIPCHK(hipSetDevice(device));
-  for (int it = 0; it < n; it++) {
-    const bool diag = (cfl || norm);
-    if (diag) HIPCHK(hipMemcpyAsync(uprev, u, (size_t)cfg.nvars * G.plane * sizeof(double),
-                                    hipMemcpyDeviceToDevice, stream));
-    enqueue_step();
-    if (diag) {
-      launch_cfl_norm(G, cfg.model_id, cfg.gamma, cfg.dt, u, uprev, dxinv, partials, nparts, stream);
-      std::vector<double> h(2 * nparts);
-      HIPCHK(hipMemcpyAsync(h.data(), partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipStreamSynchronize(stream));
-      double mx = 0, sm = 0;
-      for (int b = 0; b < nparts; b++) {
-        mx = std::max(mx, h[2 * b]);
-        sm += h[2 * b + 1];
-      }
-      if (cfl) cfl[it] = mx;
-      if (norm) norm[it] = sqrt(sm / (double)cfg.npts_int);
+  if (cases.empty()) return 0;
+  if (use_graph && !profiling) {
+    if (!gexec) {
+      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      enqueue_step();
+      HIPCHK(hipStreamEndCapture(stream, &graph));
+      HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
     }
+    for (int i = 0; i < n; i++) HIPCHK(hipGraphLaunch(gexec, stream));
+  } else {
+    for (int i = 0; i < n; i++) enqueue_step();
   }
+  HIPCHK(hipGetLastError());
+  for (Case *c : cases)
+    for (int i = 0; i < n; i++) c->t += c->cfg.dt;  // TimePostStep.c:33, same sum as on the device
+  return 0;
+}
+
+int Engine::stage_only(int s) {
+  HIPCHK(hipSetDevice(device));
+  launch_stage(d_cases, plan, s, stream, nullptr, nullptr);
   HIPCHK(hipGetLastError());
   return 0;
 }
 
-int Case::rhs_host(const double *uh, double time, double *rh) {
-  // state into u (restored afterwards), stage = u with BCs, RHS into Udot[0]
-  const size_t st = (size_t)cfg.nvars * G.plane;
-  HIPCHK(hipMemcpyAsync(uprev, u, st * sizeof(double), hipMemcpyDeviceToDevice, stream));
-  int rc = upload(uh);
-  if (rc) return rc;
-  launch_stage_bc(G, bc, u, nullptr, nullptr, 0, U, UT, stream);
-  rhs_from_stage(time, Udot[0]);
-  rc = download(rh, Udot[0]);
-  HIPCHK(hipMemcpyAsync(u, uprev, st * sizeof(double), hipMemcpyDeviceToDevice, stream));
+int Engine::sync() {
+  HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamSynchronize(stream));
-  return rc;
+  if (!cases.empty() && d_status) {
+    std::vector<int> st(cases.size());
+    HIPCHK(hipMemcpy(st.data(), d_status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < st.size(); k++)
+      if (st[k]) {
+        const std::string msg = "zero pivot in a CRWENO5 line solve (tridiagLU returns -1)";
+        cases[k]->err = msg;
+        return fail(-3, cases.size() > 1 ? "case " + std::to_string(k) + ": " + msg : msg);
+      }
+  }
+  return 0;
 }
 
-int Case::write_binary(const std::string &path) {
-  std::vector<double> uh((size_t)cfg.npts_int * cfg.nvars);
-  int rc = download(uh.data());
-  if (rc) return rc;
-  FILE *f = fopen(path.c_str(), "wb");
-  if (!f) return fail(2, "could not open " + path + " for writing");
-  int nd = cfg.ndims, nv = cfg.nvars;
-  fwrite(&nd, sizeof(int), 1, f);
-  fwrite(&nv, sizeof(int), 1, f);
-  fwrite(cfg.dim, sizeof(int), nd, f);
-  long off = 0;
-  for (int d = 0; d < nd; d++) {
-    fwrite(&cfg.x[off + cfg.ghosts], sizeof(double), cfg.dim[d], f);
-    off += cfg.dim[d] + 2 * cfg.ghosts;
+Engine::~Engine() {
+  if (device >= 0) hipSetDevice(device);
+  if (stream) hipStreamSynchronize(stream);
+  release_plan();
+  for (auto &e : ev) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
   }
-  fwrite(uh.data(), sizeof(double), uh.size(), f);
-  fclose(f);
-  return 0;
+  if (owns_cases)
+    for (Case *c : cases) delete c;
+  if (stream) hipStreamDestroy(stream);
 }
 
 }  // namespace piafs
 
 // ===========================================================================
-// C ABI
+// C ABI: single case = an engine with one member
 // ===========================================================================
-struct piafs_case : public piafs::Case {};
+using namespace piafs;
+
+struct piafs_case {
+  Engine eng;
+  Case *c = nullptr;
+  std::string err;
+};
 
 static std::string g_create_error;
+
+static int case_fail(piafs_case *pc, int rc) {
+  pc->err = !pc->c->err.empty() ? pc->c->err : pc->eng.err;
+  return rc;
+}
 
 extern "C" {
 
 int piafs_create(const char *dir, int device, piafs_case **out) {
   *out = nullptr;
-  piafs_case *c = new piafs_case();
-  int rc = c->init(dir ? dir : "", device);
+  piafs_case *pc = new piafs_case();
+  int rc = pc->eng.init(device);
+  if (rc) {
+    g_create_error = pc->eng.err;
+    delete pc;
+    return rc;
+  }
+  Case *c = new Case();
+  rc = c->init(dir ? dir : "", device, pc->eng.stream);
   if (rc) {
     g_create_error = c->err;
     delete c;
+    delete pc;
     return rc;
   }
-  *out = c;
+  pc->c = c;
+  pc->eng.add(c);
+  rc = pc->eng.build();
+  if (rc) {
+    g_create_error = pc->eng.err;
+    delete pc;
+    return rc;
+  }
+  *out = pc;
   return 0;
 }
 
-int piafs_destroy(piafs_case *c) {
-  delete c;
+int piafs_destroy(piafs_case *pc) {
+  delete pc;
   return 0;
 }
 
-const char *piafs_last_error(const piafs_case *c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+const char *piafs_last_error(const piafs_case *pc) { return pc ? pc->err.c_str() : g_create_error.c_str(); }
 
-int piafs_get_info(const piafs_case *c, piafs_info *info) {
-  if (!c || !info) return 1;
+}  // extern "C"
+
+namespace piafs {
+void fill_info(const Case *c, piafs_info *info) {
   memset(info, 0, sizeof(*info));
   info->ndims = c->cfg.ndims;
   info->nvars = c->cfg.nvars;
@@ -479,71 +500,141 @@ int piafs_get_info(const piafs_case *c, piafs_info *info) {
   info->characteristic = c->cfg.characteristic ? 1 : 0;
   info->include_chem = c->cfg.include_chem ? 1 : 0;
   info->device_bytes = c->bytes;
+}
+}  // namespace piafs
+
+extern "C" {
+
+int piafs_get_info(const piafs_case *pc, piafs_info *info) {
+  if (!pc || !info) return 1;
+  fill_info(pc->c, info);
   return 0;
 }
 
-int piafs_upload_state(piafs_case *c, const double *u) { return c->upload(u); }
-int piafs_download_state(piafs_case *c, double *u) { return c->download(u); }
-int piafs_set_time(piafs_case *c, double t) {
-  c->t = t;
-  return 0;
+int piafs_upload_state(piafs_case *pc, const double *u) {
+  int rc = pc->c->upload(u, pc->eng.stream);
+  return rc ? case_fail(pc, rc) : 0;
 }
-int piafs_step(piafs_case *c, int n) { return c->step(n, nullptr, nullptr); }
-int piafs_step_diag(piafs_case *c, int n, double *cfl, double *norm) {
-  int rc = c->step(n, cfl, norm);
-  if (rc) return rc;
-  return piafs_sync(c);
-}
-int piafs_sync(piafs_case *c) {
-  hipSetDevice(c->device);
-  hipError_t e = hipStreamSynchronize(c->stream);
-  if (e != hipSuccess) return c->fail(-1, std::string("stream: ") + hipGetErrorString(e));
-  return 0;
-}
-int piafs_rhs(piafs_case *c, const double *u, double t, double *rhs) { return c->rhs_host(u, t, rhs); }
-int piafs_write_binary(piafs_case *c, const char *path) { return c->write_binary(path); }
 
-int piafs_kernel_times(piafs_case *c, int enable, const char **names, double *ms, int *n) {
+int piafs_download_state(piafs_case *pc, double *u) {
+  int rc = pc->c->download(u, pc->eng.stream);
+  return rc ? case_fail(pc, rc) : 0;
+}
+
+int piafs_set_time(piafs_case *pc, double t) {
+  pc->c->t = t;
+  if (hipMemcpyAsync(pc->c->h.time, &t, sizeof(double), hipMemcpyHostToDevice, pc->eng.stream) != hipSuccess ||
+      hipStreamSynchronize(pc->eng.stream) != hipSuccess) {
+    pc->err = "set_time failed";
+    return -1;
+  }
+  return 0;
+}
+
+int piafs_step(piafs_case *pc, int n) {
+  int rc = pc->eng.step(n);
+  return rc ? case_fail(pc, rc) : 0;
+}
+
+int piafs_sync(piafs_case *pc) {
+  int rc = pc->eng.sync();
+  return rc ? case_fail(pc, rc) : 0;
+}
+
+int piafs_step_diag(piafs_case *pc, int n, double *cfl, double *norm) {
+  Case *c = pc->c;
+  const Grid &G = c->h.G;
+  hipStream_t st = pc->eng.stream;
+  std::vector<double> hbuf(2 * (size_t)c->nparts);
+  for (int it = 0; it < n; it++) {
+    if (hipMemcpyAsync(c->uprev, c->h.u, (size_t)c->cfg.nvars * G.plane * sizeof(double), hipMemcpyDeviceToDevice,
+                       st) != hipSuccess) {
+      pc->err = "copy failed";
+      return -1;
+    }
+    int rc = pc->eng.step(1);
+    if (rc) return case_fail(pc, rc);
+    launch_cfl_norm(G, c->cfg.model_id, c->cfg.gamma, c->cfg.dt, c->h.u, c->uprev, c->h.dxinv, c->partials,
+                    c->nparts, st);
+    if (hipMemcpyAsync(hbuf.data(), c->partials, hbuf.size() * sizeof(double), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      pc->err = "diagnostics failed";
+      return -1;
+    }
+    double mx = 0, sm = 0;
+    for (int b = 0; b < c->nparts; b++) {
+      mx = std::max(mx, hbuf[2 * b]);
+      sm += hbuf[2 * b + 1];
+    }
+    if (cfl) cfl[it] = mx;
+    if (norm) norm[it] = sqrt(sm / (double)c->cfg.npts_int);
+  }
+  return 0;
+}
+
+int piafs_rhs(piafs_case *pc, const double *u, double t, double *rhs) {
+  Case *c = pc->c;
+  hipStream_t st = pc->eng.stream;
+  const size_t n = (size_t)c->cfg.nvars * c->h.G.plane;
+  const double t_saved = c->t;
+  int rc = 0;
+  if (hipMemcpyAsync(c->uprev, c->h.u, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) return -1;
+  if ((rc = c->upload(u, st))) return case_fail(pc, rc);
+  if ((rc = piafs_set_time(pc, t))) return rc;
+  if ((rc = pc->eng.stage_only(0))) return case_fail(pc, rc);  // stage 0: U = u, time = t
+  if ((rc = c->download(rhs, st, c->h.Udot[0]))) return case_fail(pc, rc);
+  if (hipMemcpyAsync(c->h.u, c->uprev, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) return -1;
+  return piafs_set_time(pc, t_saved);
+}
+
+int piafs_write_binary(piafs_case *pc, const char *path) {
+  int rc = pc->c->write_binary(path, pc->eng.stream);
+  return rc ? case_fail(pc, rc) : 0;
+}
+
+int piafs_kernel_times(piafs_case *pc, int enable, const char **names, double *ms, int *n) {
+  Engine &e = pc->eng;
   if (enable >= 0) {
-    c->profiling = (enable != 0);
-    c->ev_used = 0;
+    e.profiling = (enable != 0);
+    e.ev_used = 0;
   }
   if (!n) return 0;
   int cap = *n;
-  piafs_sync(c);
-  double tot[NCLASS] = {0};
-  int cnt[NCLASS] = {0};
-  for (size_t k = 0; k < c->ev_used; k++) {
+  e.sync();
+  double tot[K_NCLASS] = {0};
+  int cnt[K_NCLASS] = {0};
+  for (size_t k = 0; k < e.ev_used; k++) {
     float m = 0;
-    hipEventElapsedTime(&m, c->ev[k].first, c->ev[k].second);
-    tot[c->ev_cls[k]] += m;
-    cnt[c->ev_cls[k]]++;
+    hipEventElapsedTime(&m, e.ev[k].first, e.ev[k].second);
+    tot[e.ev_cls[k]] += m;
+    cnt[e.ev_cls[k]]++;
   }
   int w = 0;
-  for (int k = 0; k < NCLASS && w < cap; k++) {
+  for (int k = 0; k < K_NCLASS && w < cap; k++) {
     if (!cnt[k]) continue;
-    if (names) names[w] = piafs::class_name(k);
+    if (names) names[w] = kclass_name(k);
     if (ms) ms[w] = tot[k] / cnt[k];
     w++;
   }
-  // second half of ms (if room) holds the launch counts
   *n = w;
-  c->ev_used = 0;
+  e.ev_used = 0;
   return 0;
 }
 
 int piafs_estimate_bytes(const char *dir, size_t *bytes) {
-  piafs::Config cfg;
+  Config cfg;
   std::string err;
   if (cfg.parse(dir ? dir : "", err)) {
     g_create_error = err;
     return 1;
   }
-  *bytes = piafs::Case::plan_bytes(cfg);
+  *bytes = Case::plan_bytes(cfg);
   return 0;
 }
 
-int piafs_solve(piafs_case *c, const char *dir, int verbose, double *seconds) {
+int piafs_solve(piafs_case *pc, const char *dir, int verbose, double *seconds) {
+  Case *c = pc->c;
   const Config &cf = c->cfg;
   std::string d = dir ? dir : "";
   auto outname = [&](int index) {
@@ -557,31 +648,28 @@ int piafs_solve(piafs_case *c, const char *dir, int verbose, double *seconds) {
   };
   bool binary = (cf.op_file_format == "binary" || cf.op_file_format == "bin");
   int index = 0;
-  if (cf.restart_iter) index = cf.restart_iter / cf.file_op_iter;  // InitializeSolvers.c:240-245
+  if (cf.restart_iter && cf.file_op_iter > 0) index = cf.restart_iter / cf.file_op_iter;  // InitializeSolvers.c:240-245
   auto t0 = std::chrono::steady_clock::now();
   std::vector<double> cfl(1), norm(1);
   for (int iter = cf.restart_iter; iter < cf.n_iter; iter++) {
     if (iter == 0 && binary) {
-      if (c->write_binary(outname(index))) return c->err.empty() ? -1 : 2;
+      if (c->write_binary(outname(index), pc->eng.stream)) return case_fail(pc, 2);
       index++;
     }
     bool screen = ((iter + 1) % cf.screen_op_iter == 0);
-    int rc = screen && verbose ? c->step(1, cfl.data(), norm.data()) : c->step(1, nullptr, nullptr);
+    int rc = screen && verbose ? piafs_step_diag(pc, 1, cfl.data(), norm.data()) : piafs_step(pc, 1);
     if (rc) return rc;
-    if (screen && verbose)
-      printf("iter=%7d  t=%1.3E  CFL=%1.3E  norm=%1.4E\n", iter + 1, c->t, cfl[0], norm[0]);
+    if (screen && verbose) printf("iter=%7d  t=%1.3E  CFL=%1.3E  norm=%1.4E\n", iter + 1, c->t, cfl[0], norm[0]);
     if (((iter + 1) % cf.file_op_iter == 0) && ((iter + 1) < cf.n_iter) && binary) {
-      if (c->write_binary(outname(index))) return 2;
+      if (c->write_binary(outname(index), pc->eng.stream)) return case_fail(pc, 2);
       index++;
     }
   }
-  int rc = piafs_sync(c);
+  int rc = piafs_sync(pc);
   if (rc) return rc;
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (seconds) *seconds = secs;
-  if (binary) {
-    if (c->write_binary(outname(index))) return 2;
-  }
+  if (binary && c->write_binary(outname(index), pc->eng.stream)) return case_fail(pc, 2);
   return 0;
 }
 
