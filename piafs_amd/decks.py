@@ -315,3 +315,81 @@ def rel_l2_per_var(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     num = np.sqrt(((a - b) ** 2).sum(axis=0))
     den = np.sqrt((b ** 2).sum(axis=0))
     return np.where(den > 0, num / np.where(den > 0, den, 1.0), num)
+
+
+# ---------------------------------------------------------------------------
+# Ensembles and request streams
+# ---------------------------------------------------------------------------
+CHEM_RATE_PER_RE = 0.145   # fastest chemical rate of the O2/O3 set, per unit Re (normalised time)
+
+
+def stable_dt(Re: float, dt_max: float = 0.01, safety: float = 1.5) -> float:
+    """Largest fixed step for classical RK4 on the photochemistry source: the
+    O(1D) quenching rate grows with Ptot, i.e. with Re (rate about
+    CHEM_RATE_PER_RE*Re), and RK4 is stable for dt*rate < 2.78; `safety`
+    keeps dt*rate at 1.5.  dt_max is the acoustic limit of the decks."""
+    return min(dt_max, safety / (CHEM_RATE_PER_RE * Re))
+
+
+@dataclass
+class Member:
+    name: str
+    deck: Deck
+    cells: int          # interior points
+    steps: int
+
+
+def ensemble(n: int = 256, seed: int = 20261202, n_iter: int = 200, side=(128, 512),
+             Re_range=(1e2, 1e4), amp_range=(0.5, 2.0)) -> list:
+    """Independent 2D NS + photochemistry cases (case k drawn from SplitMix64
+    stream seed + k): sides log-uniform in `side` (x and y independently),
+    beam amplitude uniform in `amp_range`, Re log-uniform in `Re_range` (set
+    through Ptot), WENO5 / CRWENO5 with equal probability; fixed dx = 10 pi /
+    1024; dt = stable_dt(Re); seeded density perturbation as the single-case
+    deck."""
+    dx = 10.0 * math.pi / 1024
+    out = []
+    lo, hi = math.log(side[0]), math.log(side[1])
+    for k in range(n):
+        r = splitmix64_fast(seed + k, 5)
+        nx = int(round(math.exp(lo + (hi - lo) * r[0])))
+        ny = int(round(math.exp(lo + (hi - lo) * r[1])))
+        amp = amp_range[0] + (amp_range[1] - amp_range[0]) * r[2]
+        Re = math.exp(math.log(Re_range[0]) + (math.log(Re_range[1]) - math.log(Re_range[0])) * r[3])
+        scheme = "crweno5" if r[4] < 0.5 else "weno5"
+        dk = ns2d_photochem(nx=nx, ny=ny, n_iter=n_iter, dt=stable_dt(Re), seed=seed + k, scheme=scheme,
+                            Re=Re, amplitude=amp, dx=dx, L=nx * dx)
+        out.append(Member(f"c{k:03d}_{nx}x{ny}_{scheme}_Re{Re:.0f}", dk, nx * ny, n_iter))
+    return out
+
+
+@dataclass
+class Request:
+    arrival: float      # seconds after the start of the stream
+    member: Member
+
+
+def request_stream(n: int, rate: float, seed: int = 20261203, frac_1d: float = 0.3,
+                   n1d=(512, 8192), side2d=(128, 1024)) -> list:
+    """Poisson arrivals (rate per second) of solve requests to a fixed final
+    time: a fraction frac_1d are Sod shock tubes (N log-uniform in n1d,
+    t_final 0.2, the single-case 1D deck), the rest 2D NS + photochemistry
+    (side log-uniform in side2d, the single-case 2D deck with dx = 10 pi /
+    1024, Re 1000, dt 0.01, t_final 2.0)."""
+    dx = 10.0 * math.pi / 1024
+    out = []
+    t = 0.0
+    for k in range(n):
+        r = splitmix64_fast(seed + 7919 * k, 3)
+        t += -math.log(1.0 - r[0]) / rate
+        if r[1] < frac_1d:
+            N = int(round(math.exp(math.log(n1d[0]) + (math.log(n1d[1]) - math.log(n1d[0])) * r[2])))
+            dk = sod_1d(N=N)
+            m = Member(f"r{k:03d}_sod{N}", dk, N, int(dk.solver["n_iter"]))
+        else:
+            a = math.log(side2d[0])
+            side = int(round(math.exp(a + (math.log(side2d[1]) - a) * r[2])))
+            dk = ns2d_photochem(nx=side, n_iter=200, dt=0.01, seed=seed + k, dx=dx, L=side * dx)
+            m = Member(f"r{k:03d}_ns{side}", dk, side * side, 200)
+        out.append(Request(t, m))
+    return out
