@@ -85,6 +85,128 @@ def cpu_baseline(tmp, nx_sample=256, steps=10):
             "sample": sample + "; CPU restatement (oracle/liboracle.so), serial", "seconds": secs}
 
 
+def _write_members(tmp, members, sub):
+    dirs = []
+    for m in members:
+        d = os.path.join(tmp, sub, m.name)
+        m.deck.write(d)
+        dirs.append(d)
+    return dirs
+
+
+def bench_batch(tmp, n_cases=256, steps=20, serial_steps=10, oracle_check=True):
+    """Ensemble of independent 2D NS + photochemistry cases (sides log-uniform
+    in [128, 512], Re log-uniform in [1e2, 1e4], beam amplitude in [0.5, 2],
+    WENO5/CRWENO5 50/50): the whole ensemble advanced by the batched launches
+    (one per kernel class and RK stage, replayed from a HIP graph) against the
+    same cases solved one at a time through the single-case path."""
+    import piafs_amd
+    from piafs_amd import decks
+    warm = 1
+    members = decks.ensemble(n_cases, n_iter=warm + steps)
+    t0 = time.perf_counter()
+    dirs = _write_members(tmp, members, "ens")
+    t_write = time.perf_counter() - t0
+    cells = sum(m.cells for m in members)
+    b = piafs_amd.Batch(dirs)
+    b.step(warm)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    t_b = time.perf_counter() - t0
+    out = {"cases": n_cases, "cells_per_step": cells, "steps": steps,
+           "batched_cell_updates_per_s": cells * steps / t_b, "batched_ms_per_step": 1e3 * t_b / steps,
+           "deck_write_s": t_write}
+    if oracle_check:
+        k = min(range(n_cases), key=lambda i: members[i].cells)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _oracle as orc
+        o = orc.OCase(dirs[k])
+        o.step(warm + steps)
+        ref = o.solution()
+        got = b.solution(k)
+        err = np.sqrt(((got - ref) ** 2).sum(0)) / np.maximum(np.sqrt((ref ** 2).sum(0)), 1e-300)
+        out["oracle_check"] = {"case": members[k].name, "steps": warm + steps, "max_rel_l2_per_var": float(err.max())}
+    b.close()
+    # one case at a time (eager launches of the single-case path, as a caller of piafs_step)
+    t_s = 0.0
+    for d in dirs:
+        c = piafs_amd.Case(d)
+        c.step(1)
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(serial_steps)
+        c.sync()
+        t_s += time.perf_counter() - t0
+        c.close()
+    out["serial_steps"] = serial_steps
+    out["serial_cell_updates_per_s"] = cells * serial_steps / t_s
+    out["speedup_vs_one_at_a_time"] = out["batched_cell_updates_per_s"] / out["serial_cell_updates_per_s"]
+    return out
+
+
+def bench_serve(tmp, n_requests=24, rate=None, rate_frac=0.7, batched_rate=None):
+    """Poisson stream of solve requests (30% Sod 1D, N in [512, 8192], to
+    t = 0.2; 70% 2D NS + photochemistry, side in [128, 1024], 200 steps)
+    served with continuous batching under an HBM budget, against the same
+    stream solved one request at a time in arrival order (FIFO)."""
+    import threading
+    import piafs_amd
+    from piafs_amd import decks
+    probe = decks.request_stream(n_requests, 1.0)
+    work = sum(r.member.cells * r.member.steps for r in probe)
+    if rate is None:
+        # arrivals at rate_frac of the rate the batched engine can absorb
+        rate = rate_frac * (batched_rate or 1.5e8) / (work / n_requests)
+    reqs = decks.request_stream(n_requests, rate)
+    dirs = _write_members(tmp, [r.member for r in reqs], "srv")
+    lat = [None] * n_requests
+    with piafs_amd.Server() as srv:
+        ids = [None] * n_requests
+        t_start = time.perf_counter()
+
+        def feeder():
+            for k, r in enumerate(reqs):
+                dt = t_start + r.arrival - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids[k] = srv.submit(dirs[k])
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        srv.wait_all()
+        t_end = time.perf_counter()
+        for k in range(n_requests):
+            st, lat[k] = srv.poll(ids[k])
+            if st != srv.DONE:
+                raise RuntimeError(f"request {k} ended in state {st}")
+    lat = np.array(lat)
+    out = {"requests": n_requests, "rate_per_s": rate, "cell_updates": work,
+           "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+           "throughput_cell_updates_per_s": work / (t_end - t_start)}
+    # the same stream, one request at a time in arrival order
+    t_free, lat_s, busy = 0.0, [], 0.0
+    for k, r in enumerate(reqs):
+        t0 = time.perf_counter()
+        c = piafs_amd.Case(dirs[k])
+        c.step(r.member.steps)
+        c.sync()
+        c.close()
+        dur = time.perf_counter() - t0
+        busy += dur
+        start = max(r.arrival, t_free)
+        t_free = start + dur
+        lat_s.append(t_free - r.arrival)
+    lat_s = np.array(lat_s)
+    out["serial_p50_latency_s"] = float(np.percentile(lat_s, 50))
+    out["serial_p99_latency_s"] = float(np.percentile(lat_s, 99))
+    out["serial_throughput_cell_updates_per_s"] = work / max(t_free, busy)
+    out["p99_vs_serial"] = out["p99_latency_s"] / out["serial_p99_latency_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +215,11 @@ def main():
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the ensemble leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--batch-cases", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--parity", action="store_true", help="check the final state against the reference fixture")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device-to-device copy of MB megabytes (10 times), the reference "
@@ -173,6 +300,11 @@ def main():
                 line["parity"] = {"vs": "reference run (oracle/_ref) 1024^2 x 200 steps, stride-32 subsample",
                                   "max_rel_l2_per_var": float(err.max())}
         case.close()
+        if not args.no_batch:
+            line["batched"] = bench_batch(tmp, n_cases=args.batch_cases, steps=args.batch_steps)
+        if not args.no_serve:
+            br = line.get("batched", {}).get("batched_cell_updates_per_s")
+            line["serve"] = bench_serve(tmp, n_requests=args.serve_requests, batched_rate=br)
         if not args.no_cpu_baseline:
             line["cpu_baseline"] = cpu_baseline(tmp)
     print(json.dumps(line), flush=True)

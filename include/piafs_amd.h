@@ -150,6 +150,36 @@ const char *piafs_batch_last_error(const piafs_batch *b);
  * serving).  Returns 0 on success. */
 int  piafs_estimate_bytes(const char *dir, size_t *bytes);
 
+/* ---------------------------------------------------------------------------
+ * Serving: a request loop with continuous batching.  No counterpart in the
+ * reference, which runs one simulation per process (src/main.cpp:133-235) or
+ * a serial loop over an ensemble (EnsembleSimulationsDefine.cpp:31-119).
+ * A request is the solve of one deck directory to its n_iter steps.  Requests
+ * are parsed on the submitting thread, queued, and admitted in arrival order
+ * while the device memory they hold fits the budget (hbm_budget = 0: 80% of
+ * the free device memory at create); admitted requests stay resident and are
+ * advanced together with the other members of their class (1D or 2D) by the
+ * ensemble launches, each class on its own stream and host thread; members
+ * join and leave between rounds, and every result equals that of the case
+ * run alone.  Thread-safe: any thread may submit and poll.
+ */
+typedef struct piafs_server piafs_server;
+int  piafs_server_create(int device, size_t hbm_budget, piafs_server **out);
+int  piafs_server_destroy(piafs_server *s);
+/* Returns 1 for an invalid deck, 2 for a request larger than the budget. */
+int  piafs_server_submit(piafs_server *s, const char *dir, int *id);
+/* state: 0 queued, 1 running, 2 done, < 0 failed; latency (seconds from
+ * submit to completion) once done or failed, else -1. */
+int  piafs_server_poll(piafs_server *s, int id, int *state, double *latency_seconds);
+/* Final state of a finished request (WriteBinary order) and its info. */
+int  piafs_server_result(piafs_server *s, int id, double *u_host, piafs_info *info);
+/* Forget a finished request (frees its host copy of the result). */
+int  piafs_server_release(piafs_server *s, int id);
+/* Block until no request is queued or running. */
+int  piafs_server_wait_all(piafs_server *s);
+int  piafs_server_stats(piafs_server *s, size_t *bytes_in_use, int *running, int *queued);
+const char *piafs_server_last_error(const piafs_server *s);
+
 #ifdef __cplusplus
 }
 #endif

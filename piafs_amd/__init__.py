@@ -44,7 +44,9 @@ EXPORTS = [
     "piafs_weno_weights_host", "piafs_batch_create", "piafs_batch_destroy", "piafs_batch_step",
     "piafs_batch_sync", "piafs_batch_use_graph", "piafs_batch_add", "piafs_batch_remove",
     "piafs_batch_steps_done", "piafs_batch_download", "piafs_batch_info", "piafs_batch_last_error",
-    "piafs_estimate_bytes",
+    "piafs_estimate_bytes", "piafs_server_create", "piafs_server_destroy", "piafs_server_submit",
+    "piafs_server_poll", "piafs_server_result", "piafs_server_release", "piafs_server_wait_all",
+    "piafs_server_stats", "piafs_server_last_error",
 ]
 
 
@@ -87,6 +89,18 @@ def lib():
     L.piafs_batch_last_error.argtypes = [vp]
     L.piafs_batch_last_error.restype = cp
     L.piafs_estimate_bytes.argtypes = [cp, ctypes.POINTER(ctypes.c_size_t)]
+    L.piafs_server_create.argtypes = [ip, ctypes.c_size_t, ctypes.POINTER(vp)]
+    L.piafs_server_destroy.argtypes = [vp]
+    L.piafs_server_submit.argtypes = [vp, cp, ctypes.POINTER(ctypes.c_int)]
+    L.piafs_server_poll.argtypes = [vp, ip, ctypes.POINTER(ctypes.c_int), dp]
+    L.piafs_server_result.argtypes = [vp, ip, vp, ctypes.POINTER(Info)]
+    L.piafs_server_release.argtypes = [vp, ip]
+    L.piafs_server_wait_all.argtypes = [vp]
+    L.piafs_server_wait_all.restype = ctypes.c_int
+    L.piafs_server_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int)]
+    L.piafs_server_last_error.argtypes = [vp]
+    L.piafs_server_last_error.restype = cp
     _lib = L
     return L
 
@@ -271,6 +285,92 @@ class Batch:
     def close(self):
         if getattr(self, "_h", None):
             self._L.piafs_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching (piafs_server_* in the C ABI).
+
+    ``submit(dir)`` queues the solve of a deck directory to its n_iter steps
+    and returns a request id; requests are admitted while their device memory
+    fits ``hbm_budget`` (bytes; 0 = 80% of the free device memory), stay
+    resident, and advance together with the other requests of their class.
+    ``poll(id)`` gives (state, latency seconds): state 0 queued, 1 running,
+    2 done, < 0 failed."""
+
+    QUEUED, RUNNING, DONE = 0, 1, 2
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.piafs_server_create(int(device), int(hbm_budget), ctypes.byref(h))
+        self._h, self._L = h, L
+        if rc:
+            msg = L.piafs_server_last_error(h).decode() if h else ""
+            self.close()
+            raise PiafsError(f"piafs_server_create failed ({rc}): {msg}")
+
+    def _err(self):
+        return self._L.piafs_server_last_error(self._h).decode()
+
+    def submit(self, directory: str) -> int:
+        i = ctypes.c_int(-1)
+        rc = self._L.piafs_server_submit(self._h, directory.encode(), ctypes.byref(i))
+        if rc:
+            raise PiafsError(f"submit failed ({rc}): {self._err()}")
+        return i.value
+
+    def poll(self, rid: int):
+        st, lat = ctypes.c_int(0), ctypes.c_double(0)
+        if self._L.piafs_server_poll(self._h, int(rid), ctypes.byref(st), ctypes.byref(lat)):
+            raise PiafsError(f"unknown request {rid}")
+        return st.value, lat.value
+
+    def result(self, rid: int) -> np.ndarray:
+        inf = Info()
+        st, _ = self.poll(rid)
+        if st < 0:
+            raise PiafsError(f"request {rid} failed: {self._err()}")
+        if st != self.DONE:
+            raise PiafsError(f"request {rid} not finished (state {st})")
+        rc = self._L.piafs_server_result(self._h, int(rid), None, ctypes.byref(inf))
+        if rc:
+            raise PiafsError(f"result failed ({rc}): {self._err()}")
+        n = 1
+        for d in range(inf.ndims):
+            n *= inf.dim[d]
+        out = np.empty((n, inf.nvars))
+        rc = self._L.piafs_server_result(self._h, int(rid), _ptr(out), None)
+        if rc:
+            raise PiafsError(f"result failed ({rc}): {self._err()}")
+        return out
+
+    def release(self, rid: int):
+        self._L.piafs_server_release(self._h, int(rid))
+
+    def wait_all(self):
+        self._L.piafs_server_wait_all(self._h)
+
+    def stats(self):
+        b, r, q = ctypes.c_size_t(0), ctypes.c_int(0), ctypes.c_int(0)
+        self._L.piafs_server_stats(self._h, ctypes.byref(b), ctypes.byref(r), ctypes.byref(q))
+        return {"bytes_in_use": b.value, "running": r.value, "queued": q.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.piafs_server_destroy(self._h)
             self._h = None
 
     def __enter__(self):

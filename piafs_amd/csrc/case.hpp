@@ -29,7 +29,10 @@ struct Case {
   int nparts = 0;
 
   ~Case();
+  // parse the decks in `dir` (host only), then allocate and upload
   int init(const std::string &dir, int dev, hipStream_t st);
+  // device half of init for a configuration already parsed into cfg
+  int setup(int dev, hipStream_t st);
   int fail(int code, const std::string &msg);
   double *dalloc(size_t n, hipStream_t st);
   int upload(const double *uh, hipStream_t st);

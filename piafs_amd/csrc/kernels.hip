@@ -986,9 +986,6 @@ __global__ void __launch_bounds__(256) k_upwind(CASE_ARGS) {
   }
 }
 
-// hyp = dxinv * (fI[k+1] - fI[k]) along x: HyperbolicFunction.c:97-112.
-// Output is the compact interior array h[(v*N + k)*nlines + l] (transposed in 2D).
-
 // ---------------------------------------------------------------------------
 // Parabolic term (navierstokes2d): NavierStokes2DParabolicFunction.c:41-204.
 // Serial-run semantics: d/dx of Q is evaluated on interior rows only and d/dy

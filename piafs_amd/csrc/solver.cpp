@@ -74,6 +74,10 @@ size_t Case::plan_bytes(const Config &c) {
 
 int Case::init(const std::string &dir, int dev, hipStream_t st) {
   if (cfg.parse(dir, err)) return 1;
+  return setup(dev, st);
+}
+
+int Case::setup(int dev, hipStream_t st) {
   device = dev;
   HIPCHK(hipSetDevice(device));
   CaseDev &D = h;
