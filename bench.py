@@ -109,6 +109,7 @@ def bench_batch(tmp, n_cases=256, steps=20, serial_steps=10, oracle_check=True):
     t_write = time.perf_counter() - t0
     cells = sum(m.cells for m in members)
     b = piafs_amd.Batch(dirs)
+    tuned_rows, _ = b.autotune()
     b.step(warm)
     b.sync()
     t0 = time.perf_counter()
@@ -117,7 +118,7 @@ def bench_batch(tmp, n_cases=256, steps=20, serial_steps=10, oracle_check=True):
     t_b = time.perf_counter() - t0
     out = {"cases": n_cases, "cells_per_step": cells, "steps": steps,
            "batched_cell_updates_per_s": cells * steps / t_b, "batched_ms_per_step": 1e3 * t_b / steps,
-           "deck_write_s": t_write}
+           "deck_write_s": t_write, "line_solve_rows_per_lane": tuned_rows}
     if oracle_check:
         k = min(range(n_cases), key=lambda i: members[i].cells)
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -181,7 +182,7 @@ def bench_serve(tmp, n_requests=24, rate=None, rate_frac=0.7, batched_rate=None)
         for k in range(n_requests):
             st, lat[k] = srv.poll(ids[k])
             if st != srv.DONE:
-                raise RuntimeError(f"request {k} ended in state {st}")
+                raise RuntimeError(f"request {k} ({reqs[k].member.name}) ended in state {st}: {srv._err()}")
     lat = np.array(lat)
     out = {"requests": n_requests, "rate_per_s": rate, "cell_updates": work,
            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
@@ -212,7 +213,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nx", type=int, default=1024)
+    ap.add_argument("--nx", type=int, default=None,
+                    help="grid side (same domain 10 pi, dx = 10 pi / nx; dt 0.01 * 1024 / nx)")
+    ap.add_argument("--config", type=int, default=2, choices=(2, 5),
+                    help="2: 1024^2, dt 0.01 (default); 5: 4096^2, dt 0.0025, 50 steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the ensemble leg")
@@ -243,14 +247,18 @@ def main():
     import piafs_amd
     from piafs_amd import decks
 
-    nx = args.nx
+    nx = args.nx or (4096 if args.config == 5 else 1024)
+    if args.config == 5 and args.steps == 200:
+        args.steps = 50
+    dt = 0.01 * 1024 / nx   # same acoustic CFL on the same domain
     with tempfile.TemporaryDirectory() as tmp:
-        dk = decks.ns2d_photochem(nx=nx, n_iter=args.warmup + args.steps, dx=10.0 * np.pi / 1024 if nx != 1024 else None)
+        dk = decks.ns2d_photochem(nx=nx, n_iter=args.warmup + args.steps, dt=dt)
         d = os.path.join(tmp, "case")
         dk.write(d)
         case = piafs_amd.Case(d, device=0)
         inf = case.info
         nv, g = inf.nvars, inf.ghosts
+        tuned_rows, _ = case.autotune()
         case.step(args.warmup)
         case.sync()
         case.profile(not args.no_profile)
@@ -267,8 +275,9 @@ def main():
             "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"2D NavierStokes + photochemistry {nx}x{nx}, nvars {nv}, CRWENO5 components, "
-                                   f"Roe, 4th-order viscous (Re 1000), RK4 dt 0.01",
-                       "grid": [nx, nx], "nvars": nv, "steps": args.steps, "parallelism": "single GPU"},
+                                   f"Roe, 4th-order viscous (Re 1000), RK4 dt {dt:g}",
+                       "grid": [nx, nx], "nvars": nv, "steps": args.steps, "parallelism": "single GPU",
+                       "line_solve_rows_per_lane": tuned_rows},
         }
         if ktimes:
             # per-class average launch time (ms) x launches per step -> share of the step

@@ -97,6 +97,13 @@ int  piafs_solve(piafs_case *c, const char *dir, int verbose, double *seconds);
  * names/ms arrays of length *n (in: capacity, out: count). */
 int  piafs_kernel_times(piafs_case *c, int enable, const char **names, double *ms, int *n);
 
+/* Tile-shape autotuner (no counterpart in the reference): times the
+ * reconstruction launch for line-solve segments of 4, 8, 12 and 16 rows per
+ * lane on this case and keeps the fastest.  Results do not depend on the
+ * choice (the line solves are exact).  Optional outputs: the chosen rows per
+ * lane and its reconstruction time in ms. */
+int  piafs_autotune(piafs_case *c, int *rows_per_lane, double *recon_ms);
+
 /* ---------------------------------------------------------------------------
  * Per-operator entry points for unit parity (host arrays in, host arrays out;
  * the device work runs on an internal stream and is synchronised).
@@ -142,6 +149,8 @@ int  piafs_batch_use_graph(piafs_batch *b, int on);
 int  piafs_batch_add(piafs_batch *b, const char *dir, int *slot);
 int  piafs_batch_remove(piafs_batch *b, int slot);
 int  piafs_batch_steps_done(piafs_batch *b, int slot, int *steps);
+/* piafs_autotune over all members of the batch together. */
+int  piafs_batch_autotune(piafs_batch *b, int *rows_per_lane, double *recon_ms);
 int  piafs_batch_download(piafs_batch *b, int slot, double *u_host);
 int  piafs_batch_info(piafs_batch *b, int slot, piafs_info *info);
 const char *piafs_batch_last_error(const piafs_batch *b);

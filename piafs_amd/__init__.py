@@ -44,7 +44,7 @@ EXPORTS = [
     "piafs_weno_weights_host", "piafs_batch_create", "piafs_batch_destroy", "piafs_batch_step",
     "piafs_batch_sync", "piafs_batch_use_graph", "piafs_batch_add", "piafs_batch_remove",
     "piafs_batch_steps_done", "piafs_batch_download", "piafs_batch_info", "piafs_batch_last_error",
-    "piafs_estimate_bytes", "piafs_server_create", "piafs_server_destroy", "piafs_server_submit",
+    "piafs_estimate_bytes", "piafs_autotune", "piafs_batch_autotune", "piafs_server_create", "piafs_server_destroy", "piafs_server_submit",
     "piafs_server_poll", "piafs_server_result", "piafs_server_release", "piafs_server_wait_all",
     "piafs_server_stats", "piafs_server_last_error",
 ]
@@ -89,6 +89,8 @@ def lib():
     L.piafs_batch_last_error.argtypes = [vp]
     L.piafs_batch_last_error.restype = cp
     L.piafs_estimate_bytes.argtypes = [cp, ctypes.POINTER(ctypes.c_size_t)]
+    L.piafs_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
+    L.piafs_batch_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
     L.piafs_server_create.argtypes = [ip, ctypes.c_size_t, ctypes.POINTER(vp)]
     L.piafs_server_destroy.argtypes = [vp]
     L.piafs_server_submit.argtypes = [vp, cp, ctypes.POINTER(ctypes.c_int)]
@@ -193,6 +195,13 @@ class Case:
     def profile(self, enable: bool) -> None:
         self._L.piafs_kernel_times(self._h, 1 if enable else 0, None, None, None)
 
+    def autotune(self):
+        """Pick the line-solve segment length by timing (piafs_autotune); returns
+        (rows per lane, reconstruction ms)."""
+        r, ms = ctypes.c_int(0), ctypes.c_double(0)
+        self._check(self._L.piafs_autotune(self._h, ctypes.byref(r), ctypes.byref(ms)))
+        return r.value, ms.value
+
     def kernel_times(self) -> dict:
         cap = 16
         names = (ctypes.c_char_p * cap)()
@@ -262,6 +271,12 @@ class Batch:
     def remove(self, k: int):
         self._check(self._L.piafs_batch_remove(self._h, int(k)), "batch remove")
         self.slots.remove(k)
+
+    def autotune(self):
+        """piafs_batch_autotune: (rows per lane, reconstruction ms)."""
+        r, ms = ctypes.c_int(0), ctypes.c_double(0)
+        self._check(self._L.piafs_batch_autotune(self._h, ctypes.byref(r), ctypes.byref(ms)), "batch autotune")
+        return r.value, ms.value
 
     def steps_done(self, k: int) -> int:
         n = ctypes.c_int(0)

@@ -146,6 +146,14 @@ int piafs_batch_info(piafs_batch *b, int k, piafs_info *info) {
   return 0;
 }
 
+int piafs_batch_autotune(piafs_batch *b, int *rows_per_lane, double *recon_ms) {
+  static const int cands[] = {4, 8, 12, 16};
+  b->eng.sync();
+  int rc = b->eng.autotune(cands, 4, 3, rows_per_lane, recon_ms);
+  if (rc) b->err = b->eng.err;
+  return rc;
+}
+
 int piafs_batch_steps_done(piafs_batch *b, int k, int *steps) {
   if (!slot_case(b, k) || !steps) return 1;
   *steps = b->steps[k];

@@ -69,10 +69,14 @@ struct Engine {
   ~Engine();
   int init(int dev);
   int add(Case *c);
+  bool built = false;          // plan and descriptor table valid for the member set
   int build();                 // descriptor table and plans for the current member set
+  int prepare();               // capture the step graph now (when graphs are used)
   void enqueue_step();         // eager launches of one full step
   int step(int n);             // n steps (graph replay when enabled)
   int stage_only(int s);       // one stage (used for a single RHS evaluation)
+  double time_class(int cls, int reps);
+  int autotune(const int *cands, int ncand, int reps, int *best_rows, double *best_ms);
   int sync();
   void mark(int cls, bool begin);
   int fail(int code, const std::string &msg);

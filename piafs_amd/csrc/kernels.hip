@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(256) k_flux(CASE_ARGS) {
 //     until no entry changes.  Lane 0 starts from the exact state, so the
 //     fixed point is reached after at most T passes and is exactly the
 //     sequential sweep; with the estimates (and the sweep's contraction) the
-//     ends are already exact after the first pass, so two passes are typical;
+//     ends are usually exact after the first pass, so two passes are typical;
 //  3. the back substitution is treated the same way from the top.
 // The systems are never written to memory; only the solution is.
 // ---------------------------------------------------------------------------
@@ -492,25 +492,43 @@ __device__ __forceinline__ bool team_any(bool p, int T) {
   return __any(p) != 0;
 }
 
+// v_rcp_f64 refined by two Newton steps: a reciprocal good to the last bits,
+// used only for the estimates that start the exact passes
+__device__ __forceinline__ double rcp_est(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(r, fma(-b, r, 1.0), r);
+  r = fma(r, fma(-b, r, 1.0), r);
+  return r;
+}
+
 // Solve one team's system.  On entry the n rows of this lane hold
 // (A, B, Cc, X) = (sub-, main, super-diagonal, right-hand side); on exit X
 // holds the solution and B the pivots.  The row before the team's first row
 // is an identity row carrying xfirst; the row after the team's last row (held
 // by the lane with is_last) is one carrying xlast.  Every lane of every team
 // of the block must call this (it may synchronise the block).
+// The multipliers of the
+// pivot chain (fac = a_k / b'_{k-1}, one division each) are kept per row, so
+// the x' chain of a pass costs no division: every pass takes the segments'
+// entry states, runs the x' chain along the kept multipliers, and relays the
+// segment ends; only when an entry pivot changes are the multipliers derived
+// again.  Once no entry changes, one pass without divisions writes b' and x'
+// (fac * c and fac * x' exactly as the reference computes them).  The back
+// substitution iterates the same way from the top.  If `plain_start` is set
+// the scanned estimates are replaced by the trivial ones (b' = 1, x' = 0), so
+// that the relay runs through many passes (testing).
 template <int MR>
 __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], double (&Cc)[MR], double (&X)[MR],
-                                           int n, int lane, int T, bool is_last, double xfirst, double xlast,
-                                           double *red, int *status) {
+                                                int n, int lane, int T, bool is_last, double xfirst, double xlast,
+                                                double *red, int *status, bool plain_start) {
   const bool has = n > 0;
-  // c of the row before this lane's first row (the identity row's c = 0 for lane 0)
   double cin = 0.0, unused = 0.0;
 #pragma unroll
   for (int i = 0; i < MR; i++)
     if (i == n - 1) cin = Cc[i];
   team_shift2(cin, unused, lane, T, +1, red);
   if (lane == 0) cin = 0.0;
-  // 1a. pivot entering the segment: M_k = [[b_k, -a_k c_{k-1}], [1, 0]] on (b'_{k-1}, 1)
+  // pivot entering the segment: scan of M_k = [[b_k, -a_k c_{k-1}], [1, 0]] on (b'_{k-1}, 1)
   Mob P = {1.0, 0.0, 0.0, 1.0};
 #pragma unroll
   for (int i = 0; i < MR; i++) {
@@ -522,93 +540,102 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
   }
   P = mob_norm(P);
   const Mob E = team_scan_mob(P, lane, T, red);
-  double bin = lane == 0 ? 1.0 : (E.a + E.b) / (E.c + E.d);
-  // 1b. x' entering the segment (affine maps along the estimated pivots)
+  double bin = lane == 0 ? 1.0 : (plain_start ? 1.0 : (E.a + E.b) / (E.c + E.d));
+  // multipliers along the estimated pivots; affine maps of x'
+  double Fk[MR];
+  double bend = bin;
   Aff F = {lane == 0 ? 0.0 : 1.0, lane == 0 ? xfirst : 0.0};
   {
-    double b = bin, c = cin;
+    double c = cin;
 #pragma unroll
     for (int i = 0; i < MR; i++) {
       if (i < n) {
-        const double fac = A[i] / b;
-        b = B[i] - fac * c;
+        const double fac = A[i] / bend;
+        bend = B[i] - fac * c;
         c = Cc[i];
+        Fk[i] = fac;
         F = Aff{-(fac * F.a), X[i] - fac * F.b};
+      } else {
+        Fk[i] = 0.0;
       }
     }
   }
   const Aff Ef = team_scan_aff_fwd(F, lane, T, red);
-  double xin = lane == 0 ? xfirst : Ef.a * xfirst + Ef.b;
-  // 2. forward sweep: pass until every entry state equals its predecessor's end
+  double xin = lane == 0 ? xfirst : (plain_start ? 0.0 : Ef.a * xfirst + Ef.b);
+  // relay of segment ends until every entry equals its predecessor's end
   for (int it = 0;; it++) {
-    double b = bin, x = xin, c = cin;
+    double xend = xin;
 #pragma unroll
-    for (int i = 0; i < MR; i++) {
-      if (i < n) {
-        const double fac = A[i] / b;
-        b = B[i] - fac * c;
-        x = X[i] - fac * x;
-        c = Cc[i];
+    for (int i = 0; i < MR; i++)
+      if (i < n) xend = X[i] - Fk[i] * xend;
+    double nb = bend;
+    team_shift2(nb, xend, lane, T, +1, red);
+    const bool bchg = lane != 0 && !(nb == bin);
+    const bool changed = has && lane != 0 && (bchg || !(xend == xin));
+    if (lane != 0) {
+      bin = nb;
+      xin = xend;
+    }
+    if (!team_any(changed, T) || it > T) break;
+    if (bchg) {  // pivots of this segment derived again from the new entry
+      double c = cin;
+      bend = bin;
+#pragma unroll
+      for (int i = 0; i < MR; i++) {
+        if (i < n) {
+          const double fac = A[i] / bend;
+          bend = B[i] - fac * c;
+          c = Cc[i];
+          Fk[i] = fac;
+        }
       }
     }
-    team_shift2(b, x, lane, T, +1, red);
-    const bool changed = has && lane != 0 && !(b == bin && x == xin);
-    if (lane != 0) {
-      bin = b;
-      xin = x;
-    }
-    if (!team_any(changed, T) || it >= T) break;
   }
-  bool zp = false;
+  // b' and x' of every row, in the reference's arithmetic
+  bool zp = (bin == 0.0) && has;
   {
     double b = bin, x = xin, c = cin;
 #pragma unroll
     for (int i = 0; i < MR; i++) {
       if (i < n) {
-        zp = zp || (b == 0.0);
-        const double fac = A[i] / b;
-        b = B[i] - fac * c;
-        x = X[i] - fac * x;
+        b = B[i] - Fk[i] * c;
+        x = X[i] - Fk[i] * x;
         c = Cc[i];
+        zp = zp || (b == 0.0);
         B[i] = b;
         X[i] = x;
       }
     }
   }
-  // 3. back substitution from the top: estimate of x entering from above
+  // back substitution: estimate of x entering from above, then the relay
   Aff Gm = {is_last ? 0.0 : 1.0, is_last ? xlast : 0.0};
 #pragma unroll
   for (int i = MR - 1; i >= 0; i--) {
     if (i < n) {
-      const double rb = 1.0 / B[i];
+      const double rb = rcp_est(B[i]);
       Gm = Aff{-(Cc[i] * rb) * Gm.a, (X[i] - Cc[i] * Gm.b) * rb};
     }
   }
   const Aff Eb = team_scan_aff_bwd(Gm, lane, T, red);
-  double yin = is_last ? xlast : Eb.a * xlast + Eb.b;
+  double yin = is_last ? xlast : (plain_start ? 0.0 : Eb.a * xlast + Eb.b);
   for (int it = 0;; it++) {
-    double x = yin;
-#pragma unroll
-    for (int i = MR - 1; i >= 0; i--)
-      if (i < n) x = (X[i] - Cc[i] * x) / B[i];
-    double dummy = 0.0;
-    team_shift2(x, dummy, lane, T, -1, red);
-    const bool changed = has && !is_last && !(x == yin);
-    if (!is_last) yin = x;
-    if (!team_any(changed, T) || it >= T) break;
-  }
-  {
-    double x = yin;
+    double xv = yin, dummy = 0.0;
 #pragma unroll
     for (int i = MR - 1; i >= 0; i--) {
       if (i < n) {
-        zp = zp || (B[i] == 0.0);
-        x = (X[i] - Cc[i] * x) / B[i];
-        X[i] = x;
+        xv = (X[i] - Cc[i] * xv) / B[i];
+        Fk[i] = xv;
       }
     }
+    team_shift2(xv, dummy, lane, T, -1, red);
+    const bool changed = has && !is_last && !(xv == yin);
+    if (!is_last) yin = xv;
+    if (!team_any(changed, T) || it > T) break;
   }
-  if (zp && has) atomicOr(status, 1);
+#pragma unroll
+  for (int i = 0; i < MR; i++)
+    if (i < n) X[i] = Fk[i];
+  if (zp) atomicOr(status, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -627,10 +654,9 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
 //    solve (team_solve above).
 // ---------------------------------------------------------------------------
 template <int CR, int KIND>
-__device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, int lane, double *red) {
+__device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, bool active, int lane, double *red) {
   const SweepGeom &S = dir ? C.Sy : C.Sx;
   const int nv = C.G.nv, N = S.N, nl = S.nlines, m = S.rows, g = S.g;
-  const bool active = sys < (long)nl * nv * 4;
   const long sy = active ? sys : 0;
   const int r = (int)(sy & 3), v = (int)((sy >> 2) % nv), l = (int)((sy >> 2) / nv);
   const bool left = !(r & 1);
@@ -643,6 +669,20 @@ __device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, 
   if (!active) n = 0;
   const int lastlane = (N - 2) / m;
   const int hi_pos = N + g - 1;
+  // the end interfaces 0 and N (identity rows carrying WENO5 values)
+  double x0 = 0.0, xN = 0.0;
+  for (int e = 0; e < 2; e++) {
+    const bool mine = active && (e == 0 ? lane == 0 : lane == lastlane);
+    if (mine) {
+      const int k = e == 0 ? 0 : N;
+      double w[6];
+#pragma unroll
+      for (int j = 0; j < 6; j++) w[j] = src[k - 3 + j];
+      double a_, b_, c_, x_;
+      row_coeffs<CR, KIND>(k, N, left, C.R, w[0], w[1], w[2], w[3], w[4], w[5], a_, b_, c_, x_);
+      if (e == 0) x0 = x_; else xN = x_;
+    }
+  }
   // stencil window: positions lo-3 .. lo+n+1
   double win[kMaxRows + 5];
 #pragma unroll
@@ -660,21 +700,9 @@ __device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, 
       A[i] = 0.0; B[i] = 1.0; Cc[i] = 0.0; X[i] = 0.0;
     }
   }
-  // the end interfaces 0 and N (identity rows carrying WENO5 values)
-  double x0 = 0.0, xN = 0.0;
-  for (int e = 0; e < 2; e++) {
-    const bool mine = active && (e == 0 ? lane == 0 : lane == lastlane);
-    if (mine) {
-      const int k = e == 0 ? 0 : N;
-      double w[6];
-#pragma unroll
-      for (int j = 0; j < 6; j++) w[j] = src[k - 3 + j];
-      double a_, b_, c_, x_;
-      row_coeffs<CR, KIND>(k, N, left, C.R, w[0], w[1], w[2], w[3], w[4], w[5], a_, b_, c_, x_);
-      if (e == 0) x0 = x_; else xN = x_;
-    }
-  }
-  if (CR) team_solve<kMaxRows>(A, B, Cc, X, n, lane, S.team, lane == lastlane, x0, xN, red, C.status);
+  if (CR)
+    team_solve<kMaxRows>(A, B, Cc, X, n, lane, S.team, lane == lastlane, x0, xN, red, C.status,
+                              C.R.plain_start != 0);
 #pragma unroll
   for (int i = 0; i < kMaxRows; i++)
     if (i < n) out[lo + i] = X[i];
@@ -765,15 +793,17 @@ __global__ void __launch_bounds__(256) k_recon(CASE_ARGS) {
   const int nbx = recon_sweep_blocks(C.Sx, C.G.nv);
   const int dir = bid < nbx ? 0 : 1;
   const int lb = dir ? bid - nbx : bid;
-  const int T = dir ? C.Sy.team : C.Sx.team;
+  const SweepGeom &S = dir ? C.Sy : C.Sx;
+  const int T = S.team;
   const int lane = threadIdx.x & (T - 1);
   const long sys = (long)lb * (256 / T) + threadIdx.x / T;
+  const bool active = sys < (long)S.nlines * C.G.nv * 4;
   const int cr = C.R.crweno;
   switch (C.R.kind) {
-    case 1: cr ? recon_team<1, 1>(C, dir, sys, lane, red) : recon_team<0, 1>(C, dir, sys, lane, red); break;
-    case 2: cr ? recon_team<1, 2>(C, dir, sys, lane, red) : recon_team<0, 2>(C, dir, sys, lane, red); break;
-    case 3: cr ? recon_team<1, 3>(C, dir, sys, lane, red) : recon_team<0, 3>(C, dir, sys, lane, red); break;
-    default: cr ? recon_team<1, 0>(C, dir, sys, lane, red) : recon_team<0, 0>(C, dir, sys, lane, red); break;
+    case 1: cr ? recon_team<1, 1>(C, dir, sys, active, lane, red) : recon_team<0, 1>(C, dir, sys, active, lane, red); break;
+    case 2: cr ? recon_team<1, 2>(C, dir, sys, active, lane, red) : recon_team<0, 2>(C, dir, sys, active, lane, red); break;
+    case 3: cr ? recon_team<1, 3>(C, dir, sys, active, lane, red) : recon_team<0, 3>(C, dir, sys, active, lane, red); break;
+    default: cr ? recon_team<1, 0>(C, dir, sys, active, lane, red) : recon_team<0, 0>(C, dir, sys, active, lane, red); break;
   }
 }
 
@@ -1367,7 +1397,7 @@ __global__ void __launch_bounds__(TB) k_tridiag(const double *__restrict__ a, co
       A[i] = 0.0; B[i] = 1.0; Cc[i] = 0.0; X[i] = 0.0;
     }
   }
-  team_solve<kMaxRows>(A, B, Cc, X, nr, lane, T, lane == (n - 1) / m, 0.0, 0.0, red, flag);
+  team_solve<kMaxRows>(A, B, Cc, X, nr, lane, T, lane == (n - 1) / m, 0.0, 0.0, red, flag, false);
 #pragma unroll
   for (int i = 0; i < kMaxRows; i++)
     if (i < nr) x[(long)(lo + i) * ns + d] = X[i];

@@ -40,8 +40,12 @@ struct SweepGeom {
   int rows;          // interior rows (1 .. N-1) held by each lane, <= kMaxRows
 };
 
-// Longest segment of a line system one lane keeps in registers.
-constexpr int kMaxRows = 16;
+// Longest segment of a line system one lane keeps in registers (the
+// register arrays of the reconstruction kernel are sized by it).
+#ifndef PIAFS_MAX_ROWS
+#define PIAFS_MAX_ROWS 16
+#endif
+constexpr int kMaxRows = PIAFS_MAX_ROWS;
 // Team shape for lines of N points: the fewest lanes (a power of two) whose
 // segments of at most max_rows rows cover interior rows 1..N-1.  Returns
 // false when a line is too long for one block.
@@ -72,6 +76,7 @@ struct PhysParams {
 
 struct ReconParams {
   int crweno, kind, no_limiting, characteristic;
+  int plain_start;  // start the line-solve relay from trivial estimates (testing)
   double eps;
 };
 

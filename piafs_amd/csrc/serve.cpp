@@ -57,7 +57,8 @@ struct piafs_server {
   int device = 0;
   size_t budget = 0, in_use = 0;
   double round_ms = 4.0;  // target device time of one round of a lane
-  std::mutex mu;
+  std::mutex mu;       // request table, queue and budget
+  std::mutex dev_mu;   // device memory management, plan builds, graph capture
   std::condition_variable cv;
   std::deque<Request *> pending;
   std::map<int, std::unique_ptr<Request>> reqs;
@@ -74,12 +75,44 @@ struct piafs_server {
   void run_lane(int L);
 };
 
+// Lane loop.  Device memory management, plan building, graph capture and
+// result downloads of the two lanes never overlap (dev_mu): an allocation,
+// free or synchronous copy issued by one thread while the other is capturing
+// its step graph can invalidate that capture.  Graph replays and waits run
+// concurrently.
 void piafs_server::run_lane(int L) {
   Lane &ln = lanes[L];
   hipSetDevice(device);
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
+  auto finish = [&](Request *r, int state, const std::string &msg) {  // called with mu held
+    r->state = state;
+    if (state < 0) r->err = msg;
+    r->t_done = now_s();
+    in_use -= r->bytes;
+    cv.notify_all();
+  };
+  // rebuild the plan for ln.members (dev_mu held); on failure every member fails
+  auto rebuild = [&]() {
+    ln.eng.cases.clear();
+    for (Request *r : ln.members) ln.eng.cases.push_back(r->c.get());
+    int rc = ln.eng.build();
+    if (!rc) rc = ln.eng.prepare();
+    ln.ms_per_step = 0.0;
+    if (!rc) return true;
+    std::vector<Request *> failed = ln.members;
+    ln.members.clear();
+    ln.eng.cases.clear();
+    ln.eng.build();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      err = ln.eng.err;
+      for (Request *r : failed) finish(r, -1, "ensemble setup: " + ln.eng.err);
+    }
+    for (Request *r : failed) r->c.reset();
+    return false;
+  };
   for (;;) {
     // admission: queued requests of this lane, in arrival order, while they fit
     std::vector<Request *> admit;
@@ -104,32 +137,25 @@ void piafs_server::run_lane(int L) {
         }
       }
     }
-    bool changed = false;
-    for (Request *r : admit) {
-      int rc = r->c->setup(device, ln.eng.stream);
-      std::lock_guard<std::mutex> lk(mu);
-      if (rc) {
-        r->state = -1;
-        r->err = r->c->err;
-        r->t_done = now_s();
-        in_use -= r->bytes;
-        r->c.reset();
-        cv.notify_all();
-        continue;
+    if (!admit.empty()) {
+      std::lock_guard<std::mutex> dk(dev_mu);
+      for (Request *r : admit) {
+        const int rc = r->c->setup(device, ln.eng.stream);
+        if (rc) {
+          const std::string msg = r->c->err;
+          r->c.reset();
+          std::lock_guard<std::mutex> lk(mu);
+          finish(r, -1, msg);
+          continue;
+        }
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          r->state = 1;
+          r->t_start = now_s();
+        }
+        ln.members.push_back(r);
       }
-      r->state = 1;
-      r->t_start = now_s();
-      ln.members.push_back(r);
-      changed = true;
-    }
-    if (changed) {
-      ln.eng.cases.clear();
-      for (Request *r : ln.members) ln.eng.cases.push_back(r->c.get());
-      if (ln.eng.build()) {
-        std::lock_guard<std::mutex> lk(mu);
-        err = ln.eng.err;
-      }
-      ln.ms_per_step = 0.0;
+      rebuild();
     }
     if (ln.members.empty()) continue;
     // one round: the same number of steps for every member
@@ -138,60 +164,41 @@ void piafs_server::run_lane(int L) {
     if (ln.ms_per_step > 0.0) k = std::min(k, std::max(1, (int)(round_ms / ln.ms_per_step)));
     else k = std::min(k, 1);
     hipEventRecord(e0, ln.eng.stream);
-    int rc = ln.eng.step(k);
+    int rc = ln.eng.step(k);  // graph replays of the captured step
     hipEventRecord(e1, ln.eng.stream);
     if (!rc) rc = ln.eng.sync();
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    const double per = ms / k;
-    ln.ms_per_step = ln.ms_per_step > 0.0 ? 0.5 * (ln.ms_per_step + per) : per;
-    // completions
-    std::vector<Request *> keep;
-    bool removed = false;
+    if (!rc) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      const double per = ms / k;
+      ln.ms_per_step = ln.ms_per_step > 0.0 ? 0.5 * (ln.ms_per_step + per) : per;
+    }
+    // completions (and failures)
+    std::vector<Request *> keep, gone;
     for (Request *r : ln.members) {
-      if (rc) {
-        std::lock_guard<std::mutex> lk(mu);
-        r->state = -1;
-        r->err = ln.eng.err;
-      } else {
-        r->done += k;
-        if (r->done < r->target) {
-          keep.push_back(r);
-          continue;
-        }
+      if (!rc) r->done += k;
+      if (!rc && r->done < r->target) keep.push_back(r);
+      else gone.push_back(r);
+    }
+    if (gone.empty()) continue;
+    std::lock_guard<std::mutex> dk(dev_mu);
+    for (Request *r : gone) {
+      int drc = rc;
+      std::string msg = ln.eng.err;
+      if (!rc) {
         const Config &cf = r->c->cfg;
-        size_t n = (size_t)cf.npts_int * cf.nvars;
-        r->result.resize(n);
-        int drc = r->c->download(r->result.data(), ln.eng.stream);
+        r->result.resize((size_t)cf.npts_int * cf.nvars);
+        drc = r->c->download(r->result.data(), ln.eng.stream);
+        if (drc) msg = r->c->err;
         fill_info(r->c.get(), &r->info);
-        std::lock_guard<std::mutex> lk(mu);
-        if (drc) {
-          r->state = -1;
-          r->err = r->c->err;
-        } else {
-          r->state = 2;
-        }
       }
-      removed = true;
       std::lock_guard<std::mutex> lk(mu);
-      r->t_done = now_s();
-      in_use -= r->bytes;
-      cv.notify_all();
+      finish(r, drc ? -1 : 2, msg);
     }
-    if (removed) {
-      // release the finished members' device memory after the engine forgets them
-      std::vector<Request *> gone;
-      for (Request *r : ln.members)
-        if (std::find(keep.begin(), keep.end(), r) == keep.end()) gone.push_back(r);
-      ln.members = keep;
-      ln.eng.cases.clear();
-      for (Request *r : ln.members) ln.eng.cases.push_back(r->c.get());
-      ln.eng.build();
-      ln.ms_per_step = 0.0;
-      for (Request *r : gone) r->c.reset();
-      std::lock_guard<std::mutex> lk(mu);
-      cv.notify_all();
-    }
+    // the engine forgets the finished members before their memory is released
+    ln.members = keep;
+    rebuild();
+    for (Request *r : gone) r->c.reset();
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -270,6 +277,7 @@ int piafs_server_poll(piafs_server *s, int id, int *state, double *latency) {
   if (it == s->reqs.end()) return 1;
   Request *r = it->second.get();
   if (state) *state = r->state;
+  if (r->state < 0) s->err = r->err;
   if (latency) *latency = (r->state == 2 || r->state < 0) ? r->t_done - r->t_submit : -1.0;
   return 0;
 }

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -147,6 +148,10 @@ int Case::setup(int dev, hipStream_t st) {
   D.R.no_limiting = cfg.no_limiting;
   D.R.characteristic = cfg.characteristic ? 1 : 0;
   D.R.eps = cfg.eps;
+  {
+    const char *fr = getenv("PIAFS_AMD_LINE_SOLVE_PLAIN_START");  // run the relay from trivial estimates
+    D.R.plain_start = (fr && fr[0] == '1') ? 1 : 0;
+  }
   D.model = cfg.model_id;
   D.upwind = cfg.upwind;
   D.nstages = cfg.nstages;
@@ -202,6 +207,11 @@ int Case::setup(int dev, hipStream_t st) {
 
 int Case::set_team_rows(int max_rows) {
   CaseDev &D = h;
+  if (cfg.characteristic) {  // characteristic WENO5 runs one thread per interface: no line teams
+    D.Sx.team = D.Sy.team = 1;
+    D.Sx.rows = D.Sy.rows = 1;
+    return 0;
+  }
   if (!team_shape(D.Sx.N, max_rows, D.Sx.team, D.Sx.rows) ||
       (cfg.ndims == 2 && !team_shape(D.Sy.N, max_rows, D.Sy.team, D.Sy.rows)))
     return fail(1, "grid lines longer than " + std::to_string(256 * kMaxRows + 1) +
@@ -294,6 +304,7 @@ void Engine::release_plan() {
   d_cases = nullptr;
   d_status = nullptr;
   plan = Plan();
+  built = false;
 }
 
 int Engine::build() {
@@ -302,12 +313,15 @@ int Engine::build() {
   release_plan();
   const int nc = (int)cases.size();
   plan.ncases = nc;
-  if (nc == 0) return 0;
+  if (nc == 0) {
+    built = true;
+    return 0;
+  }
   void *p = nullptr;
   HIPCHK(hipMalloc(&p, nc * sizeof(int)));
   plan_allocs.push_back(p);
   d_status = (int *)p;
-  HIPCHK(hipMemset(d_status, 0, nc * sizeof(int)));
+  HIPCHK(hipMemsetAsync(d_status, 0, nc * sizeof(int), stream));
   std::vector<CaseDev> hd(nc);
   for (int k = 0; k < nc; k++) {
     hd[k] = cases[k]->h;
@@ -316,7 +330,7 @@ int Engine::build() {
   HIPCHK(hipMalloc(&p, nc * sizeof(CaseDev)));
   plan_allocs.push_back(p);
   d_cases = (CaseDev *)p;
-  HIPCHK(hipMemcpy(d_cases, hd.data(), nc * sizeof(CaseDev), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpyAsync(d_cases, hd.data(), nc * sizeof(CaseDev), hipMemcpyHostToDevice, stream));
   for (int cls = 0; cls < K_NCLASS; cls++) {
     std::vector<int> bc, first(nc);
     for (int k = 0; k < nc; k++) {
@@ -330,12 +344,16 @@ int Engine::build() {
     HIPCHK(hipMalloc(&p, bc.size() * sizeof(int)));
     plan_allocs.push_back(p);
     plan.blockcase[cls] = (int *)p;
-    HIPCHK(hipMemcpy(p, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(p, bc.data(), bc.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));  // bc and first are host temporaries
     HIPCHK(hipMalloc(&p, nc * sizeof(int)));
     plan_allocs.push_back(p);
     plan.firstblock[cls] = (int *)p;
-    HIPCHK(hipMemcpy(p, first.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(p, first.data(), nc * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
   }
+  HIPCHK(hipStreamSynchronize(stream));  // hd is a host temporary
+  built = true;
   return 0;
 }
 
@@ -368,15 +386,32 @@ void Engine::enqueue_step() {
   if (profiling) mark(K_UPDATE, false);
 }
 
+// Capture one step of the current member set into a graph (once per member set).
+int Engine::prepare() {
+  HIPCHK(hipSetDevice(device));
+  if (!built) return fail(-1, "the ensemble plan is not built");
+  if (!use_graph || profiling || gexec || cases.empty()) return 0;
+  HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step();
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(stream, &g);
+  if (e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return fail(-1, std::string("step graph capture: ") + hipGetErrorString(e));
+  }
+  graph = g;
+  HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  return 0;
+}
+
 int Engine::step(int n) {
   HIPCHK(hipSetDevice(device));
+  if (!built) return fail(-1, "the ensemble plan is not built");
   if (cases.empty()) return 0;
   if (use_graph && !profiling) {
     if (!gexec) {
-      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      enqueue_step();
-      HIPCHK(hipStreamEndCapture(stream, &graph));
-      HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+      int rc = prepare();
+      if (rc) return rc;
     }
     for (int i = 0; i < n; i++) HIPCHK(hipGraphLaunch(gexec, stream));
   } else {
@@ -388,8 +423,63 @@ int Engine::step(int n) {
   return 0;
 }
 
+// Average device time (ms) of kernel class `cls` over `reps` evaluations of
+// stage 0 of every member.  A stage leaves the solution and the time alone
+// (it writes the stage state, the interface arrays and Udot[0] only).
+double Engine::time_class(int cls, int reps) {
+  if (!built || cases.empty()) return 0.0;
+  const bool was = profiling;
+  profiling = true;
+  ev_used = 0;
+  for (int r = 0; r < reps; r++) launch_stage(d_cases, plan, 0, stream, mark_cb, this);
+  hipStreamSynchronize(stream);
+  double tot = 0.0;
+  int cnt = 0;
+  for (size_t k = 0; k < ev_used; k++) {
+    if (ev_cls[k] != cls) continue;
+    float m = 0.f;
+    hipEventElapsedTime(&m, ev[k].first, ev[k].second);
+    tot += m;
+    cnt++;
+  }
+  ev_used = 0;
+  profiling = was;
+  return cnt ? tot / cnt : 0.0;
+}
+
+// Segment length of the line solves (rows per lane, which sets each case's
+// team size): every candidate is timed on the members themselves and the
+// fastest kept.
+int Engine::autotune(const int *cands, int ncand, int reps, int *best_rows, double *best_ms) {
+  HIPCHK(hipSetDevice(device));
+  if (cases.empty()) return 0;
+  int best = -1;
+  double bms = 0.0;
+  for (int k = 0; k < ncand; k++) {
+    bool ok = true;
+    for (Case *c : cases) ok = ok && c->set_team_rows(cands[k]) == 0;
+    if (!ok) continue;
+    int rc = build();
+    if (rc) return rc;
+    time_class(K_RECON, 1);  // warm
+    const double ms = time_class(K_RECON, reps);
+    if (best < 0 || ms < bms) {
+      best = cands[k];
+      bms = ms;
+    }
+  }
+  if (best < 0) return fail(1, "no candidate segment length fits the members' lines");
+  for (Case *c : cases) c->set_team_rows(best);
+  int rc = build();
+  if (rc) return rc;
+  if (best_rows) *best_rows = best;
+  if (best_ms) *best_ms = bms;
+  return 0;
+}
+
 int Engine::stage_only(int s) {
   HIPCHK(hipSetDevice(device));
+  if (!built) return fail(-1, "the ensemble plan is not built");
   launch_stage(d_cases, plan, s, stream, nullptr, nullptr);
   HIPCHK(hipGetLastError());
   return 0;
@@ -400,7 +490,8 @@ int Engine::sync() {
   HIPCHK(hipStreamSynchronize(stream));
   if (!cases.empty() && d_status) {
     std::vector<int> st(cases.size());
-    HIPCHK(hipMemcpy(st.data(), d_status, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(st.data(), d_status, st.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     for (size_t k = 0; k < st.size(); k++)
       if (st[k]) {
         const std::string msg = "zero pivot in a CRWENO5 line solve (tridiagLU returns -1)";
@@ -449,6 +540,7 @@ extern "C" {
 int piafs_create(const char *dir, int device, piafs_case **out) {
   *out = nullptr;
   piafs_case *pc = new piafs_case();
+  pc->eng.use_graph = true;  // steps replay a captured graph (eager launches while profiling)
   int rc = pc->eng.init(device);
   if (rc) {
     g_create_error = pc->eng.err;
@@ -624,6 +716,13 @@ int piafs_kernel_times(piafs_case *pc, int enable, const char **names, double *m
   *n = w;
   e.ev_used = 0;
   return 0;
+}
+
+int piafs_autotune(piafs_case *pc, int *rows_per_lane, double *recon_ms) {
+  static const int cands[] = {4, 8, 12, 16};
+  int rc = pc->eng.autotune(cands, 4, 3, rows_per_lane, recon_ms);
+  if (rc) pc->err = pc->eng.err;
+  return rc;
 }
 
 int piafs_estimate_bytes(const char *dir, size_t *bytes) {

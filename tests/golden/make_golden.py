@@ -14,7 +14,7 @@ Two kinds of fixtures:
    LaxShockTube} and the error norms of
    Examples/2D_NavierStokes/InviscidVortexConvection/errors.dat.
 
-Usage: python tests/golden/make_golden.py [--config2-full RUN_DIR]
+Usage: python tests/golden/make_golden.py [--config2-full RUN_DIR | --config5-nochem RUN_DIR]
 """
 from __future__ import annotations
 
@@ -121,17 +121,44 @@ def config2_full(run_dir):
     _, nv, dims, x, u = decks.read_binary_op(os.path.join(run_dir, "op.bin"))
     nx, ny = dims
     U = u.reshape(ny, nx, nv)
+    # the last screen line of the run (TimePreStep CFL, TimePostStep norm)
+    cfl = norm = np.nan
+    log = os.path.join(run_dir, "ref.log")
+    if os.path.exists(log):
+        for line in open(log):
+            m = re.search(r"iter=\s*200\s+t=\S+\s+CFL=(\S+)\s+norm=(\S+)", line)
+            if m:
+                cfl, norm = float(m.group(1)), float(m.group(2))
     np.savez_compressed(
         os.path.join(HERE, "config2_1024_ref.npz"),
         l1=np.abs(U).sum(axis=(0, 1)), l2=np.sqrt((U * U).sum(axis=(0, 1))), linf=np.abs(U).max(axis=(0, 1)),
         colsum=U.sum(axis=0), rowsum=U.sum(axis=1), sub=U[::32, ::32, :].copy(),
+        colabs=np.abs(U).sum(axis=0), rowabs=np.abs(U).sum(axis=1),
+        cfl_200=np.float64(cfl), norm_200=np.float64(norm),
     )
     print("config2 fixture written", U.shape)
+
+
+def config5_nochem(run_dir):
+    """Reduce the 4096^2 chemistry-off 10-step reference run (lines of 4097
+    interfaces: the longest team solves) to norms, sums and a stride-128
+    subsample."""
+    _, nv, dims, x, u = decks.read_binary_op(os.path.join(run_dir, "op.bin"))
+    nx, ny = dims
+    U = u.reshape(ny, nx, nv)
+    np.savez_compressed(
+        os.path.join(HERE, "config5_4096_nochem_ref.npz"),
+        l2=np.sqrt((U * U).sum(axis=(0, 1))), colsum=U.sum(axis=0), rowsum=U.sum(axis=1),
+        colabs=np.abs(U).sum(axis=0), rowabs=np.abs(U).sum(axis=1), sub=U[::128, ::128, :].copy(),
+    )
+    print("config5 chemistry-off fixture written", U.shape)
 
 
 if __name__ == "__main__":
     if len(sys.argv) > 2 and sys.argv[1] == "--config2-full":
         config2_full(sys.argv[2])
+    elif len(sys.argv) > 2 and sys.argv[1] == "--config5-nochem":
+        config5_nochem(sys.argv[2])
     else:
         ref_runs()
         example_logs()

@@ -121,3 +121,27 @@ def test_batch_create_reports_bad_member():
         decks.sod_1d(N=32, n_iter=1).write(good)
         with pytest.raises(piafs_amd.PiafsError, match="case 1"):
             piafs_amd.Batch([good, os.path.join(tmp, "missing")])
+
+
+def test_autotune_keeps_results_bitwise():
+    """The segment length the autotuner picks changes only the speed: the line
+    solves are exact for every team shape."""
+    with tempfile.TemporaryDirectory() as tmp:
+        dirs = []
+        for k, dk in enumerate(_ns2d_members()):
+            dirs.append(os.path.join(tmp, f"m{k}"))
+            dk.write(dirs[-1])
+        with piafs_amd.Batch(dirs) as a, piafs_amd.Batch(dirs) as b:
+            rows, ms = b.autotune()
+            assert rows in (4, 8, 12, 16) and ms > 0.0
+            a.step(6)
+            b.step(6)
+            for k in range(len(dirs)):
+                assert np.array_equal(a.solution(k), b.solution(k)), k
+        with piafs_amd.Case(dirs[0]) as c:
+            rows, ms = c.autotune()
+            c.step(8)
+            tuned = c.solution()
+        with piafs_amd.Case(dirs[0]) as c:
+            c.step(8)
+            assert np.array_equal(tuned, c.solution())

@@ -3,6 +3,7 @@ others run are admitted into the resident ensembles of their class, and each
 result is exactly the result of the same deck run alone (bitwise; the same
 device code runs on the same data) and matches the oracle to FP64 relative
 L2 <= 1e-11."""
+import math
 import os
 import tempfile
 import time
@@ -88,3 +89,37 @@ def test_oversized_request_is_refused():
         with piafs_amd.Server(hbm_budget=1024) as srv:
             with pytest.raises(piafs_amd.PiafsError, match="budget"):
                 srv.submit(tmp)
+
+
+def test_both_lanes_busy_with_many_admissions():
+    """Both lanes (1D and 2D) admit and release many requests while the other
+    lane is stepping: plan rebuilds and graph captures of one lane interleave
+    with the other lane's rounds.  Every result equals its solo run."""
+    mix = []
+    for k in range(12):
+        if k % 2:
+            n = 48 + 16 * k
+            mix.append(decks.sod_1d(N=n, n_iter=10 + 3 * k, t_final=0.2 * (10 + 3 * k) / math.ceil(0.88 * (n - 1))))
+        else:
+            mix.append(decks.ns2d_photochem(nx=16 + 4 * k, ny=20 + 2 * k, n_iter=4 + k,
+                                            scheme="crweno5" if k % 4 == 0 else "weno5"))
+    with tempfile.TemporaryDirectory() as tmp:
+        dirs = []
+        for k, dk in enumerate(mix):
+            dirs.append(os.path.join(tmp, f"q{k}"))
+            dk.write(dirs[-1])
+        with piafs_amd.Server() as srv:
+            ids = []
+            for d in dirs:
+                ids.append(srv.submit(d))
+                time.sleep(0.001)
+            srv.wait_all()
+            got = []
+            for i in ids:
+                st, _ = srv.poll(i)
+                assert st == srv.DONE, srv._err()
+                got.append(srv.result(i))
+        for k, d in enumerate(dirs):
+            with piafs_amd.Case(d) as c:
+                c.step(c.info.n_iter)
+                assert np.array_equal(got[k], c.solution()), k
