@@ -187,8 +187,14 @@ def bench_serve(tmp, n_requests=24, rate=None, rate_frac=0.7, batched_rate=None)
     out = {"requests": n_requests, "rate_per_s": rate, "cell_updates": work,
            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
            "throughput_cell_updates_per_s": work / (t_end - t_start)}
+    is1d = np.array([r.member.deck.solver["ndims"] == 1 for r in reqs])
+    for tag, sel in (("1d", is1d), ("2d", ~is1d)):
+        if sel.any():
+            out[f"p50_latency_{tag}_s"] = float(np.percentile(lat[sel], 50))
+            out[f"max_latency_{tag}_s"] = float(lat[sel].max())
+
     # the same stream, one request at a time in arrival order
-    t_free, lat_s, busy = 0.0, [], 0.0
+    t_free, lat_s, busy, solo = 0.0, [], 0.0, []
     for k, r in enumerate(reqs):
         t0 = time.perf_counter()
         c = piafs_amd.Case(dirs[k])
@@ -196,11 +202,15 @@ def bench_serve(tmp, n_requests=24, rate=None, rate_frac=0.7, batched_rate=None)
         c.sync()
         c.close()
         dur = time.perf_counter() - t0
+        solo.append(dur)
         busy += dur
         start = max(r.arrival, t_free)
         t_free = start + dur
         lat_s.append(t_free - r.arrival)
     lat_s = np.array(lat_s)
+    order = np.argsort(-lat)[:5]
+    out["slowest_served"] = [{"request": reqs[k].member.name, "served_latency_s": round(float(lat[k]), 4),
+                              "alone_s": round(solo[k], 4)} for k in order]
     out["serial_p50_latency_s"] = float(np.percentile(lat_s, 50))
     out["serial_p99_latency_s"] = float(np.percentile(lat_s, 99))
     out["serial_throughput_cell_updates_per_s"] = work / max(t_free, busy)
@@ -282,8 +292,8 @@ def main():
         if ktimes:
             # per-class average launch time (ms) x launches per step -> share of the step
             ns = inf.nstages
-            per_step = {"stage_bc": ns, "flux": ns, "recon": ns, "upwind": ns, "qderiv": ns, "viscflux": ns,
-                        "assemble": ns, "update": 1}
+            per_step = {"stage_bc": ns, "flux": ns, "recon_ends": ns, "recon": ns, "upwind": ns, "qderiv": ns,
+                        "viscflux": ns, "assemble": ns, "update": 1}
             share = {k: v * per_step.get(k, 1) for k, v in ktimes.items()}
             dom = max(share.items(), key=lambda kv: kv[1])[0]
             byts = kernel_algorithmic_bytes(dom, nx, nx, nv, g, inf.nstages, True)

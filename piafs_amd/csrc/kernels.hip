@@ -669,20 +669,11 @@ __device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, 
   if (!active) n = 0;
   const int lastlane = (N - 2) / m;
   const int hi_pos = N + g - 1;
-  // the end interfaces 0 and N (identity rows carrying WENO5 values)
+  // the end interfaces 0 and N (identity rows carrying WENO5 values, written
+  // by k_recon_ends)
   double x0 = 0.0, xN = 0.0;
-  for (int e = 0; e < 2; e++) {
-    const bool mine = active && (e == 0 ? lane == 0 : lane == lastlane);
-    if (mine) {
-      const int k = e == 0 ? 0 : N;
-      double w[6];
-#pragma unroll
-      for (int j = 0; j < 6; j++) w[j] = src[k - 3 + j];
-      double a_, b_, c_, x_;
-      row_coeffs<CR, KIND>(k, N, left, C.R, w[0], w[1], w[2], w[3], w[4], w[5], a_, b_, c_, x_);
-      if (e == 0) x0 = x_; else xN = x_;
-    }
-  }
+  if (CR && active && lane == 0) x0 = out[0];
+  if (CR && active && lane == lastlane) xN = out[N];
   // stencil window: positions lo-3 .. lo+n+1
   double win[kMaxRows + 5];
 #pragma unroll
@@ -706,8 +697,23 @@ __device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, 
 #pragma unroll
   for (int i = 0; i < kMaxRows; i++)
     if (i < n) out[lo + i] = X[i];
-  if (active && lane == 0) out[0] = x0;
-  if (active && lane == lastlane) out[N] = xN;
+}
+
+// The two end interfaces (0 and N) of every line system, one thread each:
+// WENO5 values on identity rows (Interp1PrimFifthOrderCRWENO.c:132-201 with
+// the WENO5 weights at the ends; plain WENO5 for the non-compact scheme).
+template <int CR, int KIND>
+__device__ __forceinline__ void recon_end(const CaseDev &C, int dir, long sys, int e) {
+  const SweepGeom &S = dir ? C.Sy : C.Sx;
+  const int nv = C.G.nv, N = S.N, nl = S.nlines;
+  const int r = (int)(sys & 3), v = (int)((sys >> 2) % nv), l = (int)((sys >> 2) / nv);
+  const double *src = ((r < 2) ? (dir ? C.UT : C.U) : (dir ? C.flux_y : C.flux_x)) + (long)v * S.plane + S.base +
+                      (long)l * S.line_stride;
+  const int k = e ? N : 0;
+  double a_, b_, c_, x_;
+  row_coeffs<CR, KIND>(k, N, !(r & 1), C.R, src[k - 3], src[k - 2], src[k - 1], src[k], src[k + 1], src[k + 2], a_,
+                       b_, c_, x_);
+  (dir ? C.iy[r] : C.ix[r])[((long)v * nl + l) * (N + 1) + k] = x_;
 }
 
 // Characteristic WENO5 for euler1d, one thread per (interface, line).
@@ -771,6 +777,25 @@ __host__ __device__ inline int recon_sweep_blocks(const SweepGeom &S, int nv) {
   const long teams_per_block = 256 / S.team;
   const long nsys = (long)S.nlines * nv * 4;
   return (int)((nsys + teams_per_block - 1) / teams_per_block);
+}
+
+__global__ void __launch_bounds__(256) k_recon_ends(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const long nsx = (long)C.Sx.nlines * C.G.nv * 4;
+  const long nsy = (C.G.ndims == 2) ? (long)C.Sy.nlines * C.G.nv * 4 : 0;
+  if (t >= 2 * (nsx + nsy)) return;
+  const int dir = t < 2 * nsx ? 0 : 1;
+  const long q = dir ? t - 2 * nsx : t;
+  const long sys = q >> 1;
+  const int e = (int)(q & 1);
+  const int cr = C.R.crweno;
+  switch (C.R.kind) {
+    case 1: cr ? recon_end<1, 1>(C, dir, sys, e) : recon_end<0, 1>(C, dir, sys, e); break;
+    case 2: cr ? recon_end<1, 2>(C, dir, sys, e) : recon_end<0, 2>(C, dir, sys, e); break;
+    case 3: cr ? recon_end<1, 3>(C, dir, sys, e) : recon_end<0, 3>(C, dir, sys, e); break;
+    default: cr ? recon_end<1, 0>(C, dir, sys, e) : recon_end<0, 0>(C, dir, sys, e); break;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_recon(CASE_ARGS) {
@@ -1423,8 +1448,8 @@ __global__ void __launch_bounds__(TB) k_weno_weights(int kind, double c1, double
 // host side: work decomposition and launchers
 // ===========================================================================
 
-static const char *kKNames[K_NCLASS] = {"stage_bc", "flux", "recon", "upwind", "qderiv", "viscflux",
-                                        "assemble", "update"};
+static const char *kKNames[K_NCLASS] = {"stage_bc", "flux", "recon_ends", "recon", "upwind", "qderiv",
+                                        "viscflux", "assemble", "update"};
 const char *kclass_name(int k) { return kKNames[k]; }
 
 bool team_shape(int N, int max_rows, int &team, int &rows) {
@@ -1442,8 +1467,13 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
   auto nb = [](long n) { return (int)((n + TB - 1) / TB); };
   blocks[K_STAGE] = two ? ((G.nxg + 31) / 32) * ((G.nyg + 31) / 32) * G.nv : nb((long)G.nxg * G.nv);
   blocks[K_FLUX] = nb((long)(c.Sx.N + 2 * c.Sx.g) * c.Sx.nlines + (two ? (long)(c.Sy.N + 2 * c.Sy.g) * c.Sy.nlines : 0));
-  if (c.R.characteristic) blocks[K_RECON] = nb((long)(c.Sx.N + 1) * c.Sx.nlines);
-  else blocks[K_RECON] = recon_sweep_blocks(c.Sx, G.nv) + (two ? recon_sweep_blocks(c.Sy, G.nv) : 0);
+  if (c.R.characteristic) {
+    blocks[K_ENDS] = 0;
+    blocks[K_RECON] = nb((long)(c.Sx.N + 1) * c.Sx.nlines);
+  } else {
+    blocks[K_ENDS] = nb(2 * 4 * (long)G.nv * (c.Sx.nlines + (two ? c.Sy.nlines : 0)));
+    blocks[K_RECON] = recon_sweep_blocks(c.Sx, G.nv) + (two ? recon_sweep_blocks(c.Sy, G.nv) : 0);
+  }
   blocks[K_UPWIND] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
   const bool visc = two && c.P.viscous;
   blocks[K_QDERIV] = visc ? nb((long)G.nxg * G.nyg) : 0;
@@ -1466,6 +1496,7 @@ void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, void 
                   void *ctx) {
   run(k_stage, K_STAGE, d, pl, s, st, mark, ctx);
   run(k_flux, K_FLUX, d, pl, s, st, mark, ctx);
+  run(k_recon_ends, K_ENDS, d, pl, s, st, mark, ctx);
   run(k_recon, K_RECON, d, pl, s, st, mark, ctx);
   run(k_upwind, K_UPWIND, d, pl, s, st, mark, ctx);
   run(k_qderiv, K_QDERIV, d, pl, s, st, mark, ctx);

@@ -98,7 +98,7 @@ struct CaseDev {
 
 // Kernel classes launched per RK stage (plus the final update).
 enum KClass {
-  K_STAGE = 0, K_FLUX, K_RECON, K_UPWIND, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE, K_NCLASS
+  K_STAGE = 0, K_FLUX, K_ENDS, K_RECON, K_UPWIND, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE, K_NCLASS
 };
 const char *kclass_name(int k);
 

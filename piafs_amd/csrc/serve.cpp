@@ -41,6 +41,7 @@ struct Request {
   int id = 0;
   std::unique_ptr<Case> c;
   size_t bytes = 0;
+  long cells = 0;       // interior points
   int lane = 0;
   int target = 0;       // steps to take (n_iter - restart_iter)
   int done = 0;         // steps taken
@@ -57,6 +58,10 @@ struct piafs_server {
   int device = 0;
   size_t budget = 0, in_use = 0;
   double round_ms = 4.0;  // target device time of one round of a lane
+  // interior points advanced together in one lane beyond which a lane admits
+  // no further request (a single 2D case of about 700^2 already fills the
+  // device; more members would only share it)
+  long inflight_cells[2] = {1L << 22, 1L << 19};
   std::mutex mu;       // request table, queue and budget
   std::mutex dev_mu;   // device memory management, plan builds, graph capture
   std::condition_variable cv;
@@ -126,15 +131,22 @@ void piafs_server::run_lane(int L) {
           return false;
         });
       if (stop) break;
+      // arrival order within the lane; a request that does not fit stops the
+      // admission (no overtaking), except into an empty lane
+      long cells = 0;
+      for (Request *r : ln.members) cells += r->cells;
       for (auto it = pending.begin(); it != pending.end();) {
         Request *r = *it;
-        if (r->lane == L && in_use + r->bytes <= budget) {
-          in_use += r->bytes;
-          admit.push_back(r);
-          it = pending.erase(it);
-        } else {
+        if (r->lane != L) {
           ++it;
+          continue;
         }
+        const bool empty = ln.members.empty() && admit.empty();
+        if (in_use + r->bytes > budget || (!empty && cells + r->cells > inflight_cells[L])) break;
+        in_use += r->bytes;
+        cells += r->cells;
+        admit.push_back(r);
+        it = pending.erase(it);
       }
     }
     if (!admit.empty()) {
@@ -248,6 +260,7 @@ int piafs_server_submit(piafs_server *s, const char *dir, int *id) {
   const Config &cf = r->c->cfg;
   r->bytes = Case::plan_bytes(cf);
   r->lane = cf.ndims == 1 ? 0 : 1;
+  r->cells = (long)cf.npts_int;
   r->target = cf.n_iter - cf.restart_iter;
   if (r->target < 0) r->target = 0;
   std::lock_guard<std::mutex> lk(s->mu);
