@@ -54,6 +54,17 @@ def kernel_algorithmic_bytes(name, nx, ny, nv, g, nstages, crweno):
     return None
 
 
+def measured_traffic(nx):
+    """HBM bytes per launch of the reconstruction kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by a copy
+    kernel of known size), or None when no profile of this grid is committed."""
+    path = os.path.join(ROOT, "profiles", f"ns2d_crweno5_{nx}_step_kernels.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"`k_recon`: HBM traffic ([0-9.]+) MB read \+ ([0-9.]+) MB written", open(path).read())
+    return (float(m.group(1)) + float(m.group(2))) * 1e6 if m else None
+
+
 def cpu_baseline(tmp, nx_sample=256, steps=10):
     """Reference PIAFS (built from source, serial, 1 core) on a bounded sample of
     the same workload: the config deck at nx_sample^2 with the same dx, dt and
@@ -229,6 +240,8 @@ def main():
                     help="2: 1024^2, dt 0.01 (default); 5: 4096^2, dt 0.0025, 50 steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default line-solve segment length (for profiling runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--batch-cases", type=int, default=256)
@@ -268,7 +281,7 @@ def main():
         case = piafs_amd.Case(d, device=0)
         inf = case.info
         nv, g = inf.nvars, inf.ghosts
-        tuned_rows, _ = case.autotune()
+        tuned_rows = None if args.no_autotune else case.autotune()[0]
         case.step(args.warmup)
         case.sync()
         case.profile(not args.no_profile)
@@ -300,7 +313,8 @@ def main():
             ms = ktimes[dom]
             ach = byts / (ms * 1e-3) / 1e9 if byts else None
             line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                "frac": (ach / HBM_PEAK_GBS) if ach else None, "traffic": None,
+                                "frac": (ach / HBM_PEAK_GBS) if ach else None,
+                                "traffic": measured_traffic(nx) if dom == "recon" else None,
                                 "kernel": dom, "kernel_ms": ms, "algorithmic_bytes_per_launch": byts}
             line["kernel_ms"] = ktimes
             line["kernel_ms_per_step"] = share
