@@ -653,67 +653,122 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
 //    rows with WENO5 right-hand sides at interfaces 0 and N), then the line
 //    solve (team_solve above).
 // ---------------------------------------------------------------------------
+// Systems of a sweep: split mode, one per (line, var, r); paired mode, one
+// team per (line, var, pair) solving the left- and right-biased systems of
+// the pair in turn (pair 0: uL, uR; pair 1: fL, fR) and writing the two
+// combinations the Roe upwinding uses, 0.5*(uR - uL) and 0.5*(fL + fR)
+// (NavierStokes2DUpwind.c:58-60, Euler1DUpwind.c:45-52).
+__host__ __device__ inline long recon_systems(const SweepGeom &S, int nv, int paired) {
+  return (long)S.nlines * nv * (paired ? 2 : 4);
+}
+
 template <int CR, int KIND>
-__device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, bool active, int lane, double *red) {
+__device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, bool active, int lane, double *red,
+                                           double *stash) {
   const SweepGeom &S = dir ? C.Sy : C.Sx;
   const int nv = C.G.nv, N = S.N, nl = S.nlines, m = S.rows, g = S.g;
+  const bool paired = C.paired != 0;
   const long sy = active ? sys : 0;
-  const int r = (int)(sy & 3), v = (int)((sy >> 2) % nv), l = (int)((sy >> 2) / nv);
-  const bool left = !(r & 1);
-  const double *src = ((r < 2) ? (dir ? C.UT : C.U) : (dir ? C.flux_y : C.flux_x)) + (long)v * S.plane + S.base +
+  const int per = paired ? 2 : 4;
+  const int q = (int)(sy % per), v = (int)((sy / per) % nv), l = (int)((sy / per) / nv);
+  const int r0 = paired ? 2 * q : q;
+  const double *src = ((r0 < 2) ? (dir ? C.UT : C.U) : (dir ? C.flux_y : C.flux_x)) + (long)v * S.plane + S.base +
                       (long)l * S.line_stride;
-  double *out = (dir ? C.iy[r] : C.ix[r]) + ((long)v * nl + l) * (N + 1);
+  double *const *io = dir ? C.iy : C.ix;
+  const long line = ((long)v * nl + l) * (N + 1);
   const int lo = 1 + lane * m;
   int n = N - lo;
   n = n < 0 ? 0 : (n > m ? m : n);
   if (!active) n = 0;
   const int lastlane = (N - 2) / m;
   const int hi_pos = N + g - 1;
-  // the end interfaces 0 and N (identity rows carrying WENO5 values, written
-  // by k_recon_ends)
-  double x0 = 0.0, xN = 0.0;
-  if (CR && active && lane == 0) x0 = out[0];
-  if (CR && active && lane == lastlane) xN = out[N];
-  // stencil window: positions lo-3 .. lo+n+1
-  double win[kMaxRows + 5];
+  const double *ends = C.ends + 4 * ((dir ? recon_systems(C.Sx, nv, 1) : 0) + sy);
+  double X[kMaxRows];
+  // solve the system of reconstruction r into X
+  auto solve = [&](int r, int side) {
+    const bool left = !(r & 1);
+    // the end interfaces 0 and N (identity rows carrying WENO5 values, from k_recon_ends)
+    double x0 = 0.0, xN = 0.0;
+    if (CR && active && lane == 0) x0 = paired ? ends[2 * side] : io[r][line];
+    if (CR && active && lane == lastlane) xN = paired ? ends[2 * side + 1] : io[r][line + N];
+    // stencil window: positions lo-3 .. lo+n+1
+    double win[kMaxRows + 5];
 #pragma unroll
-  for (int j = 0; j < kMaxRows + 5; j++) {
-    const int pos = lo - 3 + j;
-    win[j] = (n > 0 && j < n + 5 && pos <= hi_pos) ? src[pos] : 0.0;
+    for (int j = 0; j < kMaxRows + 5; j++) {
+      const int pos = lo - 3 + j;
+      win[j] = (n > 0 && j < n + 5 && pos <= hi_pos) ? src[pos] : 0.0;
+    }
+    double A[kMaxRows], B[kMaxRows], Cc[kMaxRows];
+#pragma unroll
+    for (int i = 0; i < kMaxRows; i++) {
+      if (i < n) {
+        row_coeffs<CR, KIND>(lo + i, N, left, C.R, win[i], win[i + 1], win[i + 2], win[i + 3], win[i + 4],
+                             win[i + 5], A[i], B[i], Cc[i], X[i]);
+      } else {
+        A[i] = 0.0; B[i] = 1.0; Cc[i] = 0.0; X[i] = 0.0;
+      }
+    }
+    if (CR)
+      team_solve<kMaxRows>(A, B, Cc, X, n, lane, S.team, lane == lastlane, x0, xN, red, C.status,
+                           C.R.plain_start != 0);
+  };
+  solve(r0, 0);
+  if (paired) {
+#pragma unroll
+    for (int i = 0; i < kMaxRows; i++) stash[i * 256 + threadIdx.x] = X[i];  // this lane's own rows
+    solve(r0 + 1, 1);
   }
-  double A[kMaxRows], B[kMaxRows], Cc[kMaxRows], X[kMaxRows];
+  if (!paired) {
 #pragma unroll
-  for (int i = 0; i < kMaxRows; i++) {
-    if (i < n) {
-      row_coeffs<CR, KIND>(lo + i, N, left, C.R, win[i], win[i + 1], win[i + 2], win[i + 3], win[i + 4], win[i + 5],
-                           A[i], B[i], Cc[i], X[i]);
-    } else {
-      A[i] = 0.0; B[i] = 1.0; Cc[i] = 0.0; X[i] = 0.0;
+    for (int i = 0; i < kMaxRows; i++)
+      if (i < n) io[r0][line + lo + i] = X[i];
+  } else {
+    double *out = io[q] + line;  // io[0]: 0.5*(uR - uL), io[1]: 0.5*(fL + fR)
+#pragma unroll
+    for (int i = 0; i < kMaxRows; i++) {
+      if (i < n) {
+        const double xl = stash[i * 256 + threadIdx.x];
+        out[lo + i] = q ? 0.5 * (xl + X[i]) : 0.5 * (X[i] - xl);
+      }
     }
   }
-  if (CR)
-    team_solve<kMaxRows>(A, B, Cc, X, n, lane, S.team, lane == lastlane, x0, xN, red, C.status,
-                              C.R.plain_start != 0);
-#pragma unroll
-  for (int i = 0; i < kMaxRows; i++)
-    if (i < n) out[lo + i] = X[i];
 }
 
 // The two end interfaces (0 and N) of every line system, one thread each:
 // WENO5 values on identity rows (Interp1PrimFifthOrderCRWENO.c:132-201 with
-// the WENO5 weights at the ends; plain WENO5 for the non-compact scheme).
+// the WENO5 weights at the ends; plain WENO5 for the non-compact scheme).  In
+// paired mode a thread does both systems of its pair: the raw values go to
+// the case's `ends` table for the line solves, the combination to the output.
+template <int CR, int KIND>
+__device__ __forceinline__ double end_value(const CaseDev &C, const SweepGeom &S, const double *src, int r, int k) {
+  double a_, b_, c_, x_;
+  row_coeffs<CR, KIND>(k, S.N, !(r & 1), C.R, src[k - 3], src[k - 2], src[k - 1], src[k], src[k + 1], src[k + 2],
+                       a_, b_, c_, x_);
+  return x_;
+}
+
 template <int CR, int KIND>
 __device__ __forceinline__ void recon_end(const CaseDev &C, int dir, long sys, int e) {
   const SweepGeom &S = dir ? C.Sy : C.Sx;
   const int nv = C.G.nv, N = S.N, nl = S.nlines;
-  const int r = (int)(sys & 3), v = (int)((sys >> 2) % nv), l = (int)((sys >> 2) / nv);
-  const double *src = ((r < 2) ? (dir ? C.UT : C.U) : (dir ? C.flux_y : C.flux_x)) + (long)v * S.plane + S.base +
+  const bool paired = C.paired != 0;
+  const int per = paired ? 2 : 4;
+  const int q = (int)(sys % per), v = (int)((sys / per) % nv), l = (int)((sys / per) / nv);
+  const int r0 = paired ? 2 * q : q;
+  const double *src = ((r0 < 2) ? (dir ? C.UT : C.U) : (dir ? C.flux_y : C.flux_x)) + (long)v * S.plane + S.base +
                       (long)l * S.line_stride;
   const int k = e ? N : 0;
-  double a_, b_, c_, x_;
-  row_coeffs<CR, KIND>(k, N, !(r & 1), C.R, src[k - 3], src[k - 2], src[k - 1], src[k], src[k + 1], src[k + 2], a_,
-                       b_, c_, x_);
-  (dir ? C.iy[r] : C.ix[r])[((long)v * nl + l) * (N + 1) + k] = x_;
+  double *const *io = dir ? C.iy : C.ix;
+  const long at = ((long)v * nl + l) * (N + 1) + k;
+  if (!paired) {
+    io[r0][at] = end_value<CR, KIND>(C, S, src, r0, k);
+    return;
+  }
+  const double xl = end_value<CR, KIND>(C, S, src, r0, k), xr = end_value<CR, KIND>(C, S, src, r0 + 1, k);
+  double *ends = C.ends + 4 * ((dir ? recon_systems(C.Sx, nv, 1) : 0) + sys);
+  ends[e] = xl;
+  ends[2 + e] = xr;
+  io[q][at] = q ? 0.5 * (xl + xr) : 0.5 * (xr - xl);
 }
 
 // Characteristic WENO5 for euler1d, one thread per (interface, line).
@@ -773,17 +828,17 @@ __device__ __forceinline__ void recon_char_e1d(const SweepGeom &S, int nv, const
 
 // Blocks of a sweep for the team-based reconstruction (each block holds
 // 256/team whole teams; a block never mixes sweeps).
-__host__ __device__ inline int recon_sweep_blocks(const SweepGeom &S, int nv) {
+__host__ __device__ inline int recon_sweep_blocks(const SweepGeom &S, int nv, int paired) {
   const long teams_per_block = 256 / S.team;
-  const long nsys = (long)S.nlines * nv * 4;
+  const long nsys = recon_systems(S, nv, paired);
   return (int)((nsys + teams_per_block - 1) / teams_per_block);
 }
 
 __global__ void __launch_bounds__(256) k_recon_ends(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
-  const long nsx = (long)C.Sx.nlines * C.G.nv * 4;
-  const long nsy = (C.G.ndims == 2) ? (long)C.Sy.nlines * C.G.nv * 4 : 0;
+  const long nsx = recon_systems(C.Sx, C.G.nv, C.paired);
+  const long nsy = (C.G.ndims == 2) ? recon_systems(C.Sy, C.G.nv, C.paired) : 0;
   if (t >= 2 * (nsx + nsy)) return;
   const int dir = t < 2 * nsx ? 0 : 1;
   const long q = dir ? t - 2 * nsx : t;
@@ -814,21 +869,22 @@ __global__ void __launch_bounds__(256) k_recon(CASE_ARGS) {
     return;
   }
   __shared__ double red[16];
+  __shared__ double stash[kMaxRows * 256];
   const int bid = (int)(blockIdx.x - firstblock[cidx]);
-  const int nbx = recon_sweep_blocks(C.Sx, C.G.nv);
+  const int nbx = recon_sweep_blocks(C.Sx, C.G.nv, C.paired);
   const int dir = bid < nbx ? 0 : 1;
   const int lb = dir ? bid - nbx : bid;
   const SweepGeom &S = dir ? C.Sy : C.Sx;
   const int T = S.team;
   const int lane = threadIdx.x & (T - 1);
   const long sys = (long)lb * (256 / T) + threadIdx.x / T;
-  const bool active = sys < (long)S.nlines * C.G.nv * 4;
+  const bool active = sys < recon_systems(S, C.G.nv, C.paired);
   const int cr = C.R.crweno;
   switch (C.R.kind) {
-    case 1: cr ? recon_team<1, 1>(C, dir, sys, active, lane, red) : recon_team<0, 1>(C, dir, sys, active, lane, red); break;
-    case 2: cr ? recon_team<1, 2>(C, dir, sys, active, lane, red) : recon_team<0, 2>(C, dir, sys, active, lane, red); break;
-    case 3: cr ? recon_team<1, 3>(C, dir, sys, active, lane, red) : recon_team<0, 3>(C, dir, sys, active, lane, red); break;
-    default: cr ? recon_team<1, 0>(C, dir, sys, active, lane, red) : recon_team<0, 0>(C, dir, sys, active, lane, red); break;
+    case 1: cr ? recon_team<1, 1>(C, dir, sys, active, lane, red, stash) : recon_team<0, 1>(C, dir, sys, active, lane, red, stash); break;
+    case 2: cr ? recon_team<1, 2>(C, dir, sys, active, lane, red, stash) : recon_team<0, 2>(C, dir, sys, active, lane, red, stash); break;
+    case 3: cr ? recon_team<1, 3>(C, dir, sys, active, lane, red, stash) : recon_team<0, 3>(C, dir, sys, active, lane, red, stash); break;
+    default: cr ? recon_team<1, 0>(C, dir, sys, active, lane, red, stash) : recon_team<0, 0>(C, dir, sys, active, lane, red, stash); break;
   }
 }
 
@@ -840,11 +896,15 @@ __global__ void __launch_bounds__(256) k_recon(CASE_ARGS) {
 //   entropy fix delta = 1e-6.
 //  Euler1D Roe: Euler1DUpwind.c:27-93 (no entropy fix); rf-char :107-190.
 // ---------------------------------------------------------------------------
-template <int DIR>
+// In paired mode io[0] holds 0.5*(uR - uL) and io[1] 0.5*(fL + fR) (the only
+// combinations the Roe flux uses); in split mode io[0..3] hold uL uR fL fR.
+#define UD(m) (PAIR ? io[0][(m) * vstride + ii] : 0.5 * (io[1][(m) * vstride + ii] - io[0][(m) * vstride + ii]))
+#define FS(m) (PAIR ? io[1][(m) * vstride + ii] : 0.5 * (io[2][(m) * vstride + ii] + io[3][(m) * vstride + ii]))
+
+template <int DIR, int PAIR>
 __device__ __forceinline__ void upwind_ns2d_roe(const SweepGeom &S, int nv, double g, long t,
                                                 const double *__restrict__ q, double *const *io,
                                                 double *__restrict__ fI) {
-  const double *uL = io[0], *uR = io[1], *fL = io[2], *fR = io[3];
   const int nl = S.nlines, N = S.N;
   const int k = (int)(t % (N + 1)), l = (int)(t / (N + 1));
   const long pl = S.plane;
@@ -929,7 +989,7 @@ __device__ __forceinline__ void upwind_ns2d_roe(const SweepGeom &S, int nv, doub
   const long ii = (long)l * (N + 1) + k, vstride = (long)nl * (N + 1);
   double ud[4], fo[4];
 #pragma unroll
-  for (int m = 0; m < 4; m++) ud[m] = 0.5 * (uR[m * vstride + ii] - uL[m * vstride + ii]);
+  for (int m = 0; m < 4; m++) ud[m] = UD(m);
   // modA = R (D L), row-major sums in index order (matmult_native.h:21-34)
   double DL[4][4];
 #pragma unroll
@@ -944,16 +1004,14 @@ __device__ __forceinline__ void upwind_ns2d_roe(const SweepGeom &S, int nv, doub
       const double mA = R[i][0] * DL[0][j] + R[i][1] * DL[1][j] + R[i][2] * DL[2][j] + R[i][3] * DL[3][j];
       us += mA * ud[j];
     }
-    fo[i] = 0.5 * (fL[i * vstride + ii] + fR[i * vstride + ii]) - us;
+    fo[i] = FS(i) - us;
   }
 #pragma unroll
   for (int m = 0; m < 4; m++) fI[m * vstride + ii] = fo[m];
-  for (int m = 4; m < nv; m++) {
-    const double udm = 0.5 * (uR[m * vstride + ii] - uL[m * vstride + ii]);
-    fI[m * vstride + ii] = 0.5 * (fL[m * vstride + ii] + fR[m * vstride + ii]) - Dsp * udm;
-  }
+  for (int m = 4; m < nv; m++) fI[m * vstride + ii] = FS(m) - Dsp * UD(m);
 }
 
+template <int PAIR>
 __device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g, int rf, long t,
                                            const double *__restrict__ q, double *const *io, double *__restrict__ fI) {
   const double *uL = io[0], *uR = io[1], *fL = io[2], *fR = io[3];
@@ -965,10 +1023,10 @@ __device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g,
   euler1d_roeavg(g, q[cL], q[pl + cL], q[2 * pl + cL], q[cR], q[pl + cR], q[2 * pl + cR], a0, a1, a2);
   Eig3 E;
   euler1d_eig(g, a0, a1, a2, E);
-  const long ii = (long)l * (N + 1) + k, vs = (long)nl * (N + 1);
+  const long ii = (long)l * (N + 1) + k, vs = (long)nl * (N + 1), vstride = vs;
   if (!rf) {
     double ud[3], DL[3][3];
-    for (int m = 0; m < 3; m++) ud[m] = 0.5 * (uR[m * vs + ii] - uL[m * vs + ii]);
+    for (int m = 0; m < 3; m++) ud[m] = UD(m);
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) DL[i][j] = fabs(E.D[i]) * E.L[i][j];
     for (int i = 0; i < 3; i++) {
@@ -977,13 +1035,10 @@ __device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g,
         const double mA = E.R[i][0] * DL[0][j] + E.R[i][1] * DL[1][j] + E.R[i][2] * DL[2][j];
         us += mA * ud[j];
       }
-      fI[i * vs + ii] = 0.5 * (fL[i * vs + ii] + fR[i * vs + ii]) - us;
+      fI[i * vs + ii] = FS(i) - us;
     }
     const double Dsp = fabs(E.D[0]);
-    for (int m = 3; m < nv; m++) {
-      const double udm = 0.5 * (uR[m * vs + ii] - uL[m * vs + ii]);
-      fI[m * vs + ii] = 0.5 * (fL[m * vs + ii] + fR[m * vs + ii]) - Dsp * udm;
-    }
+    for (int m = 3; m < nv; m++) fI[m * vs + ii] = FS(m) - Dsp * UD(m);
     return;
   }
   // cell eigenvalues (euler1d.h:131-147) of the two neighbours
@@ -1026,6 +1081,9 @@ __device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g,
   }
 }
 
+#undef UD
+#undef FS
+
 __global__ void __launch_bounds__(256) k_upwind(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
@@ -1033,11 +1091,14 @@ __global__ void __launch_bounds__(256) k_upwind(CASE_ARGS) {
   const long nY = (C.G.ndims == 2) ? (long)(C.Sy.N + 1) * C.Sy.nlines : 0;
   if (t >= nX + nY) return;
   if (C.model == 1) {
-    upwind_e1d(C.Sx, C.G.nv, C.P.gamma, C.upwind, t, C.U, C.ix, C.fIx);
+    if (C.paired) upwind_e1d<1>(C.Sx, C.G.nv, C.P.gamma, C.upwind, t, C.U, C.ix, C.fIx);
+    else upwind_e1d<0>(C.Sx, C.G.nv, C.P.gamma, C.upwind, t, C.U, C.ix, C.fIx);
   } else if (t < nX) {
-    upwind_ns2d_roe<0>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
+    if (C.paired) upwind_ns2d_roe<0, 1>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
+    else upwind_ns2d_roe<0, 0>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
   } else {
-    upwind_ns2d_roe<1>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.UT, C.iy, C.fIy);
+    if (C.paired) upwind_ns2d_roe<1, 1>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.UT, C.iy, C.fIy);
+    else upwind_ns2d_roe<1, 0>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.UT, C.iy, C.fIy);
   }
 }
 
@@ -1471,8 +1532,8 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
     blocks[K_ENDS] = 0;
     blocks[K_RECON] = nb((long)(c.Sx.N + 1) * c.Sx.nlines);
   } else {
-    blocks[K_ENDS] = nb(2 * 4 * (long)G.nv * (c.Sx.nlines + (two ? c.Sy.nlines : 0)));
-    blocks[K_RECON] = recon_sweep_blocks(c.Sx, G.nv) + (two ? recon_sweep_blocks(c.Sy, G.nv) : 0);
+    blocks[K_ENDS] = nb(2 * (recon_systems(c.Sx, G.nv, c.paired) + (two ? recon_systems(c.Sy, G.nv, c.paired) : 0)));
+    blocks[K_RECON] = recon_sweep_blocks(c.Sx, G.nv, c.paired) + (two ? recon_sweep_blocks(c.Sy, G.nv, c.paired) : 0);
   }
   blocks[K_UPWIND] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
   const bool visc = two && c.P.viscous;

@@ -88,10 +88,12 @@ struct CaseDev {
   PhysParams P;
   ReconParams R;
   int model, upwind, nstages;
+  int paired;    // reconstruction writes 0.5*(uR-uL), 0.5*(fL+fR) only (Roe with component-wise interpolation)
   double dt, A[16], b[4], c[4];
   double *u, *U, *UT, *Udot[4];
   double *flux_x, *flux_y, *ix[4], *iy[4], *fIx, *fIy;
   double *Qd, *FV, *dxinv, *imap;
+  double *ends;  // paired mode: raw end-interface values of both systems of every pair [dir][pair][side][0|N]
   double *time;  // current simulation time of the case (one double)
   int *status;   // bit 0: a zero pivot met in a line solve (tridiagLU.c:139-142 returns -1)
 };

@@ -154,6 +154,8 @@ int Case::setup(int dev, hipStream_t st) {
   }
   D.model = cfg.model_id;
   D.upwind = cfg.upwind;
+  // characteristic interpolation and rf-char upwinding need uL, uR, fL, fR separately
+  D.paired = (!cfg.characteristic && cfg.upwind == UPW_ROE) ? 1 : 0;
   D.nstages = cfg.nstages;
   D.dt = cfg.dt;
   for (int k = 0; k < 16; k++) D.A[k] = cfg.A[k];
@@ -182,6 +184,10 @@ int Case::setup(int dev, hipStream_t st) {
   D.dxinv = dalloc(cfg.dxinv.size(), st);
   D.imap = dalloc(G.plane, st);
   D.time = dalloc(1, st);
+  if (D.paired) {
+    D.ends = dalloc(4 * 2 * (size_t)nv * (D.Sx.nlines + (two ? D.Sy.nlines : 0)), st);
+    if (!D.ends) return fail(-2, "device allocation failed");
+  }
   nparts = cfl_norm_blocks(G);
   partials = dalloc(2 * (size_t)nparts, st);
   uprev = dalloc(st_sz, st);
