@@ -8,8 +8,8 @@
  *
  * Supported (the hot path): models euler1d / navierstokes2d (+ photochemistry),
  * hyp_space_scheme weno5 / crweno5, hyp_interp_type components (both models)
- * and characteristic (weno5, euler1d), WENO weights JS / mapped / borges(Z) /
- * yc and no_limiting, upwinding roe (both) and rf-char (euler1d), BCs periodic
+ * and characteristic (weno5, both models), WENO weights JS / mapped / borges(Z) /
+ * yc and no_limiting, upwinding roe and rf-char (both models), BCs periodic
  * and extrapolate, par_space_scheme 2 / 4, RK 1fe/22/33/44/ssprk3.
  */
 #include <math.h>
@@ -470,11 +470,11 @@ int orc_load(const char *dir, orc_case **out)
     if (c->ndims != 2) { orc_free(c); return 8; }
     if (read_kw_file(dir, "physics.inp", tab, 11)) { orc_free(c); return 8; }
   } else { orc_free(c); return 8; }
-  if (strcmp(c->upw, "roe") && !(c->model_id == 1 && !strcmp(c->upw, "rf-char"))) { orc_free(c); return 8; }
+  if (strcmp(c->upw, "roe") && strcmp(c->upw, "rf-char")) { orc_free(c); return 8; }
   c->include_chem = !strcmp(include_chem, "yes");
   if (c->include_chem) { if (chem_init(c, dir)) { orc_free(c); return 9; } }
   if (c->nvars != c->nflow + (c->include_chem ? c->n_react : 0)) { orc_free(c); return 10; }
-  if (c->is_char && (c->is_crweno || c->model_id != 1)) { orc_free(c); return 11; }
+  if (c->is_char && c->is_crweno) { orc_free(c); return 11; }
   /* time (TimeInitialize.c:51) and work arrays */
   c->waqt = (double)c->restart_iter * c->dt;
   {
@@ -922,15 +922,21 @@ static void recon_line_components(const orc_case *c, const double *fC, const dou
  * WENOFifthOrderCalculateWeightsChar* (WENOFifthOrderCalculateWeights.c:761-930),
  * Interp1PrimFifthOrderWENOChar.c:89-201. */
 static void recon_line_char(const orc_case *c, const double *fC, const double *uC, int base, int st,
-                            int N, double *uLo, double *uRo, double *fLo, double *fRo, size_t ostride)
+                            int N, int dir, double *uLo, double *uRo, double *fLo, double *fRo, size_t ostride)
 {
   int nv = c->nvars, k, v, j, kind = weight_kind(c), which;
   static const double s6 = 1.0 / 6.0;
   double L[NMAX * NMAX], R[NMAX * NMAX], ua[NMAX], fch[NMAX];
   for (k = 0; k <= N; k++) {
-    /* L from the Roe average of the two cells adjacent to interface k */
-    roeavg_euler1d(c, ua, LP(uC, k - 1), LP(uC, k));
-    eig_euler1d(c, ua, NULL, L, R);
+    /* L from the Roe average of the two cells adjacent to interface k (the
+       physics' AveragingFunction: Euler1DRoeAverage / NavierStokes2DRoeAverage) */
+    if (c->model_id == 1) {
+      roeavg_euler1d(c, ua, LP(uC, k - 1), LP(uC, k));
+      eig_euler1d(c, ua, NULL, L, R);
+    } else {
+      roeavg_ns2d(c, ua, LP(uC, k - 1), LP(uC, k));
+      eig_ns2d(c, ua, dir, NULL, L, R);
+    }
     for (which = 0; which < 4; which++) {
       const double *src = (which < 2) ? uC : fC;
       int upw = (which == 0 || which == 2) ? 1 : -1;
@@ -997,6 +1003,33 @@ static void upwind_line(const orc_case *c, const double *uC, int base, int st, i
       matvec(nv, fo, R, fc);
       continue;
     }
+    if (c->model_id == 2 && rf) {
+      /* NavierStokes2DUpwindRF (NavierStokes2DUpwind.c:106-187): Roe average of
+         the two interface states; eigenvalues of uL, uR and the average */
+      double ucL[NMAX], ucR[NMAX], fcL[NMAX], fcR[NMAX], fc[NMAX], eL[NMAX], eR[NMAX], eC[NMAX];
+      roeavg_ns2d(c, ua, uLk, uRk);
+      eig_ns2d(c, ua, dir, D, L, R);
+      matvec(nv, ucL, L, uLk); matvec(nv, ucR, L, uRk);
+      matvec(nv, fcL, L, fLk); matvec(nv, fcR, L, fRk);
+      eig_ns2d(c, uLk, dir, D, NULL, NULL);
+      for (m = 0; m < nv; m++) eL[m] = D[m * nv + m];
+      eig_ns2d(c, uRk, dir, D, NULL, NULL);
+      for (m = 0; m < nv; m++) eR[m] = D[m * nv + m];
+      eig_ns2d(c, ua, dir, D, NULL, NULL);
+      for (m = 0; m < nv; m++) eC[m] = D[m * nv + m];
+      for (m = 0; m < nv; m++) {
+        if (eL[m] > 0 && eC[m] > 0 && eR[m] > 0) fc[m] = fcL[m];
+        else if (eL[m] < 0 && eC[m] < 0 && eR[m] < 0) fc[m] = fcR[m];
+        else {
+          double aL = fabs(eL[m]), aC = fabs(eC[m]), aR = fabs(eR[m]);
+          double m1 = (aL > aC) ? aL : aC, m2 = (aC > aR) ? aC : aR;
+          double alpha = (m1 > m2) ? m1 : m2;
+          fc[m] = 0.5 * (fcL[m] + fcR[m] + alpha * (ucL[m] - ucR[m]));
+        }
+      }
+      matvec(nv, fo, R, fc);
+      continue;
+    }
     for (m = 0; m < nv; m++) ud[m] = 0.5 * (uRk[m] - uLk[m]);
     if (c->model_id == 1) {
       roeavg_euler1d(c, ua, LP(uC, k - 1), LP(uC, k));
@@ -1043,7 +1076,7 @@ static void hyperbolic(orc_case *c, const double *u, double *hyp)
       if (other >= 0) idx[other] = l;
       idx[d] = 0;
       base = gidx(c, idx);
-      if (c->is_char) recon_line_char(c, c->fluxC, u, base, st, N, uLl, uRl, fLl, fRl, (size_t)nv);
+      if (c->is_char) recon_line_char(c, c->fluxC, u, base, st, N, d, uLl, uRl, fLl, fRl, (size_t)nv);
       else recon_line_components(c, c->fluxC, u, base, st, N, uLl, uRl, fLl, fRl, (size_t)nv);
       upwind_line(c, u, base, st, N, d, uLl, uRl, fLl, fRl, fIl, (size_t)nv);
       /* hyp += dxinv * (fI[k+1] - fI[k]) at the N interior points of the line */

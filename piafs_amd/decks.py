@@ -278,6 +278,36 @@ def isentropic_vortex(N: int = 60, n_iter: int = 400, dt: float = 0.05, scheme: 
     return Deck(solver, boundary, physics, [x, y], u, weno)
 
 
+def riemann_2d(N: int = 201, n_iter: int = 250, dt: float = 0.001, interp: str = "characteristic",
+               upwinding: str = "rf-char", rk: str = "ssprk3", mapped: int = 1, scheme: str = "weno5",
+               gamma: float = 1.4) -> Deck:
+    """Two-dimensional Riemann problem on [-0.5, 0.5]^2 (grid points x_i =
+    -0.5 + i/(N-1)), inviscid, mirror ("extrapolate") BCs on all faces: the
+    four quadrant states of Examples/2D_NavierStokes/Riemann2DCase4 (rho, u, v,
+    p) = (1.1, 0.8939, 0.8939, 1.1) for x < 0, y < 0; (0.5065, 0, 0.8939,
+    0.35) for x >= 0, y < 0; (0.5065, 0.8939, 0, 0.35) for x < 0, y >= 0;
+    (1.1, 0, 0, 1.1) otherwise."""
+    h = 1.0 / (N - 1)
+    x = -0.5 + np.arange(N) * h
+    y = -0.5 + np.arange(N) * h
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    rho = np.where(X < 0, np.where(Y < 0, 1.1, 0.5065), np.where(Y < 0, 0.5065, 1.1))
+    P = np.where(X < 0, np.where(Y < 0, 1.1, 0.35), np.where(Y < 0, 0.35, 1.1))
+    uu = np.where((X < 0) & (Y >= 0), 0.8939, np.where((X < 0) & (Y < 0), 0.8939, 0.0))
+    vv = np.where((X >= 0) & (Y < 0), 0.8939, np.where((X < 0) & (Y < 0), 0.8939, 0.0))
+    u = np.stack([rho, rho * uu, rho * vv, P / (gamma - 1.0) + 0.5 * rho * (uu * uu + vv * vv)], axis=-1)
+    u = u.reshape(-1, 4)
+    solver = dict(ndims=2, nvars=4, size=(N, N), ghost=3, n_iter=n_iter, time_scheme="rk",
+                  time_scheme_type=rk, hyp_space_scheme=scheme, hyp_interp_type=interp,
+                  dt=dt, screen_op_iter=n_iter, file_op_iter=n_iter * 10,
+                  op_file_format="none", ip_file_type="binary", op_overwrite="yes", model="navierstokes2d")
+    boundary = [("extrapolate", 0, 1, 0, 0, -0.5, 0.5), ("extrapolate", 0, -1, 0, 0, -0.5, 0.5),
+                ("extrapolate", 1, 1, -0.5, 0.5, 0, 0), ("extrapolate", 1, -1, -0.5, 0.5, 0, 0)]
+    physics = dict(gamma=gamma, upwinding=upwinding)
+    weno = dict(mapped=mapped, borges=0, yc=0, no_limiting=0, epsilon=1e-6)
+    return Deck(solver, boundary, physics, [x, y], u, weno)
+
+
 def density_sine_1d(N: int = 128, n_iter: int = 50, dt: float = 0.002, scheme: str = "crweno5",
                     interp: str = "components", rk: str = "44", gamma: float = 1.4, amp: float = 0.1) -> Deck:
     """Smooth 1D density wave rho = 1 + amp sin(2 pi x), u = 1, p = 1/gamma on a

@@ -56,6 +56,18 @@ CASES = {
     "ns2d_chem_weno_mixed_cfg3_17x31": ("ns2d_photochem", dict(nx=17, ny=31, n_iter=6, scheme="weno5",
                                                                amplitude=1.7, Re=300.0, dx=10.0 * np.pi / 1024 * 8)),
     "vortex_crweno_mapped_60_400": ("isentropic_vortex", dict(N=60, n_iter=400, dt=0.05)),
+    # characteristic WENO5 and rf-char upwinding in 2D
+    "ns2d_chem_char_rf_24x20": ("ns2d_photochem", dict(nx=24, ny=20, n_iter=10, scheme="weno5",
+                                                       interp="characteristic", upwinding="rf-char")),
+    "ns2d_chem_char_roe_mapped_20x16": ("ns2d_photochem", dict(nx=20, ny=16, n_iter=8, scheme="weno5",
+                                                               interp="characteristic", weights="mapped")),
+    "ns2d_chem_comp_rf_crweno_20x24": ("ns2d_photochem", dict(nx=20, ny=24, n_iter=8, upwinding="rf-char")),
+    "ns2d_nochem_char_rf_z_16x20": ("ns2d_photochem", dict(nx=16, ny=20, n_iter=8, scheme="weno5",
+                                                           interp="characteristic", upwinding="rf-char",
+                                                           weights="z", include_chemistry=False)),
+    "riemann2d_char_rf_41_40": ("riemann_2d", dict(N=41, n_iter=40, dt=0.004)),
+    "riemann2d_comp_roe_crweno_33_30": ("riemann_2d", dict(N=33, n_iter=30, dt=0.005, interp="components",
+                                                           upwinding="roe", scheme="crweno5")),
 }
 
 
