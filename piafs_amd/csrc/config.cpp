@@ -358,7 +358,7 @@ int Config::parse(const std::string &dir, std::string &err) {
     if (read_keywords(join(dir, "physics.inp"), on_key, err) == 1) return 1;
   }
   if (upw_choice == "roe") upwind = UPW_ROE;
-  else if (upw_choice == "rf-char" && model_id == MODEL_EULER1D) upwind = UPW_RF;
+  else if (upw_choice == "rf-char") upwind = UPW_RF;
   else {
     err = upw_choice + " is not a valid upwinding scheme on this path";
     return 1;
@@ -464,8 +464,8 @@ int Config::parse(const std::string &dir, std::string &err) {
     err = "nvars has to be " + std::to_string(expect) + " in solver.inp";
     return 1;
   }
-  if (characteristic && (crweno || model_id != MODEL_EULER1D)) {
-    err = "characteristic interpolation is supported for weno5 with euler1d on this path";
+  if (characteristic && crweno) {
+    err = "characteristic interpolation is supported for weno5 on this path";
     return 1;
   }
   if (nvars > 64) {

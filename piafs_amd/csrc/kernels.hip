@@ -184,6 +184,110 @@ __device__ __forceinline__ void euler1d_eig(double g, double u0, double u1, doub
 }
 
 // ---------------------------------------------------------------------------
+// 2D Navier-Stokes flow eigenstructure along direction DIR (0: x, 1: y).
+// Only the 4x4 flow block is formed: the species block of the reference's
+// n x n matrices is the identity (L, R) with eigenvalue vn.
+// ---------------------------------------------------------------------------
+// Roe average of two conserved flow states (navierstokes2d.h:115-160)
+__device__ __forceinline__ void ns2d_roeavg(double g, const double uL[4], const double uR[4], double ua[4]) {
+  const double rhoL = uL[0], vxL = uL[1] / rhoL, vyL = uL[2] / rhoL, eL = uL[3];
+  const double vsqL = (vxL * vxL) + (vyL * vyL), PL = (eL - 0.5 * rhoL * vsqL) * (g - 1.0);
+  const double cLsq = g * PL / rhoL, HL = 0.5 * (vxL * vxL + vyL * vyL) + cLsq / (g - 1.0);
+  const double rhoR = uR[0], vxR = uR[1] / rhoR, vyR = uR[2] / rhoR, eR = uR[3];
+  const double vsqR = (vxR * vxR) + (vyR * vyR), PR = (eR - 0.5 * rhoR * vsqR) * (g - 1.0);
+  const double cRsq = g * PR / rhoR, HR = 0.5 * (vxR * vxR + vyR * vyR) + cRsq / (g - 1.0);
+  const double tL = sqrt(rhoL), tR = sqrt(rhoR);
+  const double rA = tL * tR;
+  const double vxA = (tL * vxL + tR * vxR) / (tL + tR), vyA = (tL * vyL + tR * vyR) / (tL + tR);
+  const double HA = (tL * HL + tR * HR) / (tL + tR), vsqA = vxA * vxA + vyA * vyA;
+  const double csqA = (g - 1.0) * (HA - 0.5 * vsqA), PA = csqA * rA / g;
+  ua[0] = rA;
+  ua[1] = rA * vxA;
+  ua[2] = rA * vyA;
+  ua[3] = PA / (g - 1.0) + 0.5 * rA * vsqA;
+}
+
+// Flow eigenvalues, in the reference's order (navierstokes2d.h:161-184)
+template <int DIR>
+__device__ __forceinline__ void ns2d_eigvals(double g, const double u[4], double D[4]) {
+  const double rho = u[0], vx = (rho == 0) ? 0 : u[1] / rho, vy = (rho == 0) ? 0 : u[2] / rho, e = u[3];
+  const double vsq = (vx * vx) + (vy * vy), P = (e - 0.5 * rho * vsq) * (g - 1.0);
+  const double a = sqrt(g * P / rho), vn = DIR == 0 ? vx : vy;
+  D[0] = vn - a;
+  if (DIR == 0) { D[1] = vn + a; D[2] = vn; }
+  else { D[1] = vn; D[2] = vn + a; }
+  D[3] = vn;
+}
+
+// Left (rows) and right (columns) flow eigenvectors (navierstokes2d.h:186-311)
+template <int DIR>
+__device__ __forceinline__ void ns2d_eigvecs(double g, const double u[4], double L[4][4], double R[4][4]) {
+  const double gm1 = g - 1.0;
+  const double rho = u[0], vx = (rho == 0) ? 0 : u[1] / rho, vy = (rho == 0) ? 0 : u[2] / rho, e = u[3];
+  const double vsq = (vx * vx) + (vy * vy), P = (e - 0.5 * rho * vsq) * (g - 1.0);
+  const double a = sqrt(g * P / rho), ek = 0.5 * (vx * vx + vy * vy);
+  const double aa2 = 2 * a * a, aa = a * a;
+  if (DIR == 0) {
+    const double un = vx, nx = 1.0, ny = 0.0;
+    L[0][0] = (gm1 * ek + a * un) / aa2;
+    L[0][1] = ((-gm1) * vx - a * nx) / aa2;
+    L[0][2] = ((-gm1) * vy - a * ny) / aa2;
+    L[0][3] = gm1 / aa2;
+    L[3][0] = (aa - gm1 * ek) / aa;
+    L[3][1] = (gm1 * vx) / aa;
+    L[3][2] = (gm1 * vy) / aa;
+    L[3][3] = (-gm1) / aa;
+    L[1][0] = (gm1 * ek - a * un) / aa2;
+    L[1][1] = ((-gm1) * vx + a * nx) / aa2;
+    L[1][2] = ((-gm1) * vy + a * ny) / aa2;
+    L[1][3] = gm1 / aa2;
+    L[2][0] = (vy - un * ny) / nx;
+    L[2][1] = ny;
+    L[2][2] = (ny * ny - 1.0) / nx;
+    L[2][3] = 0.0;
+  } else {
+    const double un = vy, nx = 0.0, ny = 1.0;
+    L[0][0] = (gm1 * ek + a * un) / aa2;
+    L[0][1] = ((1.0 - g) * vx - a * nx) / aa2;
+    L[0][2] = ((1.0 - g) * vy - a * ny) / aa2;
+    L[0][3] = gm1 / aa2;
+    L[3][0] = (aa - gm1 * ek) / aa;
+    L[3][1] = gm1 * vx / aa;
+    L[3][2] = gm1 * vy / aa;
+    L[3][3] = (1.0 - g) / aa;
+    L[2][0] = (gm1 * ek - a * un) / aa2;
+    L[2][1] = ((1.0 - g) * vx + a * nx) / aa2;
+    L[2][2] = ((1.0 - g) * vy + a * ny) / aa2;
+    L[2][3] = gm1 / aa2;
+    L[1][0] = (un * nx - vx) / ny;
+    L[1][1] = (1.0 - nx * nx) / ny;
+    L[1][2] = -nx;
+    L[1][3] = 0;
+  }
+  const double h0 = a * a / gm1 + ek;
+  const double nx = DIR == 0 ? 1.0 : 0.0, ny = DIR == 0 ? 0.0 : 1.0, un = DIR == 0 ? vx : vy;
+  R[0][0] = 1.0; R[1][0] = vx - a * nx; R[2][0] = vy - a * ny; R[3][0] = h0 - a * un;
+  R[0][3] = 1.0; R[1][3] = vx; R[2][3] = vy; R[3][3] = ek;
+  const int cp = DIR == 0 ? 1 : 2, cz = DIR == 0 ? 2 : 1;
+  R[0][cp] = 1.0; R[1][cp] = vx + a * nx; R[2][cp] = vy + a * ny; R[3][cp] = h0 + a * un;
+  R[0][cz] = 0.0; R[1][cz] = ny; R[2][cz] = -nx; R[3][cz] = vx * ny - vy * nx;
+}
+
+// rf-char selection for one characteristic field (Euler1DUpwindRF,
+// Euler1DUpwind.c:160-173; NavierStokes2DUpwindRF, NavierStokes2DUpwind.c:152-166):
+// upwind when the field's eigenvalue has one sign at the left, averaged and
+// right states, else a local Lax-Friedrichs flux with the largest |eigenvalue|.
+__device__ __forceinline__ double rf_pick(double eL, double eC, double eR, double ucL, double ucR, double fcL,
+                                          double fcR) {
+  if (eL > 0 && eC > 0 && eR > 0) return fcL;
+  if (eL < 0 && eC < 0 && eR < 0) return fcR;
+  const double x1 = fabs(eL), x2 = fabs(eC), x3 = fabs(eR);
+  const double mx1 = x1 > x2 ? x1 : x2, mx2 = x2 > x3 ? x2 : x3;
+  const double alpha = mx1 > mx2 ? mx1 : mx2;
+  return 0.5 * (fcL + fcR + alpha * (ucL - ucR));
+}
+
+// ---------------------------------------------------------------------------
 // Ensemble indexing: block -> case, local work item
 // ---------------------------------------------------------------------------
 #define CASE_ARGS const CaseDev *__restrict__ cases, const int *__restrict__ blockcase, \
@@ -826,6 +930,108 @@ __device__ __forceinline__ void recon_char_e1d(const SweepGeom &S, int nv, const
   }
 }
 
+// Characteristic WENO5 for navierstokes2d along DIR, one thread per
+// (interface, line): Interp1PrimFifthOrderWENOChar.c:89-201 with
+// WENOFifthOrderCalculateWeightsChar* (WENOFifthOrderCalculateWeights.c:761-930).
+// L and R come from the Roe average of the two cells adjacent to the
+// interface (navierstokes2d.h:115-311).  The flow values of the six stencil
+// positions k-3 .. k+2 are projected once and serve both biases; the species
+// rows of L and R are the identity, so species reconstruct component-wise.
+// Products are summed in index order, as MatVecMult does (the reference's
+// zero entries add exact zeros).
+template <int DIR, int KIND>
+__device__ __forceinline__ void recon_char_ns2d(const SweepGeom &S, int nv, const ReconParams &P, double gam, int l,
+                                                int k, const double *__restrict__ q, const double *__restrict__ f,
+                                                double *const *io) {
+  const int nl = S.nlines, N = S.N;
+  const long pl = S.plane;
+  const double *ql = q + S.base + (long)l * S.line_stride, *fl = f + S.base + (long)l * S.line_stride;
+  const double uLc[4] = {ql[k - 1], ql[pl + k - 1], ql[2 * pl + k - 1], ql[3 * pl + k - 1]};
+  const double uRc[4] = {ql[k], ql[pl + k], ql[2 * pl + k], ql[3 * pl + k]};
+  double ua[4], L[4][4], R[4][4];
+  ns2d_roeavg(gam, uLc, uRc, ua);
+  ns2d_eigvecs<DIR>(gam, ua, L, R);
+  const long vs = (long)nl * (N + 1), ii = (long)l * (N + 1) + k;
+  const double s6 = 1.0 / 6.0;
+  auto weno = [&](double m3, double m2, double m1, double p1, double p2) {
+    double w1, w2, w3;
+    if (P.no_limiting) { w1 = 0.1; w2 = 0.6; w3 = 0.3; }
+    else weno_w<KIND>(0.1, 0.6, 0.3, m3, m2, m1, p1, p2, P.eps, w1, w2, w3);
+    const double f1 = (2 * s6) * m3 - (7.0 * s6) * m2 + (11.0 * s6) * m1;
+    const double f2 = (-s6) * m2 + (5.0 * s6) * m1 + (2 * s6) * p1;
+    const double f3 = (2 * s6) * m1 + (5 * s6) * p1 - (s6)*p2;
+    return w1 * f1 + w2 * f2 + w3 * f3;
+  };
+#pragma unroll
+  for (int src = 0; src < 2; src++) {  // 0: the solution (uL, uR), 1: the flux (fL, fR)
+    const double *a = src ? fl : ql;
+    double c[6][4];  // characteristic values at k-3 .. k+2
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+      const int pos = k - 3 + p;
+      const double x0 = a[pos], x1 = a[pl + pos], x2 = a[2 * pl + pos], x3 = a[3 * pl + pos];
+#pragma unroll
+      for (int v = 0; v < 4; v++) c[p][v] = L[v][0] * x0 + L[v][1] * x1 + L[v][2] * x2 + L[v][3] * x3;
+    }
+#pragma unroll
+    for (int side = 0; side < 2; side++) {  // 0: left-biased (k-3 .. k+1), 1: right-biased (k+2 .. k-2)
+      double fch[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+        fch[v] = side ? weno(c[5][v], c[4][v], c[3][v], c[2][v], c[1][v])
+                      : weno(c[0][v], c[1][v], c[2][v], c[3][v], c[4][v]);
+      double *out = io[2 * src + side] + ii;
+#pragma unroll
+      for (int i = 0; i < 4; i++) out[i * vs] = R[i][0] * fch[0] + R[i][1] * fch[1] + R[i][2] * fch[2] + R[i][3] * fch[3];
+    }
+    for (int m = 4; m < nv; m++) {
+      const double *am = a + m * pl + k;
+      const double x[6] = {am[-3], am[-2], am[-1], am[0], am[1], am[2]};
+      io[2 * src][m * vs + ii] = weno(x[0], x[1], x[2], x[3], x[4]);
+      io[2 * src + 1][m * vs + ii] = weno(x[5], x[4], x[3], x[2], x[1]);
+    }
+  }
+}
+
+// Characteristic reconstruction: one thread per (interface, line) of each
+// sweep, x sweep first (from the natural layout), then y (from the
+// transposed copy).  Kept apart from the line-solve kernel so that neither's
+// register allocation limits the other's occupancy.
+__global__ void __launch_bounds__(256) k_recon_char(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const long nX = (long)(C.Sx.N + 1) * C.Sx.nlines;
+  const long nY = (C.G.ndims == 2) ? (long)(C.Sy.N + 1) * C.Sy.nlines : 0;
+  if (t >= nX + nY) return;
+  const int dir = t < nX ? 0 : 1;
+  const SweepGeom &S = dir ? C.Sy : C.Sx;
+  const long u = dir ? t - nX : t;
+  const int k = (int)(u % (S.N + 1)), l = (int)(u / (S.N + 1));
+  const int nv = C.G.nv;
+  const double gam = C.P.gamma;
+  if (C.model == 1) {
+    switch (C.R.kind) {
+      case 1: recon_char_e1d<1>(S, nv, C.R, gam, l, k, C.U, C.flux_x, C.ix); break;
+      case 2: recon_char_e1d<2>(S, nv, C.R, gam, l, k, C.U, C.flux_x, C.ix); break;
+      case 3: recon_char_e1d<3>(S, nv, C.R, gam, l, k, C.U, C.flux_x, C.ix); break;
+      default: recon_char_e1d<0>(S, nv, C.R, gam, l, k, C.U, C.flux_x, C.ix); break;
+    }
+    return;
+  }
+  const double *q = dir ? C.UT : C.U, *f = dir ? C.flux_y : C.flux_x;
+  double *const *io = dir ? C.iy : C.ix;
+#define RCHAR(KIND)                                                                  \
+  (dir ? recon_char_ns2d<1, KIND>(S, nv, C.R, gam, l, k, q, f, io)                  \
+       : recon_char_ns2d<0, KIND>(S, nv, C.R, gam, l, k, q, f, io))
+  switch (C.R.kind) {
+    case 1: RCHAR(1); break;
+    case 2: RCHAR(2); break;
+    case 3: RCHAR(3); break;
+    default: RCHAR(0); break;
+  }
+#undef RCHAR
+}
+
 // Blocks of a sweep for the team-based reconstruction (each block holds
 // 256/team whole teams; a block never mixes sweeps).
 __host__ __device__ inline int recon_sweep_blocks(const SweepGeom &S, int nv, int paired) {
@@ -856,18 +1062,6 @@ __global__ void __launch_bounds__(256) k_recon_ends(CASE_ARGS) {
 __global__ void __launch_bounds__(256) k_recon(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
-  if (C.R.characteristic) {
-    const SweepGeom &S = C.Sx;
-    if (t >= (long)(S.N + 1) * S.nlines) return;
-    const int k = (int)(t % (S.N + 1)), l = (int)(t / (S.N + 1));
-    switch (C.R.kind) {
-      case 1: recon_char_e1d<1>(S, C.G.nv, C.R, C.P.gamma, l, k, C.U, C.flux_x, C.ix); break;
-      case 2: recon_char_e1d<2>(S, C.G.nv, C.R, C.P.gamma, l, k, C.U, C.flux_x, C.ix); break;
-      case 3: recon_char_e1d<3>(S, C.G.nv, C.R, C.P.gamma, l, k, C.U, C.flux_x, C.ix); break;
-      default: recon_char_e1d<0>(S, C.G.nv, C.R, C.P.gamma, l, k, C.U, C.flux_x, C.ix); break;
-    }
-    return;
-  }
   __shared__ double red[16];
   __shared__ double stash[kMaxRows * 256];
   const int bid = (int)(blockIdx.x - firstblock[cidx]);
@@ -909,83 +1103,18 @@ __device__ __forceinline__ void upwind_ns2d_roe(const SweepGeom &S, int nv, doub
   const int k = (int)(t % (N + 1)), l = (int)(t / (N + 1));
   const long pl = S.plane;
   const long cL = S.base + (long)l * S.line_stride + (k - 1), cR = cL + 1;
-  // Roe average of the cell-centred neighbours (navierstokes2d.h:115-160)
-  const double rhoL = q[cL], vxL = q[pl + cL] / rhoL, vyL = q[2 * pl + cL] / rhoL, eL = q[3 * pl + cL];
-  const double vsqL = (vxL * vxL) + (vyL * vyL), PL = (eL - 0.5 * rhoL * vsqL) * (g - 1.0);
-  const double cLsq = g * PL / rhoL, HL = 0.5 * (vxL * vxL + vyL * vyL) + cLsq / (g - 1.0);
-  const double rhoR = q[cR], vxR = q[pl + cR] / rhoR, vyR = q[2 * pl + cR] / rhoR, eR = q[3 * pl + cR];
-  const double vsqR = (vxR * vxR) + (vyR * vyR), PR = (eR - 0.5 * rhoR * vsqR) * (g - 1.0);
-  const double cRsq = g * PR / rhoR, HR = 0.5 * (vxR * vxR + vyR * vyR) + cRsq / (g - 1.0);
-  const double tL = sqrt(rhoL), tR = sqrt(rhoR);
-  const double rA = tL * tR;
-  const double vxA = (tL * vxL + tR * vxR) / (tL + tR), vyA = (tL * vyL + tR * vyR) / (tL + tR);
-  const double HA = (tL * HL + tR * HR) / (tL + tR), vsqA = vxA * vxA + vyA * vyA;
-  const double csqA = (g - 1.0) * (HA - 0.5 * vsqA), PA = csqA * rA / g;
-  const double eA = PA / (g - 1.0) + 0.5 * rA * vsqA;
-  const double ua0 = rA, ua1 = rA * vxA, ua2 = rA * vyA, ua3 = eA;
-  // eigenstructure of the averaged state (navierstokes2d.h:166-311)
-  const double gm1 = g - 1.0;
-  const double rho = ua0, vx = (rho == 0) ? 0 : ua1 / rho, vy = (rho == 0) ? 0 : ua2 / rho, e = ua3;
-  const double vsq = (vx * vx) + (vy * vy), P = (e - 0.5 * rho * vsq) * (g - 1.0);
-  const double a = sqrt(g * P / rho), ek = 0.5 * (vx * vx + vy * vy);
-  const double vn = DIR == 0 ? vx : vy;
-  double D[4];
-  D[0] = vn - a;
-  if (DIR == 0) { D[1] = vn + a; D[2] = vn; }
-  else { D[1] = vn; D[2] = vn + a; }
-  D[3] = vn;
+  // Roe average of the cell-centred neighbours, then its eigenstructure
+  const double uLc[4] = {q[cL], q[pl + cL], q[2 * pl + cL], q[3 * pl + cL]};
+  const double uRc[4] = {q[cR], q[pl + cR], q[2 * pl + cR], q[3 * pl + cR]};
+  double ua[4], D[4], L[4][4], R[4][4];
+  ns2d_roeavg(g, uLc, uRc, ua);
+  ns2d_eigvals<DIR>(g, ua, D);
+  ns2d_eigvecs<DIR>(g, ua, L, R);
+  const double vn = D[3];
   const double delta = 0.000001, delta2 = delta * delta;
   const double Dsp = (fabs(vn) < delta ? (vn * vn + delta2) / (2 * delta) : fabs(vn));
 #pragma unroll
   for (int m = 0; m < 4; m++) D[m] = (fabs(D[m]) < delta ? (D[m] * D[m] + delta2) / (2 * delta) : fabs(D[m]));
-  double L[4][4], R[4][4];
-  const double aa2 = 2 * a * a, aa = a * a;
-  if (DIR == 0) {
-    const double un = vx, nx = 1.0, ny = 0.0;
-    L[0][0] = (gm1 * ek + a * un) / aa2;
-    L[0][1] = ((-gm1) * vx - a * nx) / aa2;
-    L[0][2] = ((-gm1) * vy - a * ny) / aa2;
-    L[0][3] = gm1 / aa2;
-    L[3][0] = (aa - gm1 * ek) / aa;
-    L[3][1] = (gm1 * vx) / aa;
-    L[3][2] = (gm1 * vy) / aa;
-    L[3][3] = (-gm1) / aa;
-    L[1][0] = (gm1 * ek - a * un) / aa2;
-    L[1][1] = ((-gm1) * vx + a * nx) / aa2;
-    L[1][2] = ((-gm1) * vy + a * ny) / aa2;
-    L[1][3] = gm1 / aa2;
-    L[2][0] = (vy - un * ny) / nx;
-    L[2][1] = ny;
-    L[2][2] = (ny * ny - 1.0) / nx;
-    L[2][3] = 0.0;
-  } else {
-    const double un = vy, nx = 0.0, ny = 1.0;
-    L[0][0] = (gm1 * ek + a * un) / aa2;
-    L[0][1] = ((1.0 - g) * vx - a * nx) / aa2;
-    L[0][2] = ((1.0 - g) * vy - a * ny) / aa2;
-    L[0][3] = gm1 / aa2;
-    L[3][0] = (aa - gm1 * ek) / aa;
-    L[3][1] = gm1 * vx / aa;
-    L[3][2] = gm1 * vy / aa;
-    L[3][3] = (1.0 - g) / aa;
-    L[2][0] = (gm1 * ek - a * un) / aa2;
-    L[2][1] = ((1.0 - g) * vx + a * nx) / aa2;
-    L[2][2] = ((1.0 - g) * vy + a * ny) / aa2;
-    L[2][3] = gm1 / aa2;
-    L[1][0] = (un * nx - vx) / ny;
-    L[1][1] = (1.0 - nx * nx) / ny;
-    L[1][2] = -nx;
-    L[1][3] = 0;
-  }
-  {
-    const double h0 = a * a / gm1 + ek;
-    const double nx = DIR == 0 ? 1.0 : 0.0, ny = DIR == 0 ? 0.0 : 1.0, un = vn;
-    R[0][0] = 1.0; R[1][0] = vx - a * nx; R[2][0] = vy - a * ny; R[3][0] = h0 - a * un;
-    R[0][3] = 1.0; R[1][3] = vx; R[2][3] = vy; R[3][3] = ek;
-    const int cp = DIR == 0 ? 1 : 2, cz = DIR == 0 ? 2 : 1;
-    R[0][cp] = 1.0; R[1][cp] = vx + a * nx; R[2][cp] = vy + a * ny; R[3][cp] = h0 + a * un;
-    R[0][cz] = 0.0; R[1][cz] = ny; R[2][cz] = -nx; R[3][cz] = vx * ny - vy * nx;
-  }
   const long ii = (long)l * (N + 1) + k, vstride = (long)nl * (N + 1);
   double ud[4], fo[4];
 #pragma unroll
@@ -1054,31 +1183,51 @@ __device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g,
       fcL += E.L[m][j] * fL[j * vs + ii];
       fcR += E.L[m][j] * fR[j * vs + ii];
     }
-    const double eL = EL.D[m], eC = E.D[m], eR = ER.D[m];
-    if (eL > 0 && eC > 0 && eR > 0) fc[m] = fcL;
-    else if (eL < 0 && eC < 0 && eR < 0) fc[m] = fcR;
-    else {
-      const double x1 = fabs(eL), x2 = fabs(eC), x3 = fabs(eR);
-      const double mx1 = x1 > x2 ? x1 : x2, mx2 = x2 > x3 ? x2 : x3;
-      const double alpha = mx1 > mx2 ? mx1 : mx2;
-      fc[m] = 0.5 * (fcL + fcR + alpha * (ucL - ucR));
-    }
+    fc[m] = rf_pick(EL.D[m], E.D[m], ER.D[m], ucL, ucR, fcL, fcR);
   }
   for (int i = 0; i < 3; i++) fI[i * vs + ii] = E.R[i][0] * fc[0] + E.R[i][1] * fc[1] + E.R[i][2] * fc[2];
   // species rows: identity eigenvectors, eigenvalue v
   const double eLs = EL.D[0], eCs = E.D[0], eRs = ER.D[0];
-  for (int m = 3; m < nv; m++) {
-    const double ucL = uL[m * vs + ii], ucR = uR[m * vs + ii], fcL = fL[m * vs + ii], fcR = fR[m * vs + ii];
-    double r;
-    if (eLs > 0 && eCs > 0 && eRs > 0) r = fcL;
-    else if (eLs < 0 && eCs < 0 && eRs < 0) r = fcR;
-    else {
-      const double x1 = fabs(eLs), x2 = fabs(eCs), x3 = fabs(eRs);
-      const double mx1 = x1 > x2 ? x1 : x2, mx2 = x2 > x3 ? x2 : x3;
-      r = 0.5 * (fcL + fcR + (mx1 > mx2 ? mx1 : mx2) * (ucL - ucR));
-    }
-    fI[m * vs + ii] = r;
+  for (int m = 3; m < nv; m++)
+    fI[m * vs + ii] = rf_pick(eLs, eCs, eRs, uL[m * vs + ii], uR[m * vs + ii], fL[m * vs + ii], fR[m * vs + ii]);
+}
+
+// NavierStokes2DUpwindRF (NavierStokes2DUpwind.c:106-187): L and R from the
+// Roe average of the two reconstructed interface states uL, uR (not of the
+// neighbouring cells); a field is upwinded by the signs of its eigenvalue at
+// uL, uR and the average.  Species fields have eigenvalue vn and identity
+// eigenvectors.  Split mode only: uL, uR, fL, fR in io[0..3].
+template <int DIR>
+__device__ __forceinline__ void upwind_ns2d_rf(const SweepGeom &S, int nv, double g, long t, double *const *io,
+                                               double *__restrict__ fI) {
+  const int nl = S.nlines, N = S.N;
+  const int k = (int)(t % (N + 1)), l = (int)(t / (N + 1));
+  const long ii = (long)l * (N + 1) + k, vs = (long)nl * (N + 1);
+  const double *uL = io[0] + ii, *uR = io[1] + ii, *fL = io[2] + ii, *fR = io[3] + ii;
+  const double uLk[4] = {uL[0], uL[vs], uL[2 * vs], uL[3 * vs]};
+  const double uRk[4] = {uR[0], uR[vs], uR[2 * vs], uR[3 * vs]};
+  const double fLk[4] = {fL[0], fL[vs], fL[2 * vs], fL[3 * vs]};
+  const double fRk[4] = {fR[0], fR[vs], fR[2 * vs], fR[3 * vs]};
+  double ua[4], L[4][4], R[4][4], eL[4], eC[4], eR[4];
+  ns2d_roeavg(g, uLk, uRk, ua);
+  ns2d_eigvecs<DIR>(g, ua, L, R);
+  ns2d_eigvals<DIR>(g, uLk, eL);
+  ns2d_eigvals<DIR>(g, uRk, eR);
+  ns2d_eigvals<DIR>(g, ua, eC);
+  double fc[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    // MatVecMult (matmult_native.h:119-130): row sums in index order
+    const double ucL = L[m][0] * uLk[0] + L[m][1] * uLk[1] + L[m][2] * uLk[2] + L[m][3] * uLk[3];
+    const double ucR = L[m][0] * uRk[0] + L[m][1] * uRk[1] + L[m][2] * uRk[2] + L[m][3] * uRk[3];
+    const double fcL = L[m][0] * fLk[0] + L[m][1] * fLk[1] + L[m][2] * fLk[2] + L[m][3] * fLk[3];
+    const double fcR = L[m][0] * fRk[0] + L[m][1] * fRk[1] + L[m][2] * fRk[2] + L[m][3] * fRk[3];
+    fc[m] = rf_pick(eL[m], eC[m], eR[m], ucL, ucR, fcL, fcR);
   }
+#pragma unroll
+  for (int i = 0; i < 4; i++) fI[i * vs + ii] = R[i][0] * fc[0] + R[i][1] * fc[1] + R[i][2] * fc[2] + R[i][3] * fc[3];
+  for (int m = 4; m < nv; m++)
+    fI[m * vs + ii] = rf_pick(eL[3], eC[3], eR[3], uL[m * vs], uR[m * vs], fL[m * vs], fR[m * vs]);
 }
 
 #undef UD
@@ -1093,6 +1242,9 @@ __global__ void __launch_bounds__(256) k_upwind(CASE_ARGS) {
   if (C.model == 1) {
     if (C.paired) upwind_e1d<1>(C.Sx, C.G.nv, C.P.gamma, C.upwind, t, C.U, C.ix, C.fIx);
     else upwind_e1d<0>(C.Sx, C.G.nv, C.P.gamma, C.upwind, t, C.U, C.ix, C.fIx);
+  } else if (C.upwind == 1) {  // rf-char
+    if (t < nX) upwind_ns2d_rf<0>(C.Sx, C.G.nv, C.P.gamma, t, C.ix, C.fIx);
+    else upwind_ns2d_rf<1>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.iy, C.fIy);
   } else if (t < nX) {
     if (C.paired) upwind_ns2d_roe<0, 1>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
     else upwind_ns2d_roe<0, 0>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
@@ -1509,7 +1661,7 @@ __global__ void __launch_bounds__(TB) k_weno_weights(int kind, double c1, double
 // host side: work decomposition and launchers
 // ===========================================================================
 
-static const char *kKNames[K_NCLASS] = {"stage_bc", "flux", "recon_ends", "recon", "upwind", "qderiv",
+static const char *kKNames[K_NCLASS] = {"stage_bc", "flux", "recon_ends", "recon", "recon_char", "upwind", "qderiv",
                                         "viscflux", "assemble", "update"};
 const char *kclass_name(int k) { return kKNames[k]; }
 
@@ -1530,8 +1682,10 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
   blocks[K_FLUX] = nb((long)(c.Sx.N + 2 * c.Sx.g) * c.Sx.nlines + (two ? (long)(c.Sy.N + 2 * c.Sy.g) * c.Sy.nlines : 0));
   if (c.R.characteristic) {
     blocks[K_ENDS] = 0;
-    blocks[K_RECON] = nb((long)(c.Sx.N + 1) * c.Sx.nlines);
+    blocks[K_RECON] = 0;
+    blocks[K_RCHAR] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
   } else {
+    blocks[K_RCHAR] = 0;
     blocks[K_ENDS] = nb(2 * (recon_systems(c.Sx, G.nv, c.paired) + (two ? recon_systems(c.Sy, G.nv, c.paired) : 0)));
     blocks[K_RECON] = recon_sweep_blocks(c.Sx, G.nv, c.paired) + (two ? recon_sweep_blocks(c.Sy, G.nv, c.paired) : 0);
   }
@@ -1559,6 +1713,7 @@ void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, void 
   run(k_flux, K_FLUX, d, pl, s, st, mark, ctx);
   run(k_recon_ends, K_ENDS, d, pl, s, st, mark, ctx);
   run(k_recon, K_RECON, d, pl, s, st, mark, ctx);
+  run(k_recon_char, K_RCHAR, d, pl, s, st, mark, ctx);
   run(k_upwind, K_UPWIND, d, pl, s, st, mark, ctx);
   run(k_qderiv, K_QDERIV, d, pl, s, st, mark, ctx);
   run(k_viscflux, K_VISCFLUX, d, pl, s, st, mark, ctx);

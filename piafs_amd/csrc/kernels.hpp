@@ -87,7 +87,7 @@ struct CaseDev {
   BCMap bc;
   PhysParams P;
   ReconParams R;
-  int model, upwind, nstages;
+  int model, upwind, nstages;  // upwind: 0 Roe, 1 rf-char
   int paired;    // reconstruction writes 0.5*(uR-uL), 0.5*(fL+fR) only (Roe with component-wise interpolation)
   double dt, A[16], b[4], c[4];
   double *u, *U, *UT, *Udot[4];
@@ -100,7 +100,7 @@ struct CaseDev {
 
 // Kernel classes launched per RK stage (plus the final update).
 enum KClass {
-  K_STAGE = 0, K_FLUX, K_ENDS, K_RECON, K_UPWIND, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE, K_NCLASS
+  K_STAGE = 0, K_FLUX, K_ENDS, K_RECON, K_RCHAR, K_UPWIND, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE, K_NCLASS
 };
 const char *kclass_name(int k);
 

@@ -61,13 +61,19 @@ double *Case::dalloc(size_t n, hipStream_t st) {
   return (double *)p;
 }
 
+// Paired reconstruction (two interface arrays per sweep instead of four):
+// component-wise interpolation with Roe upwinding, which reads only
+// 0.5*(uR - uL) and 0.5*(fL + fR).  Characteristic interpolation and rf-char
+// upwinding need uL, uR, fL, fR separately.
+static bool paired_mode(const Config &c) { return !c.characteristic && c.upwind == UPW_ROE; }
+
 size_t Case::plan_bytes(const Config &c) {
   Grid g = make_grid(c);
   size_t plane = g.plane, nv = c.nvars;
   size_t n = nv * plane * (3 + c.nstages + 3);  // u, U, UT, Udot[s], flux_x, flux_y, uprev
   size_t ix = nv * (size_t)(c.dim[0] + 1) * (c.ndims == 2 ? c.dim[1] : 1);
   size_t iy = c.ndims == 2 ? nv * (size_t)(c.dim[1] + 1) * c.dim[0] : 0;
-  n += 5 * (ix + iy);
+  n += (paired_mode(c) ? 3 : 5) * (ix + iy);
   if (c.model_id == MODEL_NS2D) n += 12 * (size_t)g.nxg * g.nyg;
   n += plane + c.x.size() + 4096;
   return n * sizeof(double);
@@ -154,8 +160,7 @@ int Case::setup(int dev, hipStream_t st) {
   }
   D.model = cfg.model_id;
   D.upwind = cfg.upwind;
-  // characteristic interpolation and rf-char upwinding need uL, uR, fL, fR separately
-  D.paired = (!cfg.characteristic && cfg.upwind == UPW_ROE) ? 1 : 0;
+  D.paired = paired_mode(cfg) ? 1 : 0;
   D.nstages = cfg.nstages;
   D.dt = cfg.dt;
   for (int k = 0; k < 16; k++) D.A[k] = cfg.A[k];
@@ -169,12 +174,13 @@ int Case::setup(int dev, hipStream_t st) {
   for (int s = 0; s < cfg.nstages; s++) D.Udot[s] = dalloc(st_sz, st);
   D.flux_x = dalloc(st_sz, st);
   const size_t ix = (size_t)nv * (D.Sx.N + 1) * D.Sx.nlines;
-  for (int k = 0; k < 4; k++) D.ix[k] = dalloc(ix, st);
+  const int narr = D.paired ? 2 : 4;
+  for (int k = 0; k < narr; k++) D.ix[k] = dalloc(ix, st);
   D.fIx = dalloc(ix, st);
   if (two) {
     const size_t iy = (size_t)nv * (D.Sy.N + 1) * D.Sy.nlines;
     D.flux_y = dalloc(st_sz, st);
-    for (int k = 0; k < 4; k++) D.iy[k] = dalloc(iy, st);
+    for (int k = 0; k < narr; k++) D.iy[k] = dalloc(iy, st);
     D.fIy = dalloc(iy, st);
   }
   if (P.viscous) {

@@ -63,6 +63,8 @@ def test_reference_example_decks_parse(example):
     (lambda dk: dk.solver.__setitem__("time_scheme_type", "rk5"), "not a supported multi-stage"),
     (lambda dk: dk.physics.__setitem__("upwinding", "llf-char"), "not a valid upwinding"),
     (lambda dk: dk.solver.__setitem__("par_space_scheme", "3"), "parabolic"),
+    (lambda dk: (dk.solver.__setitem__("hyp_space_scheme", "crweno5"),
+                 dk.solver.__setitem__("hyp_interp_type", "characteristic")), "supported for weno5"),
     (lambda dk: dk.boundary.__setitem__(0, ("slip-wall", 0, 1, 0, 0, 0, 0)), "not supported"),
 ])
 def test_deck_errors_are_reported(mutate, msg):
@@ -74,6 +76,19 @@ def test_deck_errors_are_reported(mutate, msg):
             piafs_amd.estimate_bytes(d)
         err = piafs_amd.lib().piafs_last_error(None).decode()
     assert msg in err, err
+
+
+def test_every_reference_fixture_deck_is_accepted(ref_runs):
+    """Every configuration the shipped reference runs cover (both models,
+    component-wise and characteristic interpolation, Roe and rf-char) is on
+    this path: the library parses and sizes each deck."""
+    from conftest import build_deck
+    _, meta = ref_runs
+    for name, m in meta.items():
+        dk = build_deck(m)
+        with tempfile.TemporaryDirectory() as d:
+            dk.write(d)
+            assert piafs_amd.estimate_bytes(d) > 0, name
 
 
 def test_missing_initial_solution_is_an_error():

@@ -85,7 +85,9 @@ FIXTURES = [
     "sine_weno_comp_N100", "ns2d_chem_crweno_js_24x20", "ns2d_chem_weno_z_16x24", "ns2d_chem_par2_yc_16x16",
     "ns2d_chem_mapped_nolimit_16x16", "ns2d_chem_pulse_24x20", "ns2d_chem_2layer_16x16",
     "ns2d_nochem_mapped_20x16", "ns2d_chem_weno_mixed_cfg3_17x31", "ns2d_chem_crweno_48x40",
-    "sod_char_roe_ssprk3_N1024",
+    "sod_char_roe_ssprk3_N1024", "ns2d_chem_char_rf_24x20", "ns2d_chem_char_roe_mapped_20x16",
+    "ns2d_chem_comp_rf_crweno_20x24", "ns2d_nochem_char_rf_z_16x20", "riemann2d_char_rf_41_40",
+    "riemann2d_comp_roe_crweno_33_30",
 ]
 
 
