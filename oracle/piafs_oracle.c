@@ -9,8 +9,11 @@
  * Supported (the hot path): models euler1d / navierstokes2d (+ photochemistry),
  * hyp_space_scheme weno5 / crweno5, hyp_interp_type components (both models)
  * and characteristic (weno5, both models), WENO weights JS / mapped / borges(Z) /
- * yc and no_limiting, upwinding roe and rf-char (both models), BCs periodic
- * and extrapolate, par_space_scheme 2 / 4, RK 1fe/22/33/44/ssprk3.
+ * yc and no_limiting, upwinding roe / rf-char / llf-char / rusanov (both
+ * models), every boundary type of boundary.inp (periodic, extrapolate,
+ * dirichlet, reflect, sponge, slip and no-slip walls, subsonic inflow /
+ * outflow / ambivalent, supersonic inflow / outflow), par_space_scheme 2 / 4,
+ * RK 1fe/22/33/44/ssprk3.
  */
 #include <math.h>
 #include <stdio.h>
@@ -28,6 +31,9 @@ typedef struct {
   double xmin[3], xmax[3];
   int    is[3], ie[3];
   int    on;
+  /* type-specific data, InitializeBoundaries.c:95-150 */
+  double rho, vel[3], p, gamma;
+  double vals[NMAX];  /* dirichlet / sponge values, or inflow passive scalars */
 } obc;
 
 struct orc_case {
@@ -158,13 +164,16 @@ static int read_kw_file(const char *dir, const char *name, const kw *tab, int nt
   return 0;
 }
 
-/* boundary.inp -- InitializeBoundaries.c:60-200 (only the keyword set used
- * by periodic/extrapolate; other BC types are rejected). */
+/* boundary.inp -- InitializeBoundaries.c:60-200.  Type-specific values are
+ * read as the reference reads them (a missing value leaves zero). */
+static const char *bc_types[] = {"periodic", "extrapolate", "dirichlet", "reflect", "sponge", "noslip-wall",
+                                 "slip-wall", "subsonic-inflow", "subsonic-outflow", "subsonic-ambivalent",
+                                 "supersonic-inflow", "supersonic-outflow"};
 static int read_boundary(orc_case *c, const char *dir)
 {
   char fn[1024];
   FILE *f;
-  int n, d;
+  int n, d, v, known;
   snprintf(fn, sizeof fn, "%s/boundary.inp", dir);
   f = fopen(fn, "r");
   if (!f) return 1;
@@ -176,10 +185,32 @@ static int read_boundary(orc_case *c, const char *dir)
     if (fscanf(f, "%d %d", &b->dim, &b->face) != 2) { fclose(f); return 1; }
     for (d = 0; d < c->ndims; d++)
       if (fscanf(f, "%lf %lf", &b->xmin[d], &b->xmax[d]) != 2) { fclose(f); return 1; }
-    if (strcmp(b->type, "periodic") && strcmp(b->type, "extrapolate")) {
+    for (known = 0, v = 0; v < 12; v++) known |= !strcmp(b->type, bc_types[v]);
+    if (!known) {
       fprintf(stderr, "oracle: boundary type %s not supported\n", b->type);
       fclose(f);
       return 1;
+    }
+    if (!strcmp(b->type, "dirichlet") || !strcmp(b->type, "sponge"))
+      for (v = 0; v < c->nvars; v++) if (fscanf(f, "%lf", &b->vals[v]) != 1) break;
+    if (!strcmp(b->type, "slip-wall") || !strcmp(b->type, "noslip-wall"))
+      for (d = 0; d < c->ndims; d++) if (fscanf(f, "%lf", &b->vel[d]) != 1) break;
+    if (!strcmp(b->type, "subsonic-inflow")) {
+      if (fscanf(f, "%lf", &b->rho) != 1) {}
+      for (d = 0; d < c->ndims; d++) if (fscanf(f, "%lf", &b->vel[d]) != 1) break;
+      for (v = 0; v < c->nvars - (2 + c->ndims); v++) if (fscanf(f, "%lf", &b->vals[v]) != 1) break;
+    }
+    if (!strcmp(b->type, "subsonic-outflow")) { if (fscanf(f, "%lf", &b->p) != 1) {} }
+    if (!strcmp(b->type, "subsonic-ambivalent")) {
+      if (fscanf(f, "%lf", &b->rho) != 1) {}
+      for (d = 0; d < c->ndims; d++) if (fscanf(f, "%lf", &b->vel[d]) != 1) break;
+      if (fscanf(f, "%lf", &b->p) != 1) {}
+    }
+    if (!strcmp(b->type, "supersonic-inflow")) {
+      if (fscanf(f, "%lf", &b->rho) != 1) {}
+      for (d = 0; d < c->ndims; d++) if (fscanf(f, "%lf", &b->vel[d]) != 1) break;
+      if (fscanf(f, "%lf", &b->p) != 1) {}
+      for (v = 0; v < c->nvars - (2 + c->ndims); v++) if (fscanf(f, "%lf", &b->vals[v]) != 1) break;
     }
   }
   fclose(f);
@@ -208,7 +239,10 @@ static void boundary_extents(orc_case *c)
     int off = 0;
     b->on = 1;
     for (d = 0; d < c->ndims; d++) {
-      if (d == b->dim) {
+      if (!strcmp(b->type, "sponge")) {  /* the zone is interior points in every dimension */
+        find_interval(b->xmin[d], b->xmax[d], &c->x[off + c->ghosts], c->dim[d], &b->is[d], &b->ie[d]);
+        if (b->ie[d] - b->is[d] <= 0) b->on = 0;
+      } else if (d == b->dim) {
         if (b->face == 1) { b->is[d] = -c->ghosts; b->ie[d] = 0; }
         else { b->is[d] = c->dim[d]; b->ie[d] = c->dim[d] + c->ghosts; }
       } else {
@@ -470,7 +504,11 @@ int orc_load(const char *dir, orc_case **out)
     if (c->ndims != 2) { orc_free(c); return 8; }
     if (read_kw_file(dir, "physics.inp", tab, 11)) { orc_free(c); return 8; }
   } else { orc_free(c); return 8; }
-  if (strcmp(c->upw, "roe") && strcmp(c->upw, "rf-char")) { orc_free(c); return 8; }
+  if (strcmp(c->upw, "roe") && strcmp(c->upw, "rf-char") && strcmp(c->upw, "llf-char")
+      && strcmp(c->upw, "rusanov")) { orc_free(c); return 8; }
+  /* the zones' gamma: NavierStokes2DInitialize.c:184 sets the physics.inp value
+     before chemistry replaces it; Euler1DInitialize never sets it */
+  for (s = 0; s < c->nbc; s++) c->bc[s].gamma = (c->model_id == 2) ? c->gamma : 0.0;
   c->include_chem = !strcmp(include_chem, "yes");
   if (c->include_chem) { if (chem_init(c, dir)) { orc_free(c); return 9; } }
   if (c->nvars != c->nflow + (c->include_chem ? c->n_react : 0)) { orc_free(c); return 10; }
@@ -524,32 +562,142 @@ double orc_param(const orc_case *c, const char *name)
 }
 
 /* ------------------------------------------------------------------ */
-/* boundary conditions: BCPeriodic.c:22-62, BCExtrapolate.c:16-51       */
+/* boundary conditions, applied zone by zone in list order            */
+/* (ApplyBoundaryConditions.c:20-40): BCPeriodic.c:22-62,              */
+/* BCExtrapolate.c:16-51, BCDirichlet.c:16-45, BCReflect.c:17-50,      */
+/* BCSlipWall.c:24-135, BCNoslipWall.c:22-80, BCSubsonicInflow.c:22-90,*/
+/* BCSubsonicOutflow.c:21-78, BCSubsonicAmbivalent.c:30-125,           */
+/* BCSupersonicInflow.c:26-80, BCSupersonicOutflow.c:25-80.  A sponge  */
+/* zone acts through the source term (BCSponge.c:29-74).               */
 /* ------------------------------------------------------------------ */
+
+/* _NavierStokes2DGetFlowVar_, navierstokes2d.h:70-79 */
+static void ns2d_flowvar(const double *u, double gam, double *rho, double *vx, double *vy, double *e, double *P)
+{
+  double vsq;
+  *rho = u[0];
+  *vx = (*rho == 0) ? 0 : u[1] / *rho;
+  *vy = (*rho == 0) ? 0 : u[2] / *rho;
+  *e = u[3];
+  vsq = (*vx * *vx) + (*vy * *vy);
+  *P = (*e - 0.5 * *rho * vsq) * (gam - 1.0);
+}
 
 static void apply_bcs(orc_case *c, double *phi)
 {
   int n, nv = c->nvars, g = c->ghosts;
   for (n = 0; n < c->nbc; n++) {
     obc *b = &c->bc[n];
+    const char *t = b->type;
     int d = b->dim, bounds[3], i1[3], i2[3], k, done = 0;
-    if (!b->on) continue;
+    int generic = !strcmp(t, "periodic") || !strcmp(t, "extrapolate") || !strcmp(t, "dirichlet")
+                  || !strcmp(t, "reflect");
+    if (!b->on || !strcmp(t, "sponge")) continue;
+    /* the flow BCs exist for ndims 2 only, slip-wall also for ndims 1 */
+    if (!generic && c->ndims != 2 && !(c->ndims == 1 && !strcmp(t, "slip-wall"))) continue;
     for (k = 0; k < c->ndims; k++) { bounds[k] = b->ie[k] - b->is[k]; i1[k] = 0; }
     while (!done) {
       int p1, p2, v, tmp[3];
       for (k = 0; k < c->ndims; k++) tmp[k] = i1[k] + b->is[k];
       p1 = gidx(c, tmp);
-      for (k = 0; k < c->ndims; k++) i2[k] = tmp[k];
-      if (!strcmp(b->type, "periodic")) {
+      if (!strcmp(t, "periodic")) {
+        /* the source index is taken without the zone offset (BCPeriodic.c:44-52) */
         for (k = 0; k < c->ndims; k++) i2[k] = i1[k];
         if (b->face == 1) i2[d] = i1[d] + c->dim[d] - g;
-        /* face -1: source index = i1 (interior counted from 0) */
-      } else { /* extrapolate: mirror copy */
+        p2 = gidx(c, i2);
+        for (v = 0; v < nv; v++) phi[nv * p1 + v] = phi[nv * p2 + v];
+      } else if (!strcmp(t, "dirichlet")) {
+        for (v = 0; v < nv; v++) phi[nv * p1 + v] = b->vals[v];
+      } else {
+        for (k = 0; k < c->ndims; k++) i2[k] = tmp[k];
         if (b->face == 1) i2[d] = g - 1 - i1[d];
         else i2[d] = c->dim[d] - i1[d] - 1;
+        p2 = gidx(c, i2);
+        if (!strcmp(t, "extrapolate")) {
+          for (v = 0; v < nv; v++) phi[nv * p1 + v] = phi[nv * p2 + v];
+        } else if (!strcmp(t, "reflect")) {
+          for (v = 0; v < nv; v++) phi[nv * p1 + v] = (-1.0) * phi[nv * p2 + v];
+        } else if (c->ndims == 1) {
+          /* slip-wall, 1D (Euler1D never sets the zone's gamma: it stays 0) */
+          double gam = b->gamma, igm1 = 1.0 / (gam - 1.0);
+          double rho = phi[nv * p2], uvel = phi[nv * p2 + 1] / rho, energy = phi[nv * p2 + 2];
+          double pressure = (energy - 0.5 * rho * uvel * uvel) * (gam - 1.0);
+          double rg = rho, pg = pressure, ug = 2.0 * b->vel[0] - uvel;
+          double eg = igm1 * pg + 0.5 * rg * ug * ug;
+          phi[nv * p1 + 0] = rg;
+          phi[nv * p1 + 1] = rg * ug;
+          phi[nv * p1 + 2] = eg;
+        } else {
+          double gam = b->gamma, igm1 = 1.0 / (gam - 1.0);
+          double rho, uvel, vvel, energy, pressure, rg, ug, vg, pg, eg;
+          int scalars = 0;
+          if (!strcmp(t, "subsonic-ambivalent")) {
+            double nx = (d == 0) ? 1.0 : 0.0, ny = (d == 1) ? 1.0 : 0.0, u1, v1, u2, v2, vn;
+            int ia[3], ib[3];
+            nx *= (double)b->face; ny *= (double)b->face;
+            for (k = 0; k < c->ndims; k++) ia[k] = ib[k] = tmp[k];
+            ia[d] = (b->face == 1) ? 0 : c->dim[d] - 1;
+            ib[d] = (b->face == 1) ? ia[d] + 1 : ia[d] - 1;
+            ns2d_flowvar(phi + nv * gidx(c, ia), gam, &rho, &u1, &v1, &energy, &pressure);
+            ns2d_flowvar(phi + nv * gidx(c, ib), gam, &rho, &u2, &v2, &energy, &pressure);
+            vn = (1.5 * u1 - 0.5 * u2) * nx + (1.5 * v1 - 0.5 * v2) * ny;
+            ns2d_flowvar(phi + nv * p2, gam, &rho, &uvel, &vvel, &energy, &pressure);
+            if (vn > 0) { rg = b->rho; pg = pressure; ug = b->vel[0]; vg = b->vel[1]; }
+            else { rg = rho; pg = b->p; ug = uvel; vg = vvel; }
+          } else {
+            ns2d_flowvar(phi + nv * p2, gam, &rho, &uvel, &vvel, &energy, &pressure);
+            rg = rho; pg = pressure; ug = uvel; vg = vvel;  /* supersonic-outflow */
+            if (!strcmp(t, "slip-wall")) {
+              if (d == 0) ug = 2.0 * b->vel[0] - uvel;
+              else vg = 2.0 * b->vel[1] - vvel;
+            } else if (!strcmp(t, "noslip-wall")) {
+              ug = 2.0 * b->vel[0] - uvel;
+              vg = 2.0 * b->vel[1] - vvel;
+            } else if (!strcmp(t, "subsonic-inflow")) {
+              rg = b->rho; ug = b->vel[0]; vg = b->vel[1]; scalars = 1;
+            } else if (!strcmp(t, "subsonic-outflow")) {
+              pg = b->p;
+            } else if (!strcmp(t, "supersonic-inflow")) {
+              rg = b->rho; pg = b->p; ug = b->vel[0]; vg = b->vel[1]; scalars = 1;
+            }
+          }
+          eg = igm1 * pg + 0.5 * rg * (ug * ug + vg * vg);
+          phi[nv * p1 + 0] = rg;
+          phi[nv * p1 + 1] = rg * ug;
+          phi[nv * p1 + 2] = rg * vg;
+          phi[nv * p1 + 3] = eg;
+          if (scalars) for (v = 4; v < nv; v++) phi[nv * p1 + v] = b->vals[v - 4];
+        }
       }
-      p2 = gidx(c, i2);
-      for (v = 0; v < nv; v++) phi[nv * p1 + v] = phi[nv * p2 + v];
+      for (k = 0; k < c->ndims; k++) {
+        if (i1[k] == bounds[k] - 1) i1[k] = 0;
+        else { i1[k]++; break; }
+      }
+      if (k == c->ndims) done = 1;
+    }
+  }
+}
+
+/* BCSpongeSource (BCSponge.c:29-74), called by SourceFunction.c:44-58
+ * after the physics source, zone by zone */
+static void sponge_source(orc_case *c, const double *u, double *src)
+{
+  int n, nv = c->nvars, g = c->ghosts;
+  for (n = 0; n < c->nbc; n++) {
+    obc *b = &c->bc[n];
+    int d = b->dim, bounds[3], i1[3], k, v, done = 0, off = 0;
+    if (!b->on || strcmp(b->type, "sponge")) continue;
+    for (k = 0; k < d; k++) off += c->dim[k] + 2 * g;
+    for (k = 0; k < c->ndims; k++) { bounds[k] = b->ie[k] - b->is[k]; i1[k] = 0; }
+    while (!done) {
+      int tmp[3], p;
+      double x, sigma;
+      for (k = 0; k < c->ndims; k++) tmp[k] = i1[k] + b->is[k];
+      x = c->x[off + g + tmp[d]];
+      if (b->face > 0) sigma = (x - b->xmin[d]) / (b->xmax[d] - b->xmin[d]);
+      else sigma = (x - b->xmax[d]) / (b->xmin[d] - b->xmax[d]);
+      p = gidx(c, tmp);
+      for (v = 0; v < nv; v++) src[nv * p + v] -= (sigma * (u[nv * p + v] - b->vals[v]));
       for (k = 0; k < c->ndims; k++) {
         if (i1[k] == bounds[k] - 1) i1[k] = 0;
         else { i1[k]++; break; }
@@ -975,11 +1123,76 @@ static void upwind_line(const orc_case *c, const double *uC, int base, int st, i
   int nv = c->nvars, k, m;
   double D[NMAX * NMAX], L[NMAX * NMAX], R[NMAX * NMAX], DL[NMAX * NMAX], mA[NMAX * NMAX];
   double ua[NMAX], ud[NMAX], us[NMAX];
-  int rf = !strcmp(c->upw, "rf-char");
+  int rf = !strcmp(c->upw, "rf-char"), llf = !strcmp(c->upw, "llf-char"), rus = !strcmp(c->upw, "rusanov");
   for (k = 0; k <= N; k++) {
     const double *uLk = uLi + k * ostride, *uRk = uRi + k * ostride;
     const double *fLk = fLi + k * ostride, *fRk = fRi + k * ostride;
     double *fo = fI + k * ostride;
+    if (rus) {
+      /* Euler1DUpwindRusanov (Euler1DUpwind.c:282-353; its wave speed is
+         gamma*P/rho + |u|, without a square root) and NavierStokes2DUpwindRusanov
+         (NavierStokes2DUpwind.c:271-332) */
+      double a[3], alpha;
+      const double *pts[3];
+      int q;
+      if (c->model_id == 1) roeavg_euler1d(c, ua, LP(uC, k - 1), LP(uC, k));
+      else roeavg_ns2d(c, ua, LP(uC, k - 1), LP(uC, k));
+      pts[0] = LP(uC, k - 1); pts[1] = LP(uC, k); pts[2] = ua;
+      for (q = 0; q < 3; q++) {
+        if (c->model_id == 1) {
+          double rho = pts[q][0], uvel = pts[q][1] / rho, E = pts[q][2];
+          double P = (E - 0.5 * rho * uvel * uvel) * (c->gamma - 1.0);
+          a[q] = c->gamma * P / rho + fabs(uvel);
+        } else {
+          double rho, vx, vy, E, P;
+          ns2d_flowvar(pts[q], c->gamma, &rho, &vx, &vy, &E, &P);
+          a[q] = sqrt(c->gamma * P / rho) + fabs(dir == 0 ? vx : vy);
+        }
+      }
+      {  /* max3(alphaL, alphaR, alphaavg), math_ops.h:30 */
+        double m1 = (a[0] > a[1]) ? a[0] : a[1], m2 = (a[1] > a[2]) ? a[1] : a[2];
+        alpha = (m1 > m2) ? m1 : m2;
+      }
+      for (m = 0; m < nv; m++) fo[m] = 0.5 * (fLk[m] + fRk[m]) - alpha * (0.5 * (uRk[m] - uLk[m]));
+      continue;
+    }
+    if (llf) {
+      /* Euler1DUpwindLLF (Euler1DUpwind.c:199-266): Roe average and eigenvalues
+         of the neighbouring cells; NavierStokes2DUpwindLLF
+         (NavierStokes2DUpwind.c:190-258): of the interface states */
+      double ucL[NMAX], ucR[NMAX], fcL[NMAX], fcR[NMAX], fc[NMAX], eL[NMAX], eR[NMAX], eC[NMAX];
+      if (c->model_id == 1) {
+        roeavg_euler1d(c, ua, LP(uC, k - 1), LP(uC, k));
+        eig_euler1d(c, ua, D, L, R);
+      } else {
+        roeavg_ns2d(c, ua, uLk, uRk);
+        eig_ns2d(c, ua, dir, D, L, R);
+      }
+      matvec(nv, ucL, L, uLk); matvec(nv, ucR, L, uRk);
+      matvec(nv, fcL, L, fLk); matvec(nv, fcR, L, fRk);
+      if (c->model_id == 1) {
+        eig_euler1d(c, LP(uC, k - 1), D, NULL, NULL);
+        for (m = 0; m < nv; m++) eL[m] = D[m * nv + m];
+        eig_euler1d(c, LP(uC, k), D, NULL, NULL);
+        for (m = 0; m < nv; m++) eR[m] = D[m * nv + m];
+        eig_euler1d(c, ua, D, NULL, NULL);
+      } else {
+        eig_ns2d(c, uLk, dir, D, NULL, NULL);
+        for (m = 0; m < nv; m++) eL[m] = D[m * nv + m];
+        eig_ns2d(c, uRk, dir, D, NULL, NULL);
+        for (m = 0; m < nv; m++) eR[m] = D[m * nv + m];
+        eig_ns2d(c, ua, dir, D, NULL, NULL);
+      }
+      for (m = 0; m < nv; m++) eC[m] = D[m * nv + m];
+      for (m = 0; m < nv; m++) {
+        double aL = fabs(eL[m]), aC = fabs(eC[m]), aR = fabs(eR[m]);
+        double m1 = (aL > aC) ? aL : aC, m2 = (aC > aR) ? aC : aR;
+        double alpha = (m1 > m2) ? m1 : m2;
+        fc[m] = 0.5 * (fcL[m] + fcR[m] + alpha * (ucL[m] - ucR[m]));
+      }
+      matvec(nv, fo, R, fc);
+      continue;
+    }
     if (c->model_id == 1 && rf) {
       double ucL[NMAX], ucR[NMAX], fcL[NMAX], fcR[NMAX], fc[NMAX], Dl[NMAX * NMAX];
       roeavg_euler1d(c, ua, LP(uC, k - 1), LP(uC, k));
@@ -1218,7 +1431,7 @@ static void source(orc_case *c, const double *u, double t, double *src)
   int nv = c->nvars, nz = c->nz_layers, q, iz, d;
   size_t n = (size_t)c->npts * nv;
   memset(src, 0, n * sizeof(double));
-  if (!c->include_chem) return;
+  if (!c->include_chem) { sponge_source(c, u, src); return; }
   for (q = 0; q < c->npts_int; q++) {
     int idx[3] = {0, 0, 0}, r = q, p;
     double sigma = c->t_pulse_norm / 2.35482, I0 = 0.0;
@@ -1267,6 +1480,7 @@ static void source(orc_case *c, const double *u, double t, double *src)
                          + c->q5 * c->k5 * nO3 * n3Su
                          + c->q6 * c->k6 * n1Dg * nO3) / (c->ch_gamma - 1.0);
     }
+  sponge_source(c, u, src);
 }
 
 /* ------------------------------------------------------------------ */

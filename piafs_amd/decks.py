@@ -148,7 +148,7 @@ def ptot_for_reynolds(Re_target: float) -> float:
 def sod_1d(N: int = 1024, n_iter: int | None = None, t_final: float = 0.2, scheme: str = "weno5",
            interp: str = "characteristic", upwinding: str = "roe", rk: str = "ssprk3",
            mapped: int = 0, gamma: float = 1.4, x_from_generator: bool = False,
-           op_file_format: str = "none") -> Deck:
+           op_file_format: str = "none", boundary: list | None = None) -> Deck:
     """Sod shock tube on [0,1]: (rho,u,p) = (1,0,1) | (0.125,0,0.1), mirror
     ("extrapolate") BCs, fixed dt = t_final/n_iter with n_iter = ceil(0.88 (N-1))
     (acoustic CFL about 0.5 for max|u|+c = 2.19 of the exact solution)."""
@@ -166,7 +166,8 @@ def sod_1d(N: int = 1024, n_iter: int | None = None, t_final: float = 0.2, schem
                   dt=dt, screen_op_iter=max(n_iter, 1), file_op_iter=n_iter * 10,
                   op_file_format=op_file_format, ip_file_type="binary", op_overwrite="yes",
                   model="euler1d")
-    boundary = [("extrapolate", 0, 1, 0, 0), ("extrapolate", 0, -1, 0, 0)]
+    if boundary is None:
+        boundary = [("extrapolate", 0, 1, 0, 0), ("extrapolate", 0, -1, 0, 0)]
     physics = dict(gamma=gamma, upwinding=upwinding)
     weno = dict(mapped=mapped, borges=0, yc=0, no_limiting=0, epsilon=1e-6)
     return Deck(solver, boundary, physics, [x], u, weno)
@@ -229,6 +230,63 @@ def ns2d_photochem(nx: int = 1024, ny: int | None = None, n_iter: int = 200, dt:
         r2 = (X - 0.5 * Lx) ** 2 + (Y - 0.5 * Ly) ** 2
         w = beam_width * L
         imap = (amplitude * np.exp(-r2 / (2.0 * w * w))).reshape(-1)
+    return Deck(solver, boundary, physics, [x, y], u, weno, chem, imap)
+
+
+def ns2d_open_box(nx: int = 24, ny: int = 20, n_iter: int = 20, dt: float = 0.002, boundary: list | None = None,
+                  flow=(1.0, 0.3, 0.0, None), bump: float = 0.05, pert: float = 1e-3, seed: int = 20261204,
+                  scheme: str = "weno5", interp: str = "components", upwinding: str = "roe",
+                  Re: float | None = None, rk: str = "44", weights: str = "js", par: str = "4",
+                  include_chemistry: bool = False, L: float = 1.0) -> Deck:
+    """2D Navier-Stokes on [0, L] x [0, L ny/nx] (grid points x_i = i h,
+    h = L/(nx-1)) with open or wall boundaries: a uniform flow (rho, u, v, p) =
+    `flow` (p defaults to 1/gamma) carrying a Gaussian bump of relative size
+    `bump` in density and pressure at (0.35 L, 0.5 L ny/nx), density noise of
+    size `pert` (SplitMix64, `seed`).  `boundary` lists the zones as written
+    to boundary.inp: (type, dim, face, xmin, xmax, ymin, ymax, values...).
+    With chemistry the species start at O2 = 1, O3 = 0.01 and Re is set
+    through Ptot."""
+    h = L / (nx - 1)
+    x = np.arange(nx, dtype=np.float64) * h
+    y = np.arange(ny, dtype=np.float64) * h
+    _, _, gchem = chemistry_reynolds(101325.0)
+    g = gchem if include_chemistry else 1.4
+    nvars = 11 if include_chemistry else 4
+    rho0, u0, v0, p0 = flow
+    p0 = (1.0 / g) if p0 is None else p0
+    X, Y = np.meshgrid(x, y, indexing="xy")
+    r2 = ((X - 0.35 * L) ** 2 + (Y - 0.5 * y[-1]) ** 2) / (0.1 * L) ** 2
+    b = bump * np.exp(-r2)
+    noise = pert * (2.0 * splitmix64_fast(seed, nx * ny).reshape(ny, nx) - 1.0)
+    rho = rho0 * (1.0 + b + noise)
+    P = p0 * (1.0 + b)
+    u = np.zeros((nx * ny, nvars))
+    u[:, 0] = rho.reshape(-1)
+    u[:, 1] = (rho * u0).reshape(-1)
+    u[:, 2] = (rho * v0).reshape(-1)
+    u[:, 3] = (P / (g - 1.0) + 0.5 * rho * (u0 * u0 + v0 * v0)).reshape(-1)
+    if include_chemistry:
+        u[:, 4] = 1.0
+        u[:, 5] = 0.01
+    solver = dict(ndims=2, nvars=nvars, size=(nx, ny), ghost=3, n_iter=n_iter, time_scheme="rk",
+                  time_scheme_type=rk, hyp_space_scheme=scheme, hyp_interp_type=interp,
+                  par_space_scheme=par, dt=dt, screen_op_iter=max(n_iter, 1), file_op_iter=n_iter * 10,
+                  op_file_format="none", ip_file_type="binary", op_overwrite="yes", model="navierstokes2d")
+    if boundary is None:
+        boundary = [("extrapolate", 0, 1, 0, 0, -1e99, 1e99), ("extrapolate", 0, -1, 0, 0, -1e99, 1e99),
+                    ("extrapolate", 1, 1, -1e99, 1e99, 0, 0), ("extrapolate", 1, -1, -1e99, 1e99, 0, 0)]
+    physics = dict(gamma=1.4, upwinding=upwinding, include_chemistry="yes" if include_chemistry else "no")
+    chem, imap = None, None
+    if include_chemistry:
+        physics["Re"] = 1000.0 if Re is None else float(Re)
+        chem = dict(f_CO2=0.0, f_O3=0.01, z_mm=0.0, nz=20,
+                    Ptot=ptot_for_reynolds(Re) if Re is not None else 101325.0)
+        imap = (np.exp(-r2 / 4.0)).reshape(-1)
+    elif Re is not None:
+        physics["Re"] = float(Re)
+        physics["Pr"] = 0.72
+    weno = dict(mapped=int(weights == "mapped"), borges=int(weights == "z"), yc=int(weights == "yc"),
+                no_limiting=int(weights == "none"), epsilon=1e-6)
     return Deck(solver, boundary, physics, [x, y], u, weno, chem, imap)
 
 

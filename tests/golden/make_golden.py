@@ -68,6 +68,51 @@ CASES = {
     "riemann2d_char_rf_41_40": ("riemann_2d", dict(N=41, n_iter=40, dt=0.004)),
     "riemann2d_comp_roe_crweno_33_30": ("riemann_2d", dict(N=33, n_iter=30, dt=0.005, interp="components",
                                                            upwinding="roe", scheme="crweno5")),
+    # the remaining boundary types and the llf-char / rusanov upwinders
+    "ns2d_walls_inout_re200_24x20": ("ns2d_open_box", dict(nx=24, ny=20, n_iter=20, Re=200.0, boundary=[
+        ("subsonic-inflow", 0, 1, 0, 0, -1e99, 1e99, 1.0, 0.3, 0.0),
+        ("subsonic-outflow", 0, -1, 0, 0, -1e99, 1e99, 0.7142857142857143),
+        ("slip-wall", 1, 1, -1e99, 0.3, 0, 0, 0.0, 0.0),
+        ("noslip-wall", 1, 1, 0.3, 1e99, 0, 0, 0.0, 0.0),
+        ("noslip-wall", 1, -1, -1e99, 1e99, 0, 0, 0.1, 0.0)])),
+    "ns2d_walls_overlap_crweno_llf_20x24": ("ns2d_open_box", dict(nx=20, ny=24, n_iter=20, scheme="crweno5",
+                                                                  upwinding="llf-char", boundary=[
+        ("subsonic-inflow", 0, 1, 0, 0, -1e99, 1e99, 1.0, 0.3, 0.0),
+        ("subsonic-outflow", 0, -1, 0, 0, -1e99, 1e99, 0.7142857142857143),
+        ("slip-wall", 1, 1, -1e99, 7.0 / 19.0, 0, 0, 0.0, 0.0),
+        ("noslip-wall", 1, 1, 7.0 / 19.0, 1e99, 0, 0, 0.0, 0.0),
+        ("slip-wall", 1, -1, -1e99, 1e99, 0, 0, 0.0, 0.05)])),
+    "ns2d_chem_supersonic_jet_char_rf_16x20": ("ns2d_open_box", dict(
+        nx=16, ny=20, n_iter=15, flow=(1.0, 0.0, 1.5, 0.7), include_chemistry=True, Re=1000.0,
+        interp="characteristic", upwinding="rf-char", boundary=[
+            ("periodic", 0, 1, 0, 0, -1e99, 1e99), ("periodic", 0, -1, 0, 0, -1e99, 1e99),
+            ("supersonic-inflow", 1, 1, -1e99, 1e99, 0, 0, 1.0, 0.0, 1.5, 0.7, 1.0, 0.01, 0, 0, 0, 0, 0),
+            ("supersonic-outflow", 1, -1, -1e99, 1e99, 0, 0)])),
+    "ns2d_chem_subsonic_jet_char_llf_16x20": ("ns2d_open_box", dict(
+        nx=16, ny=20, n_iter=15, flow=(1.0, 0.0, 0.3, 0.7), include_chemistry=True, Re=1000.0,
+        interp="characteristic", upwinding="llf-char", boundary=[
+            ("periodic", 0, 1, 0, 0, -1e99, 1e99), ("periodic", 0, -1, 0, 0, -1e99, 1e99),
+            ("subsonic-inflow", 1, 1, -1e99, 1e99, 0, 0, 1.0, 0.0, 0.3, 1.0, 0.01, 0, 0, 0, 0, 0),
+            ("extrapolate", 1, -1, -1e99, 1e99, 0, 0)])),
+    "ns2d_dirichlet_ambivalent_sponge_rusanov_24x20": ("ns2d_open_box", dict(
+        nx=24, ny=20, n_iter=20, flow=(1.0, 0.3, 0.0, 0.7), upwinding="rusanov", boundary=[
+            ("dirichlet", 0, 1, 0, 0, -1e99, 1e99, 1.0, 0.3, 0.0, 0.7 / 0.4 + 0.5 * 0.09),
+            ("subsonic-ambivalent", 0, -1, 0, 0, -1e99, 1e99, 1.0, 0.3, 0.0, 0.7),
+            ("supersonic-outflow", 1, 1, -1e99, 1e99, 0, 0),
+            ("subsonic-ambivalent", 1, -1, -1e99, 1e99, 0, 0, 1.0, 0.3, 0.02, 0.7),
+            ("sponge", 0, -1, 0.7, 1.0, -1e99, 1e99, 1.0, 0.3, 0.0, 0.7 / 0.4 + 0.5 * 0.09)])),
+    "riemann2d_char_rusanov_33_30": ("riemann_2d", dict(N=33, n_iter=30, dt=0.005, upwinding="rusanov")),
+    "riemann2d_comp_llf_crweno_33_30": ("riemann_2d", dict(N=33, n_iter=30, dt=0.005, interp="components",
+                                                           upwinding="llf-char", scheme="crweno5")),
+    "sod_slipwall_char_llf_N64": ("sod_1d", dict(N=64, n_iter=20, t_final=0.07, upwinding="llf-char", boundary=[
+        ("slip-wall", 0, 1, 0, 0, 0.0), ("slip-wall", 0, -1, 0, 0, 0.0)])),
+    "sod_dirichlet_crweno_rusanov_N80": ("sod_1d", dict(N=80, n_iter=25, t_final=0.07, scheme="crweno5", interp="components",
+                                                        upwinding="rusanov", boundary=[
+        ("dirichlet", 0, 1, 0, 0, 1.0, 0.0, 2.5), ("dirichlet", 0, -1, 0, 0, 0.125, 0.0, 0.25)])),
+    "sod_sponge_comp_llf_N64": ("sod_1d", dict(N=64, n_iter=20, t_final=0.07, interp="components", upwinding="llf-char",
+                                               boundary=[("extrapolate", 0, 1, 0, 0), ("extrapolate", 0, -1, 0, 0),
+                                                         ("sponge", 0, -1, 0.8, 1.0, 0.125, 0.0, 0.25)])),
+    "sod_char_rusanov_N64": ("sod_1d", dict(N=64, n_iter=20, t_final=0.07, upwinding="rusanov")),
 }
 
 
