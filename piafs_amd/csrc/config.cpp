@@ -221,9 +221,13 @@ int Config::parse(const std::string &dir, std::string &err) {
           err = "bad boundary.inp";
           return 1;
         }
-      if (b.type == "periodic") b.bctype = BC_PERIODIC;
-      else if (b.type == "extrapolate") b.bctype = BC_EXTRAPOLATE;
-      else {
+      static const char *names[BC_NTYPES] = {"periodic", "extrapolate", "dirichlet", "reflect", "sponge",
+                                             "noslip-wall", "slip-wall", "subsonic-inflow", "subsonic-outflow",
+                                             "subsonic-ambivalent", "supersonic-inflow", "supersonic-outflow"};
+      b.bctype = -1;
+      for (int k = 0; k < BC_NTYPES; k++)
+        if (b.type == names[k]) b.bctype = k;
+      if (b.bctype < 0) {
         err = "boundary type " + b.type + " is not supported on this path";
         return 1;
       }
@@ -231,10 +235,48 @@ int Config::parse(const std::string &dir, std::string &err) {
         err = "invalid boundary dim/face";
         return 1;
       }
+      // type-specific values, read as the reference reads them (a missing
+      // value leaves zero): InitializeBoundaries.c:95-150
+      auto rd = [&](double &v) { if (!(in >> v)) { in.clear(); v = 0.0; } };
+      const int nsc = nvars - (2 + ndims);
+      switch (b.bctype) {
+        case BC_DIRICHLET:
+        case BC_SPONGE:
+          b.vals.assign(nvars, 0.0);
+          for (double &v : b.vals) rd(v);
+          break;
+        case BC_SLIP_WALL:
+        case BC_NOSLIP_WALL:
+          for (int d = 0; d < ndims; d++) rd(b.vel[d]);
+          break;
+        case BC_SUBSONIC_INFLOW:
+          rd(b.rho);
+          for (int d = 0; d < ndims; d++) rd(b.vel[d]);
+          b.vals.assign(nsc > 0 ? nsc : 0, 0.0);
+          for (double &v : b.vals) rd(v);
+          break;
+        case BC_SUBSONIC_OUTFLOW: rd(b.p); break;
+        case BC_SUBSONIC_AMBIVALENT:
+          rd(b.rho);
+          for (int d = 0; d < ndims; d++) rd(b.vel[d]);
+          rd(b.p);
+          break;
+        case BC_SUPERSONIC_INFLOW:
+          rd(b.rho);
+          for (int d = 0; d < ndims; d++) rd(b.vel[d]);
+          rd(b.p);
+          b.vals.assign(nsc > 0 ? nsc : 0, 0.0);
+          for (double &v : b.vals) rd(v);
+          break;
+        default: break;
+      }
       // CalculateLocalExtent, InitializeBoundaries.c:296-377 (one process)
       long off = 0;
       for (int d = 0; d < ndims; d++) {
-        if (d == b.dim) {
+        if (b.bctype == BC_SPONGE) {  // a sponge zone is a box of interior points
+          find_interval(b.xmin[d], b.xmax[d], &x[off + ghosts], dim[d], &b.is[d], &b.ie[d]);
+          if (b.ie[d] - b.is[d] <= 0) b.on = false;
+        } else if (d == b.dim) {
           if (b.face == 1) { b.is[d] = -ghosts; b.ie[d] = 0; }
           else { b.is[d] = dim[d]; b.ie[d] = dim[d] + ghosts; }
         } else {
@@ -359,11 +401,16 @@ int Config::parse(const std::string &dir, std::string &err) {
   }
   if (upw_choice == "roe") upwind = UPW_ROE;
   else if (upw_choice == "rf-char") upwind = UPW_RF;
+  else if (upw_choice == "llf-char") upwind = UPW_LLF;
+  else if (upw_choice == "rusanov") upwind = UPW_RUSANOV;
   else {
     err = upw_choice + " is not a valid upwinding scheme on this path";
     return 1;
   }
   include_chem = (include_chem_s == "yes");
+  // the zones' gamma: NavierStokes2DInitialize.c:184 copies the physics.inp
+  // value before chemistry replaces it; Euler1DInitialize never sets it
+  for (Boundary &b : bcs) b.gamma = (model_id == MODEL_NS2D) ? gamma : 0.0;
 
   // ---- chemistry.inp: ChemistryInitialize.c:69-481 ------------------------
   if (include_chem) {

@@ -8,9 +8,15 @@
 namespace piafs {
 
 enum Model { MODEL_EULER1D = 1, MODEL_NS2D = 2 };
-enum BCType { BC_PERIODIC = 0, BC_EXTRAPOLATE = 1 };
+// boundary.inp zone types (include/boundaryconditions.h:15-42), in the order
+// of the device zone kinds
+enum BCType {
+  BC_PERIODIC = 0, BC_EXTRAPOLATE, BC_DIRICHLET, BC_REFLECT, BC_SPONGE, BC_NOSLIP_WALL, BC_SLIP_WALL,
+  BC_SUBSONIC_INFLOW, BC_SUBSONIC_OUTFLOW, BC_SUBSONIC_AMBIVALENT, BC_SUPERSONIC_INFLOW, BC_SUPERSONIC_OUTFLOW,
+  BC_NTYPES
+};
 enum WeightKind { W_JS = 0, W_MAPPED = 1, W_Z = 2, W_YC = 3 };
-enum Upwind { UPW_ROE = 0, UPW_RF = 1 };
+enum Upwind { UPW_ROE = 0, UPW_RF = 1, UPW_LLF = 2, UPW_RUSANOV = 3 };
 
 struct Boundary {
   std::string type;
@@ -19,6 +25,10 @@ struct Boundary {
   double xmin[3] = {0, 0, 0}, xmax[3] = {0, 0, 0};
   int is[3] = {0, 0, 0}, ie[3] = {0, 0, 0};
   bool on = true;
+  // type-specific data (InitializeBoundaries.c:95-150); gamma is the
+  // physics.inp value for navierstokes2d and 0 for euler1d, which never sets it
+  double rho = 0, vel[3] = {0, 0, 0}, p = 0, gamma = 0;
+  std::vector<double> vals;  // dirichlet / sponge values (nvars), inflow passive scalars
 };
 
 // Photochemistry constants after normalisation (ChemistryInitialize.c:289-353).

@@ -92,21 +92,6 @@ __device__ __forceinline__ void crweno5_cands(double m2, double m1, double p1, d
   f3 = (s6)*m1 + (5 * s6) * p1;
 }
 
-__device__ __forceinline__ int bc_src(int i, int N, const FaceBC &lo, const FaceBC &hi, int other, bool &ok,
-                                      int &other_shift) {
-  other_shift = 0;
-  if (i >= 0 && i < N) return i;
-  const FaceBC &f = (i < 0) ? lo : hi;
-  if (f.type < 0 || other < f.lo || other >= f.hi) {
-    ok = false;
-    return 0;
-  }
-  if (f.type == 0) {  // periodic: the reference indexes the source without the zone offset
-    other_shift = -f.lo;
-    return (i < 0) ? i + N : i - N;
-  }
-  return (i < 0) ? -i - 1 : 2 * N - i - 1;
-}
 
 // Coefficients of row k of one line system: A, B, C and the right-hand side
 // (for WENO5 the right-hand side is the interface value itself).
@@ -277,10 +262,13 @@ __device__ __forceinline__ void ns2d_eigvecs(double g, const double u[4], double
 // Euler1DUpwind.c:160-173; NavierStokes2DUpwindRF, NavierStokes2DUpwind.c:152-166):
 // upwind when the field's eigenvalue has one sign at the left, averaged and
 // right states, else a local Lax-Friedrichs flux with the largest |eigenvalue|.
+// With llf set, the local Lax-Friedrichs flux is taken for every field
+// (Euler1DUpwindLLF, Euler1DUpwind.c:259-261; NavierStokes2DUpwindLLF,
+// NavierStokes2DUpwind.c:248-250).
 __device__ __forceinline__ double rf_pick(double eL, double eC, double eR, double ucL, double ucR, double fcL,
-                                          double fcR) {
-  if (eL > 0 && eC > 0 && eR > 0) return fcL;
-  if (eL < 0 && eC < 0 && eR < 0) return fcR;
+                                          double fcR, bool llf) {
+  if (!llf && eL > 0 && eC > 0 && eR > 0) return fcL;
+  if (!llf && eL < 0 && eC < 0 && eR < 0) return fcR;
   const double x1 = fabs(eL), x2 = fabs(eC), x3 = fabs(eR);
   const double mx1 = x1 > x2 ? x1 : x2, mx2 = x2 > x3 ? x2 : x3;
   const double alpha = mx1 > mx2 ? mx1 : mx2;
@@ -318,6 +306,134 @@ __device__ __forceinline__ double stage_value(const CaseDev &C, int s, long p) {
   return val;
 }
 
+// ---------------------------------------------------------------------------
+// Boundary conditions as a gather: each ghost point finds the last zone (in
+// list order) that sets its variable and evaluates that zone's rule on the
+// stage values of the interior points it reads.  The zones read interior
+// points only, so list order matters only where zones overlap.
+//  periodic BCPeriodic.c:22-62, extrapolate BCExtrapolate.c:16-51,
+//  dirichlet BCDirichlet.c:16-45, reflect BCReflect.c:17-50, walls
+//  BCSlipWall.c:24-135 / BCNoslipWall.c:22-80, BCSubsonicInflow.c:22-90,
+//  BCSubsonicOutflow.c:21-78, BCSubsonicAmbivalent.c:30-125,
+//  BCSupersonicInflow.c:26-80, BCSupersonicOutflow.c:25-80.
+// A ghost point no zone sets keeps the value the reference never changes
+// from its zero initialisation.
+// ---------------------------------------------------------------------------
+// _NavierStokes2DGetFlowVar_, navierstokes2d.h:70-79
+__device__ __forceinline__ void ns2d_flowvar(const double u[4], double gam, double &rho, double &vx, double &vy,
+                                             double &e, double &P) {
+  rho = u[0];
+  vx = (rho == 0) ? 0 : u[1] / rho;
+  vy = (rho == 0) ? 0 : u[2] / rho;
+  e = u[3];
+  const double vsq = (vx * vx) + (vy * vy);
+  P = (e - 0.5 * rho * vsq) * (gam - 1.0);
+}
+
+// does zone z set variable v (the flow BCs exist for ndims 2 only, the slip
+// wall also for ndims 1; walls and outflows leave passive scalars alone)
+__device__ __forceinline__ bool zone_sets(const Zone &z, int v, int ndims) {
+  switch (z.kind) {
+    case Z_PERIODIC: case Z_EXTRAPOLATE: case Z_DIRICHLET: case Z_REFLECT: return true;
+    case Z_SPONGE: return false;
+    case Z_SLIP_WALL: return ndims == 1 ? v < 3 : v < 4;
+    case Z_SUBSONIC_INFLOW: case Z_SUPERSONIC_INFLOW: return ndims == 2;
+    default: return ndims == 2 && v < 4;
+  }
+}
+
+
+// stage values of the four flow variables at point p
+__device__ __forceinline__ void stage_flow(const CaseDev &C, int s, long p, double u[4]) {
+#pragma unroll
+  for (int m = 0; m < 4; m++) u[m] = stage_value(C, s, (long)m * C.G.plane + p);
+}
+
+// value of variable v at ghost point (i, j) for stage s
+__device__ double ghost_value(const CaseDev &C, int s, int v, int i, int j) {
+  const Grid &G = C.G;
+  const int nd = G.ndims;
+  int d, face, gd, o;
+  if (i < 0 || i >= G.nx) {
+    d = 0; face = (i < 0) ? 1 : -1; gd = i; o = j;
+    if (nd == 2 && (j < 0 || j >= G.ny)) return 0.0;  // corner: no zone reaches it
+  } else {
+    d = 1; face = (j < 0) ? 1 : -1; gd = j; o = i;
+  }
+  const int N = d ? G.ny : G.nx, od = 1 - d;
+  int z = C.nzones - 1;
+  for (; z >= 0; z--) {
+    const Zone &Z = C.zones[z];
+    if (Z.kind == Z_SPONGE || Z.dim != d || Z.face != face || !zone_sets(Z, v, nd)) continue;
+    if (nd == 2 && (o < Z.is[od] || o >= Z.ie[od])) continue;
+    break;
+  }
+  if (z < 0) return 0.0;
+  const Zone &Z = C.zones[z];
+  auto at = [&](int a, int b) -> long {  // ghosted offset of (index along d, index across d)
+    const int ii = d ? b : a, jj = d ? a : b;
+    return (long)(jj + G.gy) * G.nxg + (ii + G.g);
+  };
+  if (Z.kind == Z_DIRICHLET) return C.bcvals[Z.vals + v];
+  if (Z.kind == Z_PERIODIC) {
+    // the source index across d is taken without the zone offset (BCPeriodic.c:44-52)
+    const int src = (face == 1) ? gd + N : gd - N;
+    return stage_value(C, s, (long)v * G.plane + at(src, nd == 2 ? o - Z.is[od] : 0));
+  }
+  const int mir = (face == 1) ? -gd - 1 : 2 * N - gd - 1;
+  const long pm = at(mir, nd == 2 ? o : 0);
+  if (Z.kind == Z_EXTRAPOLATE) return stage_value(C, s, (long)v * G.plane + pm);
+  if (Z.kind == Z_REFLECT) return (-1.0) * stage_value(C, s, (long)v * G.plane + pm);
+  const double gam = Z.gamma, igm1 = 1.0 / (gam - 1.0);
+  if (nd == 1) {  // slip wall, euler1d
+    const double rho = stage_value(C, s, pm), uvel = stage_value(C, s, G.plane + pm) / rho;
+    const double energy = stage_value(C, s, 2 * G.plane + pm);
+    const double pressure = (energy - 0.5 * rho * uvel * uvel) * (gam - 1.0);
+    const double rg = rho, pg = pressure, ug = 2.0 * Z.vel[0] - uvel;
+    if (v == 0) return rg;
+    if (v == 1) return rg * ug;
+    return igm1 * pg + 0.5 * rg * ug * ug;
+  }
+  if (v >= 4) return C.bcvals[Z.vals + v - 4];  // inflow passive scalars
+  double um[4], rho, uvel, vvel, energy, pressure, rg, ug, vg, pg;
+  if (Z.kind == Z_SUBSONIC_AMBIVALENT) {
+    const double nx = (d == 0 ? 1.0 : 0.0) * (double)face, ny = (d == 1 ? 1.0 : 0.0) * (double)face;
+    const int i1 = (face == 1) ? 0 : N - 1, i2 = (face == 1) ? 1 : N - 2;
+    double u1, v1, u2, v2;
+    stage_flow(C, s, at(i1, o), um);
+    ns2d_flowvar(um, gam, rho, u1, v1, energy, pressure);
+    stage_flow(C, s, at(i2, o), um);
+    ns2d_flowvar(um, gam, rho, u2, v2, energy, pressure);
+    const double vn = (1.5 * u1 - 0.5 * u2) * nx + (1.5 * v1 - 0.5 * v2) * ny;
+    stage_flow(C, s, pm, um);
+    ns2d_flowvar(um, gam, rho, uvel, vvel, energy, pressure);
+    if (vn > 0) { rg = Z.rho; pg = pressure; ug = Z.vel[0]; vg = Z.vel[1]; }
+    else { rg = rho; pg = Z.p; ug = uvel; vg = vvel; }
+  } else {
+    stage_flow(C, s, pm, um);
+    ns2d_flowvar(um, gam, rho, uvel, vvel, energy, pressure);
+    rg = rho; pg = pressure; ug = uvel; vg = vvel;  // supersonic outflow
+    switch (Z.kind) {
+      case Z_SLIP_WALL:
+        if (d == 0) ug = 2.0 * Z.vel[0] - uvel;
+        else vg = 2.0 * Z.vel[1] - vvel;
+        break;
+      case Z_NOSLIP_WALL:
+        ug = 2.0 * Z.vel[0] - uvel;
+        vg = 2.0 * Z.vel[1] - vvel;
+        break;
+      case Z_SUBSONIC_INFLOW: rg = Z.rho; ug = Z.vel[0]; vg = Z.vel[1]; break;
+      case Z_SUBSONIC_OUTFLOW: pg = Z.p; break;
+      case Z_SUPERSONIC_INFLOW: rg = Z.rho; pg = Z.p; ug = Z.vel[0]; vg = Z.vel[1]; break;
+      default: break;
+    }
+  }
+  if (v == 0) return rg;
+  if (v == 1) return rg * ug;
+  if (v == 2) return rg * vg;
+  return igm1 * pg + 0.5 * rg * (ug * ug + vg * vg);
+}
+
 __global__ void __launch_bounds__(256) k_stage(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
@@ -326,10 +442,8 @@ __global__ void __launch_bounds__(256) k_stage(CASE_ARGS) {
   if (G.ndims == 1) {
     if (t >= (long)G.nxg * G.nv) return;
     const int v = (int)(t / G.nxg), i = (int)(t % G.nxg) - G.g;
-    bool ok = true;
-    int sh;
-    const int si = bc_src(i, G.nx, C.bc.f[0][0], C.bc.f[0][1], 0, ok, sh);
-    C.U[(long)v * G.plane + i + G.g] = ok ? stage_value(C, s, (long)v * G.plane + si + G.g) : 0.0;
+    const long p = (long)v * G.plane + i + G.g;
+    C.U[p] = (i >= 0 && i < G.nx) ? stage_value(C, s, p) : ghost_value(C, s, v, i, 0);
     return;
   }
   // 2D: 32x32 tiles, 256 threads; natural write, then transposed write via LDS
@@ -340,26 +454,27 @@ __global__ void __launch_bounds__(256) k_stage(CASE_ARGS) {
   const int i0 = (rem % ntx) * 32 - G.g, j0 = (rem / ntx) * 32 - G.gy;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const long pl = (long)v * G.plane;
+  // interior points first; the ghost points of tiles that reach past the
+  // interior afterwards, so that the boundary rules stay out of this loop
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + tx, j = j0 + r;
     double val = 0.0;
-    if (i < G.nx + G.g && j < G.ny + G.gy) {
-      bool ok = true;
-      int sx, sy;
-      int si = bc_src(i, G.nx, C.bc.f[0][0], C.bc.f[0][1], j, ok, sx);
-      int sj = j;
-      if (ok) {
-        if (i >= 0 && i < G.nx) {
-          sj = bc_src(j, G.ny, C.bc.f[1][0], C.bc.f[1][1], i, ok, sy);
-          si = i + sy;
-        } else {
-          sj = j + sx;
-        }
-      }
-      if (ok) val = stage_value(C, s, pl + (long)(sj + G.gy) * G.nxg + (si + G.g));
-      C.U[pl + (long)(j + G.gy) * G.nxg + (i + G.g)] = val;
+    if (i >= 0 && i < G.nx && j >= 0 && j < G.ny) {
+      const long p = pl + (long)(j + G.gy) * G.nxg + (i + G.g);
+      val = stage_value(C, s, p);
+      C.U[p] = val;
     }
     tile[r][tx] = val;
+  }
+  if (i0 < 0 || j0 < 0 || i0 + 32 > G.nx || j0 + 32 > G.ny) {
+    for (int r = ty; r < 32; r += 8) {
+      const int i = i0 + tx, j = j0 + r;
+      if (i < G.nx + G.g && j < G.ny + G.gy && (i < 0 || i >= G.nx || j < 0 || j >= G.ny)) {
+        const double val = ghost_value(C, s, v, i, j);
+        C.U[pl + (long)(j + G.gy) * G.nxg + (i + G.g)] = val;
+        tile[r][tx] = val;
+      }
+    }
   }
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
@@ -1140,8 +1255,11 @@ __device__ __forceinline__ void upwind_ns2d_roe(const SweepGeom &S, int nv, doub
   for (int m = 4; m < nv; m++) fI[m * vstride + ii] = FS(m) - Dsp * UD(m);
 }
 
+// mode: 0 Roe (Euler1DUpwind.c:27-93), 1 rf-char (:107-190), 2 llf-char
+// (:199-266), 3 rusanov (:282-353; its wave speed is gamma P/rho + |u|, with
+// no square root, as the reference computes it)
 template <int PAIR>
-__device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g, int rf, long t,
+__device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g, int mode, long t,
                                            const double *__restrict__ q, double *const *io, double *__restrict__ fI) {
   const double *uL = io[0], *uR = io[1], *fL = io[2], *fR = io[3];
   const int nl = S.nlines, N = S.N;
@@ -1150,10 +1268,22 @@ __device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g,
   const long cL = S.base + (long)l * S.line_stride + (k - 1), cR = cL + 1;
   double a0, a1, a2;
   euler1d_roeavg(g, q[cL], q[pl + cL], q[2 * pl + cL], q[cR], q[pl + cR], q[2 * pl + cR], a0, a1, a2);
+  const long ii = (long)l * (N + 1) + k, vs = (long)nl * (N + 1), vstride = vs;
+  if (mode == 3) {
+    auto speed = [&](double rho, double mom, double e) {
+      const double uvel = mom / rho, P = (e - 0.5 * rho * uvel * uvel) * (g - 1.0);
+      return g * P / rho + fabs(uvel);
+    };
+    const double aL = speed(q[cL], q[pl + cL], q[2 * pl + cL]), aR = speed(q[cR], q[pl + cR], q[2 * pl + cR]);
+    const double aA = speed(a0, a1, a2);
+    const double m1 = aL > aR ? aL : aR, m2 = aR > aA ? aR : aA;  // max3, math_ops.h:30
+    const double alpha = m1 > m2 ? m1 : m2;
+    for (int m = 0; m < nv; m++) fI[m * vs + ii] = FS(m) - alpha * UD(m);
+    return;
+  }
   Eig3 E;
   euler1d_eig(g, a0, a1, a2, E);
-  const long ii = (long)l * (N + 1) + k, vs = (long)nl * (N + 1), vstride = vs;
-  if (!rf) {
+  if (mode == 0) {
     double ud[3], DL[3][3];
     for (int m = 0; m < 3; m++) ud[m] = UD(m);
     for (int i = 0; i < 3; i++)
@@ -1183,13 +1313,14 @@ __device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g,
       fcL += E.L[m][j] * fL[j * vs + ii];
       fcR += E.L[m][j] * fR[j * vs + ii];
     }
-    fc[m] = rf_pick(EL.D[m], E.D[m], ER.D[m], ucL, ucR, fcL, fcR);
+    fc[m] = rf_pick(EL.D[m], E.D[m], ER.D[m], ucL, ucR, fcL, fcR, mode == 2);
   }
   for (int i = 0; i < 3; i++) fI[i * vs + ii] = E.R[i][0] * fc[0] + E.R[i][1] * fc[1] + E.R[i][2] * fc[2];
   // species rows: identity eigenvectors, eigenvalue v
   const double eLs = EL.D[0], eCs = E.D[0], eRs = ER.D[0];
   for (int m = 3; m < nv; m++)
-    fI[m * vs + ii] = rf_pick(eLs, eCs, eRs, uL[m * vs + ii], uR[m * vs + ii], fL[m * vs + ii], fR[m * vs + ii]);
+    fI[m * vs + ii] =
+        rf_pick(eLs, eCs, eRs, uL[m * vs + ii], uR[m * vs + ii], fL[m * vs + ii], fR[m * vs + ii], mode == 2);
 }
 
 // NavierStokes2DUpwindRF (NavierStokes2DUpwind.c:106-187): L and R from the
@@ -1197,7 +1328,7 @@ __device__ __forceinline__ void upwind_e1d(const SweepGeom &S, int nv, double g,
 // neighbouring cells); a field is upwinded by the signs of its eigenvalue at
 // uL, uR and the average.  Species fields have eigenvalue vn and identity
 // eigenvectors.  Split mode only: uL, uR, fL, fR in io[0..3].
-template <int DIR>
+template <int DIR, bool LLF>
 __device__ __forceinline__ void upwind_ns2d_rf(const SweepGeom &S, int nv, double g, long t, double *const *io,
                                                double *__restrict__ fI) {
   const int nl = S.nlines, N = S.N;
@@ -1222,12 +1353,38 @@ __device__ __forceinline__ void upwind_ns2d_rf(const SweepGeom &S, int nv, doubl
     const double ucR = L[m][0] * uRk[0] + L[m][1] * uRk[1] + L[m][2] * uRk[2] + L[m][3] * uRk[3];
     const double fcL = L[m][0] * fLk[0] + L[m][1] * fLk[1] + L[m][2] * fLk[2] + L[m][3] * fLk[3];
     const double fcR = L[m][0] * fRk[0] + L[m][1] * fRk[1] + L[m][2] * fRk[2] + L[m][3] * fRk[3];
-    fc[m] = rf_pick(eL[m], eC[m], eR[m], ucL, ucR, fcL, fcR);
+    fc[m] = rf_pick(eL[m], eC[m], eR[m], ucL, ucR, fcL, fcR, LLF);
   }
 #pragma unroll
   for (int i = 0; i < 4; i++) fI[i * vs + ii] = R[i][0] * fc[0] + R[i][1] * fc[1] + R[i][2] * fc[2] + R[i][3] * fc[3];
   for (int m = 4; m < nv; m++)
-    fI[m * vs + ii] = rf_pick(eL[3], eC[3], eR[3], uL[m * vs], uR[m * vs], fL[m * vs], fR[m * vs]);
+    fI[m * vs + ii] = rf_pick(eL[3], eC[3], eR[3], uL[m * vs], uR[m * vs], fL[m * vs], fR[m * vs], LLF);
+}
+
+// NavierStokes2DUpwindRusanov (NavierStokes2DUpwind.c:271-332): the largest of
+// c + |v_n| at the two neighbouring cells and their Roe average.
+template <int DIR, int PAIR>
+__device__ __forceinline__ void upwind_ns2d_rusanov(const SweepGeom &S, int nv, double g, long t,
+                                                    const double *__restrict__ q, double *const *io,
+                                                    double *__restrict__ fI) {
+  const int nl = S.nlines, N = S.N;
+  const int k = (int)(t % (N + 1)), l = (int)(t / (N + 1));
+  const long pl = S.plane;
+  const long cL = S.base + (long)l * S.line_stride + (k - 1), cR = cL + 1;
+  const double uLc[4] = {q[cL], q[pl + cL], q[2 * pl + cL], q[3 * pl + cL]};
+  const double uRc[4] = {q[cR], q[pl + cR], q[2 * pl + cR], q[3 * pl + cR]};
+  double ua[4];
+  ns2d_roeavg(g, uLc, uRc, ua);
+  auto speed = [&](const double u[4]) {
+    double rho, vx, vy, e, P;
+    ns2d_flowvar(u, g, rho, vx, vy, e, P);
+    return sqrt(g * P / rho) + fabs(DIR == 0 ? vx : vy);
+  };
+  const double aL = speed(uLc), aR = speed(uRc), aA = speed(ua);
+  const double m1 = aL > aR ? aL : aR, m2 = aR > aA ? aR : aA;  // max3, math_ops.h:30
+  const double alpha = m1 > m2 ? m1 : m2;
+  const long ii = (long)l * (N + 1) + k, vstride = (long)nl * (N + 1);
+  for (int m = 0; m < nv; m++) fI[m * vstride + ii] = FS(m) - alpha * UD(m);
 }
 
 #undef UD
@@ -1242,9 +1399,23 @@ __global__ void __launch_bounds__(256) k_upwind(CASE_ARGS) {
   if (C.model == 1) {
     if (C.paired) upwind_e1d<1>(C.Sx, C.G.nv, C.P.gamma, C.upwind, t, C.U, C.ix, C.fIx);
     else upwind_e1d<0>(C.Sx, C.G.nv, C.P.gamma, C.upwind, t, C.U, C.ix, C.fIx);
-  } else if (C.upwind == 1) {  // rf-char
-    if (t < nX) upwind_ns2d_rf<0>(C.Sx, C.G.nv, C.P.gamma, t, C.ix, C.fIx);
-    else upwind_ns2d_rf<1>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.iy, C.fIy);
+  } else if (C.upwind == 1 || C.upwind == 2) {  // rf-char, llf-char
+    const bool llf = C.upwind == 2;
+    if (t < nX) {
+      if (llf) upwind_ns2d_rf<0, true>(C.Sx, C.G.nv, C.P.gamma, t, C.ix, C.fIx);
+      else upwind_ns2d_rf<0, false>(C.Sx, C.G.nv, C.P.gamma, t, C.ix, C.fIx);
+    } else {
+      if (llf) upwind_ns2d_rf<1, true>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.iy, C.fIy);
+      else upwind_ns2d_rf<1, false>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.iy, C.fIy);
+    }
+  } else if (C.upwind == 3) {  // rusanov
+    if (t < nX) {
+      if (C.paired) upwind_ns2d_rusanov<0, 1>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
+      else upwind_ns2d_rusanov<0, 0>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
+    } else {
+      if (C.paired) upwind_ns2d_rusanov<1, 1>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.UT, C.iy, C.fIy);
+      else upwind_ns2d_rusanov<1, 0>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.UT, C.iy, C.fIy);
+    }
   } else if (t < nX) {
     if (C.paired) upwind_ns2d_roe<0, 1>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
     else upwind_ns2d_roe<0, 0>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
@@ -1424,7 +1595,31 @@ __device__ __forceinline__ void viscous_par(const CaseDev &C, int i, int j, long
   }
 }
 
-__device__ __forceinline__ void chem_source(const CaseDev &C, int stage, long p, double *udot) {
+// Sponge zones act as a source (BCSponge.c:29-74, called by
+// SourceFunction.c:44-58 after the physics source): zone by zone,
+// s - sigma (U - uref) with sigma linear in the coordinate along the zone's dim.
+__device__ __forceinline__ bool in_sponge(const CaseDev &C, int i, int j) {
+  for (int z = 0; z < C.nzones; z++) {
+    const Zone &Z = C.zones[z];
+    if (Z.kind == Z_SPONGE && i >= Z.is[0] && i < Z.ie[0] && j >= Z.is[1] && j < Z.ie[1]) return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ double sponge(const CaseDev &C, int i, int j, int v, long p, double s) {
+  const Grid &G = C.G;
+  for (int z = 0; z < C.nzones; z++) {
+    const Zone &Z = C.zones[z];
+    if (Z.kind != Z_SPONGE || i < Z.is[0] || i >= Z.ie[0] || j < Z.is[1] || j >= Z.ie[1]) continue;
+    const double x = Z.dim ? C.xc[(G.nx + 2 * G.g) + G.g + j] : C.xc[G.g + i];
+    const double sigma = (Z.face > 0) ? (x - Z.xmin) / (Z.xmax - Z.xmin) : (x - Z.xmax) / (Z.xmin - Z.xmax);
+    s -= (sigma * (C.U[(long)v * G.plane + p] - C.bcvals[Z.vals + v]));
+  }
+  return s;
+}
+
+__device__ __forceinline__ void chem_source(const CaseDev &C, int stage, long p, double *udot, bool sp, int i,
+                                            int j) {
   const Grid &G = C.G;
   const PhysParams &P = C.P;
   const double *U = C.U;
@@ -1457,7 +1652,7 @@ __device__ __forceinline__ void chem_source(const CaseDev &C, int stage, long p,
     s[6] = 0.0;
     for (int m = 0; m < 7; m++) {
       double *o = udot + (long)(nfv + m) * pl + p;
-      *o = *o + s[m];
+      *o = *o + (sp ? sponge(C, i, j, nfv + m, p, s[m]) : s[m]);
     }
     src_e = ((P.q0a * P.k0a + P.q0b * P.k0b) * nh * nO3 + (P.q1a * P.k1a + P.q1b * P.k1b) * nO2 * n1D +
              (P.q2a * P.k2a + P.q2b * P.k2b) * nO3 * n1D + (P.q3a * P.k3a + P.q3b * P.k3b) * nO3 * n1Sg +
@@ -1465,7 +1660,20 @@ __device__ __forceinline__ void chem_source(const CaseDev &C, int stage, long p,
             P.ch_gm1;
   }
   double *o = udot + (long)(P.nflow - 1) * pl + p;
-  *o = *o + src_e;
+  *o = *o + (sp ? sponge(C, i, j, P.nflow - 1, p, src_e) : src_e);
+}
+
+// The source term at interior point (i, j), ghosted offset p: chemistry,
+// then sponge zones (SourceFunction.c:21-60); rhs += source.
+__device__ __forceinline__ void add_source(const CaseDev &C, int stage, int i, int j, long p, double *udot) {
+  const bool sp = C.sponge && in_sponge(C, i, j);
+  if (C.P.chem) chem_source(C, stage, p, udot, sp, i, j);
+  if (!sp) return;
+  const int nplain = C.P.chem ? C.P.nflow - 1 : C.G.nv;  // variables chemistry leaves at zero
+  for (int v = 0; v < nplain; v++) {
+    double *o = udot + (long)v * C.G.plane + p;
+    *o = *o + sponge(C, i, j, v, p, 0.0);
+  }
 }
 
 __global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
@@ -1484,7 +1692,7 @@ __global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
       const double hyp = dxi * (C.fIx[a + 1] - C.fIx[a]);
       udot[(long)v * G.plane + p] = -hyp;
     }
-    if (C.P.chem) chem_source(C, stage, p, udot);
+    add_source(C, stage, i, 0, p, udot);
     return;
   }
   // 2D: a 32 x 32 tile of points, four per thread
@@ -1530,11 +1738,11 @@ __global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
     }
     __syncthreads();
   }
-  if (C.P.chem) {
+  if (C.P.chem || C.sponge) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int j = j0 + ty + 8 * q;
-      if (i < G.nx && j < G.ny) chem_source(C, stage, (long)(j + G.gy) * G.nxg + (i + G.g), udot);
+      if (i < G.nx && j < G.ny) add_source(C, stage, i, j, (long)(j + G.gy) * G.nxg + (i + G.g), udot);
     }
   }
 }

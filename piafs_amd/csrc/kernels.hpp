@@ -51,13 +51,21 @@ constexpr int kMaxRows = PIAFS_MAX_ROWS;
 // false when a line is too long for one block.
 bool team_shape(int N, int max_rows, int &team, int &rows);
 
-// Boundary map for one face: type (0 periodic, 1 extrapolate, -1 none) and the
-// covered interior range [lo, hi) in the other dimension.
-struct FaceBC {
-  int type, lo, hi;
+// Boundary zones of boundary.inp, applied in list order: a later zone
+// overwrites an earlier one where both set a variable at a ghost point.
+enum ZoneKind {
+  Z_PERIODIC = 0, Z_EXTRAPOLATE, Z_DIRICHLET, Z_REFLECT, Z_SPONGE, Z_NOSLIP_WALL, Z_SLIP_WALL,
+  Z_SUBSONIC_INFLOW, Z_SUBSONIC_OUTFLOW, Z_SUBSONIC_AMBIVALENT, Z_SUPERSONIC_INFLOW, Z_SUPERSONIC_OUTFLOW
 };
-struct BCMap {
-  FaceBC f[2][2];  // [dim][face: 0 = left (face +1), 1 = right (face -1)]
+constexpr int kMaxZones = 16;
+struct Zone {
+  int kind, dim, face;  // face +1: the low end of dim, -1: the high end (boundary.inp)
+  int is[2], ie[2];     // index box [is, ie) per dimension: the ghost layer along dim and the
+                        // covered interior range across it; a sponge: a box of interior points
+  int vals;             // offset of the zone's values in CaseDev::bcvals (dirichlet / sponge:
+                        // nvars values; inflow: the passive scalars), -1 if none
+  double rho, vel[2], p, gamma;
+  double xmin, xmax;    // sponge: the zone's extent along dim
 };
 
 struct PhysParams {
@@ -84,15 +92,18 @@ struct ReconParams {
 struct CaseDev {
   Grid G;
   SweepGeom Sx, Sy;
-  BCMap bc;
+  int nzones, sponge;  // sponge: the case has a sponge zone
+  Zone zones[kMaxZones];
   PhysParams P;
   ReconParams R;
-  int model, upwind, nstages;  // upwind: 0 Roe, 1 rf-char
+  int model, upwind, nstages;  // upwind: 0 Roe, 1 rf-char, 2 llf-char, 3 rusanov
   int paired;    // reconstruction writes 0.5*(uR-uL), 0.5*(fL+fR) only (Roe with component-wise interpolation)
   double dt, A[16], b[4], c[4];
   double *u, *U, *UT, *Udot[4];
   double *flux_x, *flux_y, *ix[4], *iy[4], *fIx, *fIy;
   double *Qd, *FV, *dxinv, *imap;
+  double *xc;      // grid coordinates with ghosts, one dimension after the other
+  double *bcvals;  // the zones' values (Zone::vals)
   double *ends;  // paired mode: raw end-interface values of both systems of every pair [dir][pair][side][0|N]
   double *time;  // current simulation time of the case (one double)
   int *status;   // bit 0: a zero pivot met in a line solve (tridiagLU.c:139-142 returns -1)

@@ -62,10 +62,12 @@ double *Case::dalloc(size_t n, hipStream_t st) {
 }
 
 // Paired reconstruction (two interface arrays per sweep instead of four):
-// component-wise interpolation with Roe upwinding, which reads only
-// 0.5*(uR - uL) and 0.5*(fL + fR).  Characteristic interpolation and rf-char
-// upwinding need uL, uR, fL, fR separately.
-static bool paired_mode(const Config &c) { return !c.characteristic && c.upwind == UPW_ROE; }
+// component-wise interpolation with Roe or Rusanov upwinding, which read only
+// 0.5*(uR - uL) and 0.5*(fL + fR).  Characteristic interpolation and the
+// rf-char / llf-char upwinders need uL, uR, fL, fR separately.
+static bool paired_mode(const Config &c) {
+  return !c.characteristic && (c.upwind == UPW_ROE || c.upwind == UPW_RUSANOV);
+}
 
 size_t Case::plan_bytes(const Config &c) {
   Grid g = make_grid(c);
@@ -103,22 +105,34 @@ int Case::setup(int dev, hipStream_t st) {
     D.Sy.nlines = G.nx; D.Sy.base = (long)G.g * G.nyg + G.gy; D.Sy.line_stride = G.nyg;
   }
   if (set_team_rows(kMaxRows)) return 1;
-  // boundary map: one zone per face (a later zone on the same face replaces an earlier one)
-  for (int d = 0; d < 2; d++)
-    for (int f = 0; f < 2; f++) D.bc.f[d][f] = FaceBC{-1, 0, 0};
+  // boundary zones, in list order (ApplyBoundaryConditions.c:20-40)
+  std::vector<double> zvals;
+  D.nzones = 0;
+  D.sponge = 0;
   for (const Boundary &b : cfg.bcs) {
     if (!b.on) continue;
-    FaceBC fb;
-    fb.type = (b.bctype == BC_PERIODIC) ? 0 : 1;
-    if (cfg.ndims == 2) {
-      int o = 1 - b.dim;
-      fb.lo = b.is[o];
-      fb.hi = b.ie[o];
-    } else {
-      fb.lo = 0;
-      fb.hi = 1;
+    if (D.nzones == kMaxZones) return fail(1, "more than " + std::to_string(kMaxZones) + " boundary zones");
+    Zone &z = D.zones[D.nzones++];
+    z.kind = b.bctype;
+    z.dim = b.dim;
+    z.face = b.face;
+    for (int d = 0; d < 2; d++) {
+      z.is[d] = d < cfg.ndims ? b.is[d] : 0;
+      z.ie[d] = d < cfg.ndims ? b.ie[d] : 1;
     }
-    D.bc.f[b.dim][b.face == 1 ? 0 : 1] = fb;
+    z.rho = b.rho;
+    z.vel[0] = b.vel[0];
+    z.vel[1] = b.vel[1];
+    z.p = b.p;
+    z.gamma = b.gamma;
+    z.xmin = b.xmin[b.dim];
+    z.xmax = b.xmax[b.dim];
+    z.vals = -1;
+    if (!b.vals.empty()) {
+      z.vals = (int)zvals.size();
+      zvals.insert(zvals.end(), b.vals.begin(), b.vals.end());
+    }
+    if (b.bctype == BC_SPONGE) D.sponge = 1;
   }
   PhysParams &P = D.P;
   P.gamma = cfg.gamma;
@@ -188,6 +202,8 @@ int Case::setup(int dev, hipStream_t st) {
     D.FV = dalloc(6 * (size_t)G.nxg * G.nyg, st);
   }
   D.dxinv = dalloc(cfg.dxinv.size(), st);
+  D.xc = dalloc(cfg.x.size(), st);
+  D.bcvals = dalloc(zvals.size() + 1, st);
   D.imap = dalloc(G.plane, st);
   D.time = dalloc(1, st);
   if (D.paired) {
@@ -202,7 +218,11 @@ int Case::setup(int dev, hipStream_t st) {
   if (!D.u || !D.U || !D.flux_x || !D.dxinv || !D.imap || !D.time || !partials || !uprev ||
       (two && (!D.UT || !D.flux_y || !D.fIy)))
     return fail(-2, "device allocation failed");
+  if (!D.xc || !D.bcvals) return fail(-2, "device allocation failed");
   HIPCHK(hipMemcpyAsync(D.dxinv, cfg.dxinv.data(), cfg.dxinv.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(D.xc, cfg.x.data(), cfg.x.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  if (!zvals.empty())
+    HIPCHK(hipMemcpyAsync(D.bcvals, zvals.data(), zvals.size() * sizeof(double), hipMemcpyHostToDevice, st));
   if (cfg.include_chem) {
     std::vector<double> im(G.plane, 0.0);
     for (long p = 0; p < (long)cfg.imap.size(); p++) im[p] = cfg.imap[p];

@@ -61,11 +61,11 @@ def test_reference_example_decks_parse(example):
     (lambda dk: dk.solver.__setitem__("hyp_space_scheme", "muscl3"), "not a supported spatial interpolation"),
     (lambda dk: dk.solver.__setitem__("nvars", 7), "nvars has to be 11"),
     (lambda dk: dk.solver.__setitem__("time_scheme_type", "rk5"), "not a supported multi-stage"),
-    (lambda dk: dk.physics.__setitem__("upwinding", "llf-char"), "not a valid upwinding"),
+    (lambda dk: dk.physics.__setitem__("upwinding", "hllc"), "not a valid upwinding"),
     (lambda dk: dk.solver.__setitem__("par_space_scheme", "3"), "parabolic"),
     (lambda dk: (dk.solver.__setitem__("hyp_space_scheme", "crweno5"),
                  dk.solver.__setitem__("hyp_interp_type", "characteristic")), "supported for weno5"),
-    (lambda dk: dk.boundary.__setitem__(0, ("slip-wall", 0, 1, 0, 0, 0, 0)), "not supported"),
+    (lambda dk: dk.boundary.__setitem__(0, ("thermal-wall", 0, 1, 0, 0, 0, 0)), "not supported"),
 ])
 def test_deck_errors_are_reported(mutate, msg):
     dk = decks.ns2d_photochem(nx=16, ny=16, n_iter=1)

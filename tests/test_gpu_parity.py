@@ -87,7 +87,11 @@ FIXTURES = [
     "ns2d_nochem_mapped_20x16", "ns2d_chem_weno_mixed_cfg3_17x31", "ns2d_chem_crweno_48x40",
     "sod_char_roe_ssprk3_N1024", "ns2d_chem_char_rf_24x20", "ns2d_chem_char_roe_mapped_20x16",
     "ns2d_chem_comp_rf_crweno_20x24", "ns2d_nochem_char_rf_z_16x20", "riemann2d_char_rf_41_40",
-    "riemann2d_comp_roe_crweno_33_30",
+    "riemann2d_comp_roe_crweno_33_30", "ns2d_walls_inout_re200_24x20", "ns2d_walls_overlap_crweno_llf_20x24",
+    "ns2d_chem_supersonic_jet_char_rf_16x20", "ns2d_chem_subsonic_jet_char_llf_16x20",
+    "ns2d_dirichlet_ambivalent_sponge_rusanov_24x20", "riemann2d_char_rusanov_33_30",
+    "riemann2d_comp_llf_crweno_33_30", "sod_slipwall_char_llf_N64", "sod_dirichlet_crweno_rusanov_N80",
+    "sod_sponge_comp_llf_N64", "sod_char_rusanov_N64",
 ]
 
 
