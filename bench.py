@@ -305,7 +305,7 @@ def main():
         if ktimes:
             # per-class average launch time (ms) x launches per step -> share of the step
             ns = inf.nstages
-            per_step = {"stage_bc": ns, "flux": ns, "recon_ends": ns, "recon": ns, "recon_char": ns, "upwind": ns, "qderiv": ns,
+            per_step = {"stage_bc": ns, "ghost": ns, "bflux": ns, "flux": ns, "recon_ends": ns, "recon": ns, "recon_char": ns, "upwind": ns, "qderiv": ns,
                         "viscflux": ns, "assemble": ns, "update": 1}
             share = {k: v * per_step.get(k, 1) for k, v in ktimes.items()}
             dom = max(share.items(), key=lambda kv: kv[1])[0]

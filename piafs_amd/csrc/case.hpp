@@ -14,6 +14,7 @@
 namespace piafs {
 
 Grid make_grid(const Config &c);
+std::string output_extension(const Config &c);  // ".bin", ".dat" or "" (op_file_format none)
 
 // One PIAFS simulation: parsed configuration, device buffers and the
 // descriptor the kernels read.
@@ -38,6 +39,15 @@ struct Case {
   int upload(const double *uh, hipStream_t st);
   int download(double *uh, hipStream_t st, const double *src = nullptr);
   int write_binary(const std::string &path, hipStream_t st);
+  // run outputs (output.cpp): the solution in the deck's op_file_format, the
+  // species files, the exact-solution errors and errors.dat & co.
+  int write_solution(const std::string &path, hipStream_t st);
+  int write_species(const std::string &dir, const std::string &suffix, double stage_time, hipStream_t st);
+  int exact_errors(const std::string &dir, double err3[3], hipStream_t st);
+  int write_run_files(const std::string &dir, const double err3[3], long steps, double solver_rt, double main_rt,
+                      hipStream_t st);
+  // conservation errors per variable (CalculateConservationError.c:20-40); -1 when unchecked
+  int conservation_errors(std::vector<double> &ce, hipStream_t st);
   // lanes per line system: segments of at most max_rows rows (descriptor only;
   // the engine's plan must be rebuilt afterwards)
   int set_team_rows(int max_rows);

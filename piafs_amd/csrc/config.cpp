@@ -108,6 +108,10 @@ static int read_array(const Config &c, const std::string &dir, const char *root,
   return 0;
 }
 
+int read_exact(const Config &c, const std::string &dir, std::vector<double> &u, bool &found, std::string &err) {
+  return read_array(c, dir, "exact", c.nvars, nullptr, u, found, err);
+}
+
 int Config::parse(const std::string &dir, std::string &err) {
   // ---- solver.inp: ReadInputs.c:200-480 -----------------------------------
   {
@@ -121,6 +125,7 @@ int Config::parse(const std::string &dir, std::string &err) {
       if (w == "ndims") s >> ndims;
       else if (w == "nvars") s >> nvars;
       else if (w == "size") { for (int d = 0; d < ndims && d < 3; d++) s >> dim[d]; }
+      else if (w == "size_exact") { for (int d = 0; d < ndims && d < 3; d++) s >> dim_exact[d]; }
       else if (w == "ghost") s >> ghosts;
       else if (w == "n_iter") s >> n_iter;
       else if (w == "restart_iter") s >> restart_iter;
@@ -147,6 +152,13 @@ int Config::parse(const std::string &dir, std::string &err) {
     int r = read_keywords(fn, on_key, err);
     if (r != 2) {
       if (err.empty()) err = "illegal format in solver.inp";
+      return 1;
+    }
+    for (int d = 0; d < 3; d++)
+      if (dim_exact[d] <= 0) dim_exact[d] = dim[d];  // ReadInputs.c:283-288: defaults to size
+    if (op_file_format != "text" && op_file_format != "tecplot2d" && op_file_format != "binary" &&
+        op_file_format != "bin" && op_file_format != "none") {  // InitializeSolvers.c:224-239
+      err = op_file_format + " is not a supported file format";
       return 1;
     }
     if (screen_op_iter <= 0) screen_op_iter = 1;
@@ -408,6 +420,9 @@ int Config::parse(const std::string &dir, std::string &err) {
     return 1;
   }
   include_chem = (include_chem_s == "yes");
+  // PhysicsOutput: Euler1DInitialize.c:145 always, NavierStokes2DInitialize.c:189
+  // when write_output is "yes"
+  species_output = include_chem && (model_id == MODEL_EULER1D || write_op == "yes");
   // the zones' gamma: NavierStokes2DInitialize.c:184 copies the physics.inp
   // value before chemistry replaces it; Euler1DInitialize never sets it
   for (Boundary &b : bcs) b.gamma = (model_id == MODEL_NS2D) ? gamma : 0.0;
@@ -438,6 +453,7 @@ int Config::parse(const std::string &dir, std::string &err) {
       return false;
     };
     if (read_keywords(join(dir, "chemistry.inp"), on_key, err) == 1) return 1;
+    species_all_zlocs = (wz != "no");  // ChemistryWriteSpecies.c:34: only "no" restricts it
     if (f_CO2 > 1.0 || f_CO2 < 0.0 || f_O3 > 1.0 || f_O3 < 0.0) {
       err = "f_CO2 / f_O3 must be between 0 and 1";
       return 1;

@@ -45,7 +45,7 @@ struct Chem {
 struct Config {
   // solver.inp (ReadInputs.c:200-480)
   int ndims = 1, nvars = 1, ghosts = 1, n_iter = 0, restart_iter = 0;
-  int dim[3] = {0, 0, 0};
+  int dim[3] = {0, 0, 0}, dim_exact[3] = {0, 0, 0};
   int screen_op_iter = 1, file_op_iter = 1000;
   double dt = 0.0;
   std::string time_scheme = "euler", time_scheme_type = " ", hyp_scheme = "1", par_scheme = "2";
@@ -60,6 +60,7 @@ struct Config {
   std::string upw_choice = "roe";
   int upwind = UPW_ROE;
   bool include_chem = false;
+  bool species_output = false, species_all_zlocs = true;  // op_species_z* files
   int nflow = 0;
   // weno.inp
   int mapped = 0, borges = 0, yc = 0, no_limiting = 0;
@@ -78,5 +79,9 @@ struct Config {
 
   int parse(const std::string &dir, std::string &err);
 };
+
+// exact.inp (ExactSolution.c:17-60 with ReadArray): the interior solution,
+// point-major; found is false when the file is absent
+int read_exact(const Config &c, const std::string &dir, std::vector<double> &u, bool &found, std::string &err);
 
 }  // namespace piafs

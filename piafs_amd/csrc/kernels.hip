@@ -440,47 +440,70 @@ __global__ void __launch_bounds__(256) k_stage(CASE_ARGS) {
   const Grid &G = C.G;
   const int s = stage;
   if (G.ndims == 1) {
-    if (t >= (long)G.nxg * G.nv) return;
-    const int v = (int)(t / G.nxg), i = (int)(t % G.nxg) - G.g;
+    if (t >= (long)G.nx * G.nv) return;
+    const int v = (int)(t / G.nx), i = (int)(t % G.nx);
     const long p = (long)v * G.plane + i + G.g;
-    C.U[p] = (i >= 0 && i < G.nx) ? stage_value(C, s, p) : ghost_value(C, s, v, i, 0);
+    C.U[p] = stage_value(C, s, p);
     return;
   }
-  // 2D: 32x32 tiles, 256 threads; natural write, then transposed write via LDS
+  // 2D interior: 32x32 tiles, 256 threads; natural write, then transposed
+  // write via LDS.  The ghost layer is k_ghost's.
   __shared__ double tile[32][33];
-  const int ntx = (G.nxg + 31) / 32, nty = (G.nyg + 31) / 32;
+  const int ntx = (G.nx + 31) / 32, nty = (G.ny + 31) / 32;
   const int id = (int)(blockIdx.x - firstblock[cidx]);
   const int v = id / (ntx * nty), rem = id % (ntx * nty);
-  const int i0 = (rem % ntx) * 32 - G.g, j0 = (rem / ntx) * 32 - G.gy;
+  const int i0 = (rem % ntx) * 32, j0 = (rem / ntx) * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const long pl = (long)v * G.plane;
-  // interior points first; the ghost points of tiles that reach past the
-  // interior afterwards, so that the boundary rules stay out of this loop
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + tx, j = j0 + r;
     double val = 0.0;
-    if (i >= 0 && i < G.nx && j >= 0 && j < G.ny) {
+    if (i < G.nx && j < G.ny) {
       const long p = pl + (long)(j + G.gy) * G.nxg + (i + G.g);
       val = stage_value(C, s, p);
       C.U[p] = val;
     }
     tile[r][tx] = val;
   }
-  if (i0 < 0 || j0 < 0 || i0 + 32 > G.nx || j0 + 32 > G.ny) {
-    for (int r = ty; r < 32; r += 8) {
-      const int i = i0 + tx, j = j0 + r;
-      if (i < G.nx + G.g && j < G.ny + G.gy && (i < 0 || i >= G.nx || j < 0 || j >= G.ny)) {
-        const double val = ghost_value(C, s, v, i, j);
-        C.U[pl + (long)(j + G.gy) * G.nxg + (i + G.g)] = val;
-        tile[r][tx] = val;
-      }
-    }
-  }
   __syncthreads();
   for (int r = ty; r < 32; r += 8) {
     const int i = i0 + r, j = j0 + tx;
-    if (i < G.nx + G.g && j < G.ny + G.gy) C.UT[pl + (long)(i + G.g) * G.nyg + (j + G.gy)] = tile[tx][r];
+    if (i < G.nx && j < G.ny) C.UT[pl + (long)(i + G.g) * G.nyg + (j + G.gy)] = tile[tx][r];
   }
+}
+
+// Ghost points of the stage state, one thread per (variable, ghost point):
+// the g rows below and above the interior (full ghosted width), then the g
+// columns left and right of it (interior height).  Written to both layouts.
+__host__ __device__ inline long ghost_points(const Grid &G) {
+  return G.ndims == 1 ? 2L * G.g : 2L * G.gy * G.nxg + 2L * G.g * G.ny;
+}
+
+__global__ void __launch_bounds__(256) k_ghost(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const Grid &G = C.G;
+  const long ng = ghost_points(G);
+  if (t >= ng * G.nv) return;
+  const int v = (int)(t / ng);
+  long q = t % ng;
+  int i, j = 0;
+  if (G.ndims == 1) {
+    i = (q < G.g) ? (int)q - G.g : G.nx + (int)(q - G.g);
+  } else if (q < 2L * G.gy * G.nxg) {
+    const int row = (int)(q / G.nxg);
+    i = (int)(q % G.nxg) - G.g;
+    j = row < G.gy ? row - G.gy : G.ny + (row - G.gy);
+  } else {
+    q -= 2L * G.gy * G.nxg;
+    const int col = (int)(q / G.ny);
+    j = (int)(q % G.ny);
+    i = col < G.g ? col - G.g : G.nx + (col - G.g);
+  }
+  const double val = ghost_value(C, stage, v, i, j);
+  const long pl = (long)v * G.plane;
+  C.U[pl + (long)(j + G.gy) * G.nxg + (i + G.g)] = val;
+  if (G.ndims == 2) C.UT[pl + (long)(i + G.g) * G.nyg + (j + G.gy)] = val;
 }
 
 
@@ -1425,6 +1448,46 @@ __global__ void __launch_bounds__(256) k_upwind(CASE_ARGS) {
   }
 }
 
+// Conservation check, boundary part: the interface fluxes on the domain
+// boundary summed over the lines in index order (HyperbolicFunction.c:104-110),
+// one thread per variable; after the last stage the step's integral
+// sum_s dt b_s (stage sums) (TimeRK.c:86-97), weighted by the face elements
+// (BoundaryIntegral.c:37-58), is added to the case's total.
+__global__ void __launch_bounds__(256) k_bflux(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const int nv = C.G.nv, nd = C.G.ndims;
+  if (t >= nv) return;
+  const int v = (int)t, nf = 2 * nd * nv;
+  double *bf = C.bflux + (long)stage * nf;
+  for (int d = 0; d < nd; d++) {
+    const SweepGeom &S = d ? C.Sy : C.Sx;
+    const double *fI = d ? C.fIy : C.fIx;
+    const int N = S.N, nl = S.nlines;
+    double lo = 0.0, hi = 0.0;
+    for (int l = 0; l < nl; l++) {
+      const long a = ((long)v * nl + l) * (N + 1);
+      lo -= fI[a];
+      hi += fI[a + N];
+    }
+    bf[(2 * d + 0) * nv + v] = lo;
+    bf[(2 * d + 1) * nv + v] = hi;
+  }
+  if (stage != C.nstages - 1) return;
+  double local = 0.0;
+  for (int d = 0; d < nd; d++) {
+    double step[2];
+    for (int f = 0; f < 2; f++) {
+      double sb = 0.0;
+      for (int s = 0; s < C.nstages; s++) sb += (C.dt * C.b[s]) * C.bflux[(long)s * nf + (2 * d + f) * nv + v];
+      step[f] = sb;
+    }
+    local += step[0] * C.dS[d];
+    local += step[1] * C.dS[d];
+  }
+  C.tbi[v] += 1.0 * local;
+}
+
 // ---------------------------------------------------------------------------
 // Parabolic term (navierstokes2d): NavierStokes2DParabolicFunction.c:41-204.
 // Serial-run semantics: d/dx of Q is evaluated on interior rows only and d/dy
@@ -1869,8 +1932,8 @@ __global__ void __launch_bounds__(TB) k_weno_weights(int kind, double c1, double
 // host side: work decomposition and launchers
 // ===========================================================================
 
-static const char *kKNames[K_NCLASS] = {"stage_bc", "flux", "recon_ends", "recon", "recon_char", "upwind", "qderiv",
-                                        "viscflux", "assemble", "update"};
+static const char *kKNames[K_NCLASS] = {"stage_bc", "ghost", "flux", "recon_ends", "recon", "recon_char", "upwind",
+                                        "bflux", "qderiv", "viscflux", "assemble", "update"};
 const char *kclass_name(int k) { return kKNames[k]; }
 
 bool team_shape(int N, int max_rows, int &team, int &rows) {
@@ -1886,7 +1949,8 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
   const Grid &G = c.G;
   const bool two = (G.ndims == 2);
   auto nb = [](long n) { return (int)((n + TB - 1) / TB); };
-  blocks[K_STAGE] = two ? ((G.nxg + 31) / 32) * ((G.nyg + 31) / 32) * G.nv : nb((long)G.nxg * G.nv);
+  blocks[K_STAGE] = two ? ((G.nx + 31) / 32) * ((G.ny + 31) / 32) * G.nv : nb((long)G.nx * G.nv);
+  blocks[K_GHOST] = nb(ghost_points(G) * G.nv);
   blocks[K_FLUX] = nb((long)(c.Sx.N + 2 * c.Sx.g) * c.Sx.nlines + (two ? (long)(c.Sy.N + 2 * c.Sy.g) * c.Sy.nlines : 0));
   if (c.R.characteristic) {
     blocks[K_ENDS] = 0;
@@ -1898,6 +1962,7 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
     blocks[K_RECON] = recon_sweep_blocks(c.Sx, G.nv, c.paired) + (two ? recon_sweep_blocks(c.Sy, G.nv, c.paired) : 0);
   }
   blocks[K_UPWIND] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
+  blocks[K_BFLUX] = c.cons ? 1 : 0;
   const bool visc = two && c.P.viscous;
   blocks[K_QDERIV] = visc ? nb((long)G.nxg * G.nyg) : 0;
   blocks[K_VISCFLUX] = visc ? nb((long)G.nxg * G.nyg) : 0;
@@ -1918,11 +1983,13 @@ static void run(KernFn k, int cls, const CaseDev *d, const Plan &pl, int s, hipS
 void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, void (*mark)(void *, int, bool),
                   void *ctx) {
   run(k_stage, K_STAGE, d, pl, s, st, mark, ctx);
+  run(k_ghost, K_GHOST, d, pl, s, st, mark, ctx);
   run(k_flux, K_FLUX, d, pl, s, st, mark, ctx);
   run(k_recon_ends, K_ENDS, d, pl, s, st, mark, ctx);
   run(k_recon, K_RECON, d, pl, s, st, mark, ctx);
   run(k_recon_char, K_RCHAR, d, pl, s, st, mark, ctx);
   run(k_upwind, K_UPWIND, d, pl, s, st, mark, ctx);
+  run(k_bflux, K_BFLUX, d, pl, s, st, mark, ctx);
   run(k_qderiv, K_QDERIV, d, pl, s, st, mark, ctx);
   run(k_viscflux, K_VISCFLUX, d, pl, s, st, mark, ctx);
   run(k_assemble, K_ASSEMBLE, d, pl, s, st, mark, ctx);

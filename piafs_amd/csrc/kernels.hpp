@@ -106,12 +106,19 @@ struct CaseDev {
   double *bcvals;  // the zones' values (Zone::vals)
   double *ends;  // paired mode: raw end-interface values of both systems of every pair [dir][pair][side][0|N]
   double *time;  // current simulation time of the case (one double)
+  // conservation check (conservation_check yes): boundary flux sums of each
+  // stage [stage][(2 d + face) nv + v], the total boundary integral [nv], and
+  // dS[d], the face element of dimension d (BoundaryIntegral.c:37-44)
+  int cons;
+  double dS[2];
+  double *bflux, *tbi;
   int *status;   // bit 0: a zero pivot met in a line solve (tridiagLU.c:139-142 returns -1)
 };
 
 // Kernel classes launched per RK stage (plus the final update).
 enum KClass {
-  K_STAGE = 0, K_FLUX, K_ENDS, K_RECON, K_RCHAR, K_UPWIND, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE, K_NCLASS
+  K_STAGE = 0, K_GHOST, K_FLUX, K_ENDS, K_RECON, K_RCHAR, K_UPWIND, K_BFLUX, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE,
+  K_NCLASS
 };
 const char *kclass_name(int k);
 

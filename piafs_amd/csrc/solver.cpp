@@ -206,6 +206,25 @@ int Case::setup(int dev, hipStream_t st) {
   D.bcvals = dalloc(zvals.size() + 1, st);
   D.imap = dalloc(G.plane, st);
   D.time = dalloc(1, st);
+  // conservation check (InitialSolution.c:115-135, TimePostStep.c:68-80)
+  D.cons = (cfg.conservation_check == "yes") ? 1 : 0;
+  if (D.cons) {
+    long off = 0;
+    double dxm[2] = {1.0, 1.0};  // dxinv at the middle of each dimension (BoundaryIntegral.c:38-41)
+    for (int d = 0; d < cfg.ndims; d++) {
+      dxm[d] = cfg.dxinv[off + cfg.ghosts + cfg.dim[d] / 2];
+      off += cfg.dim[d] + 2 * cfg.ghosts;
+    }
+    for (int d = 0; d < 2; d++) {
+      double dS = 1.0;
+      for (int k = 0; k < cfg.ndims; k++)
+        if (k != d) dS *= (1.0 / dxm[k]);
+      D.dS[d] = dS;
+    }
+    D.bflux = dalloc(4 * 2 * (size_t)cfg.ndims * nv, st);
+    D.tbi = dalloc(nv, st);
+    if (!D.bflux || !D.tbi) return fail(-2, "device allocation failed");
+  }
   if (D.paired) {
     D.ends = dalloc(4 * 2 * (size_t)nv * (D.Sx.nlines + (two ? D.Sy.nlines : 0)), st);
     if (!D.ends) return fail(-2, "device allocation failed");
@@ -771,40 +790,68 @@ int piafs_estimate_bytes(const char *dir, size_t *bytes) {
 int piafs_solve(piafs_case *pc, const char *dir, int verbose, double *seconds) {
   Case *c = pc->c;
   const Config &cf = c->cfg;
+  const auto t_main = std::chrono::steady_clock::now();
   std::string d = dir ? dir : "";
-  auto outname = [&](int index) {
-    std::string fn = d.empty() ? "op" : d + "/op";
-    if (cf.op_overwrite == "no") {
-      char b[16];
-      snprintf(b, sizeof b, "_%05d", index);
-      fn += b;
-    }
-    return fn + ".bin";
-  };
-  bool binary = (cf.op_file_format == "binary" || cf.op_file_format == "bin");
+  const std::string ext = output_extension(cf);
+  const bool write = !ext.empty();
+  // op[_NNNNN]: the index advances after every write when op_overwrite is "no"
+  // (OutputSolution.cpp:58-62); a restart fast-forwards it (InitializeSolvers.c:240-245)
   int index = 0;
-  if (cf.restart_iter && cf.file_op_iter > 0) index = cf.restart_iter / cf.file_op_iter;  // InitializeSolvers.c:240-245
+  if (cf.op_overwrite == "no" && cf.restart_iter)
+    for (int it = 0; it < cf.restart_iter; it++)
+      if ((it + 1) % cf.file_op_iter == 0) index++;
+  auto suffix = [&]() {
+    if (cf.op_overwrite != "no") return std::string();
+    char b[16];
+    snprintf(b, sizeof b, "_%05d", index);
+    return std::string(b);
+  };
+  double stage_time = -1.0;  // time of the latest right-hand-side evaluation (none yet)
+  auto output = [&]() -> int {  // PhysicsOutput, then OutputSolution (Solve.cpp:95-107)
+    if (!write) return 0;
+    if (cf.species_output && c->write_species(d, suffix(), stage_time, pc->eng.stream)) return 2;
+    std::string fn = (d.empty() ? std::string() : d + "/") + "op" + suffix() + ext;
+    if (verbose) printf("Writing solution file %s.\n", fn.c_str());
+    if (c->write_solution(fn, pc->eng.stream)) return 2;
+    if (cf.op_overwrite == "no") index++;
+    return 0;
+  };
   auto t0 = std::chrono::steady_clock::now();
   std::vector<double> cfl(1), norm(1);
+  long steps = 0;
   for (int iter = cf.restart_iter; iter < cf.n_iter; iter++) {
-    if (iter == 0 && binary) {
-      if (c->write_binary(outname(index), pc->eng.stream)) return case_fail(pc, 2);
-      index++;
-    }
+    if (iter == 0 && output()) return case_fail(pc, 2);
+    const double t_start = c->t;
     bool screen = ((iter + 1) % cf.screen_op_iter == 0);
     int rc = screen && verbose ? piafs_step_diag(pc, 1, cfl.data(), norm.data()) : piafs_step(pc, 1);
     if (rc) return rc;
+    steps++;
+    stage_time = t_start + cf.c[cf.nstages - 1] * cf.dt;
     if (screen && verbose) printf("iter=%7d  t=%1.3E  CFL=%1.3E  norm=%1.4E\n", iter + 1, c->t, cfl[0], norm[0]);
-    if (((iter + 1) % cf.file_op_iter == 0) && ((iter + 1) < cf.n_iter) && binary) {
-      if (c->write_binary(outname(index), pc->eng.stream)) return case_fail(pc, 2);
-      index++;
-    }
+    if (((iter + 1) % cf.file_op_iter == 0) && ((iter + 1) < cf.n_iter) && output()) return case_fail(pc, 2);
   }
   int rc = piafs_sync(pc);
   if (rc) return rc;
   double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (seconds) *seconds = secs;
-  if (binary && c->write_binary(outname(index), pc->eng.stream)) return case_fail(pc, 2);
+  // CalculateError, then the final output (Solve.cpp:86-107), then SimWriteErrors (main.cpp)
+  double err3[3];
+  if (c->exact_errors(d, err3, pc->eng.stream)) return case_fail(pc, 1);
+  if (output()) return case_fail(pc, 2);
+  double main_secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_main).count();
+  if (c->write_run_files(d, err3, steps, secs, main_secs, pc->eng.stream)) return case_fail(pc, 2);
+  if (verbose && err3[0] >= 0) {
+    printf("Computed errors for domain 0:\n");
+    printf("  L1         Error           : %1.16E\n", err3[0]);
+    printf("  L2         Error           : %1.16E\n", err3[1]);
+    printf("  Linfinity  Error           : %1.16E\n", err3[2]);
+  }
+  if (verbose && c->h.cons) {
+    std::vector<double> ce;
+    if (c->conservation_errors(ce, pc->eng.stream)) return case_fail(pc, -1);
+    printf("Conservation Errors:\n");
+    for (double e : ce) printf("\t%1.16E\n", e);
+  }
   return 0;
 }
 

@@ -211,11 +211,66 @@ def config5_nochem(run_dir):
     print("config5 chemistry-off fixture written", U.shape)
 
 
+# Decks whose complete file output (solution, species, errors.dat and the
+# other run files) is kept as reference output: name -> (builder, kwargs,
+# solver.inp overrides, exact.inp = initial.inp)
+OUTPUT_CASES = {
+    "tecplot_species_exact": ("ns2d_photochem", dict(nx=10, ny=8, n_iter=6, z_mm=1.5),
+                              dict(op_file_format="tecplot2d", op_overwrite="no", file_op_iter=2), True),
+    "text_sod_exact": ("sod_1d", dict(N=24, n_iter=5, t_final=0.02),
+                       dict(op_file_format="text", op_overwrite="yes", file_op_iter=2), True),
+    "binary_restart": ("riemann_2d", dict(N=12, n_iter=8, dt=0.004),
+                       dict(op_file_format="binary", op_overwrite="no", file_op_iter=2, restart_iter=4), False),
+    "conservation_riemann": ("riemann_2d", dict(N=16, n_iter=10, dt=0.004),
+                             dict(op_file_format="none", conservation_check="yes"), False),
+    "conservation_sod_char": ("sod_1d", dict(N=32, n_iter=12, t_final=0.04),
+                              dict(op_file_format="binary", op_overwrite="yes", conservation_check="yes"), True),
+}
+RUN_FILES = ("errors.dat", "conservation.dat", "function_counts.dat")
+
+
+def output_deck(name):
+    fn, kw, over, exact = OUTPUT_CASES[name]
+    dk = getattr(decks, fn)(**kw)
+    dk.solver.update(over)
+    return dk, exact
+
+
+def outputs():
+    """Run each output deck through the reference and keep every file it writes."""
+    base = os.path.join(HERE, "outputs")
+    meta = {}
+    for name in OUTPUT_CASES:
+        dk, exact = output_deck(name)
+        dst = os.path.join(base, name)
+        if os.path.isdir(dst):
+            shutil.rmtree(dst)
+        os.makedirs(dst)
+        with tempfile.TemporaryDirectory() as d:
+            dk.write(d)
+            if exact:
+                shutil.copy(os.path.join(d, "initial.inp"), os.path.join(d, "exact.inp"))
+            before = set(os.listdir(d))
+            r = subprocess.run([REF_BIN], cwd=d, capture_output=True, text=True)
+            if r.returncode:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            made = sorted(set(os.listdir(d)) - before)
+            for fn in made:
+                shutil.copy(os.path.join(d, fn), os.path.join(dst, fn))
+        meta[name] = {"builder": OUTPUT_CASES[name][0], "kwargs": OUTPUT_CASES[name][1],
+                      "solver": OUTPUT_CASES[name][2], "exact": OUTPUT_CASES[name][3], "files": made}
+        print(name, made)
+    with open(os.path.join(base, "outputs.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 2 and sys.argv[1] == "--config2-full":
         config2_full(sys.argv[2])
     elif len(sys.argv) > 2 and sys.argv[1] == "--config5-nochem":
         config5_nochem(sys.argv[2])
+    elif len(sys.argv) > 1 and sys.argv[1] == "--outputs":
+        outputs()
     else:
         ref_runs()
         example_logs()
