@@ -86,10 +86,14 @@ int  piafs_rhs(piafs_case *c, const double *u_host, double t, double *rhs_host);
  * OutputSolution (src/Simulation/OutputSolution.cpp:25). */
 int  piafs_write_binary(piafs_case *c, const char *path);
 
-/* Whole solve as the reference's Solve (src/Simulation/Solve.cpp:29-123):
- * n_iter steps from restart_iter, screen output every screen_op_iter, binary
- * output files op[_NNNNN].bin as configured, final solution written.
- * Returns the wall time of the time loop in *seconds (may be NULL). */
+/* Whole solve as the reference's Solve (src/Simulation/Solve.cpp:29-123)
+ * followed by SimWriteErrors (src/Simulation/SimulationWriteErrors.c:21-100):
+ * n_iter steps from restart_iter, screen output every screen_op_iter, the
+ * solution files op[_NNNNN].{bin,dat} in the deck's op_file_format
+ * (OutputSolution.cpp:22-70), op_species_z* with chemistry
+ * (ChemistryWriteSpecies.c:18-80), then errors.dat (against exact.inp when
+ * present, CalculateError.c:28-121), conservation.dat and function_counts.dat
+ * in dir.  Returns the wall time of the time loop in *seconds (may be NULL). */
 int  piafs_solve(piafs_case *c, const char *dir, int verbose, double *seconds);
 
 /* Device timing of the dominant kernel: average duration (ms) of each
