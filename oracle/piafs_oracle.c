@@ -7,8 +7,8 @@
  * order as the reference so that gcc -O2 builds agree bit for bit.
  *
  * Supported (the hot path): models euler1d / navierstokes2d (+ photochemistry),
- * hyp_space_scheme weno5 / crweno5, hyp_interp_type components (both models)
- * and characteristic (weno5, both models), WENO weights JS / mapped / borges(Z) /
+ * hyp_space_scheme weno5 / crweno5, hyp_interp_type components and
+ * characteristic (both models; characteristic CRWENO5 with the block solver), WENO weights JS / mapped / borges(Z) /
  * yc and no_limiting, upwinding roe / rf-char / llf-char / rusanov (both
  * models), every boundary type of boundary.inp (periodic, extrapolate,
  * dirichlet, reflect, sponge, slip and no-slip walls, subsonic inflow /
@@ -512,7 +512,6 @@ int orc_load(const char *dir, orc_case **out)
   c->include_chem = !strcmp(include_chem, "yes");
   if (c->include_chem) { if (chem_init(c, dir)) { orc_free(c); return 9; } }
   if (c->nvars != c->nflow + (c->include_chem ? c->n_react : 0)) { orc_free(c); return 10; }
-  if (c->is_char && c->is_crweno) { orc_free(c); return 11; }
   /* time (TimeInitialize.c:51) and work arrays */
   c->waqt = (double)c->restart_iter * c->dt;
   {
@@ -1113,6 +1112,175 @@ static void recon_line_char(const orc_case *c, const double *fC, const double *u
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* characteristic CRWENO5: Interp1PrimFifthOrderCRWENOChar.c:96-280     */
+/* with the serial block solver blocktridiagLU.c:110-322 and the dense  */
+/* matrix macros of matops.h:16-130, all n x n (n = nvars)              */
+/* ------------------------------------------------------------------ */
+
+/* _MatrixInvert_ (matops.h:96-130): Gauss-Jordan without pivoting */
+static void bt_inv(int n, const double *A, double *B)
+{
+  double Ac[NMAX * NMAX], f, sum;
+  int i, j, k;
+  for (i = 0; i < n * n; i++) Ac[i] = A[i];
+  for (i = 0; i < n * n; i++) B[i] = 0.0;
+  for (i = 0; i < n; i++) B[i * n + i] = 1.0;
+  for (i = 0; i < n - 1; i++)
+    for (j = i + 1; j < n; j++) {
+      f = Ac[j * n + i] / Ac[i * n + i];
+      for (k = i + 1; k < n; k++) Ac[j * n + k] -= (f * Ac[i * n + k]);
+      for (k = 0; k < j; k++) B[j * n + k] -= (f * B[i * n + k]);
+    }
+  for (i = n - 1; i >= 0; i--)
+    for (k = 0; k < n; k++) {
+      sum = 0.0;
+      for (j = i + 1; j < n; j++) sum += (Ac[i * n + j] * B[j * n + k]);
+      B[i * n + k] = (B[i * n + k] - sum) / Ac[i * n + i];
+    }
+}
+/* _MatrixMultiply_: C = A B */
+static void bt_mul(int n, const double *A, const double *B, double *C)
+{
+  int i, j, k;
+  for (i = 0; i < n; i++)
+    for (j = 0; j < n; j++) {
+      C[i * n + j] = 0;
+      for (k = 0; k < n; k++) C[i * n + j] += (A[i * n + k] * B[k * n + j]);
+    }
+}
+/* _MatrixMultiplySubtract_: C -= A B */
+static void bt_mulsub(int n, double *C, const double *A, const double *B)
+{
+  int i, j, k;
+  for (i = 0; i < n; i++)
+    for (j = 0; j < n; j++)
+      for (k = 0; k < n; k++) C[i * n + j] -= (A[i * n + k] * B[k * n + j]);
+}
+/* _MatVecMultiplySubtract_: y -= A x */
+static void bt_mvsub(int n, double *y, const double *A, const double *x)
+{
+  int i, j;
+  for (i = 0; i < n; i++)
+    for (j = 0; j < n; j++) y[i] -= (A[i * n + j] * x[j]);
+}
+/* _MatVecMultiply_: y = A x */
+static void bt_mv(int n, const double *A, const double *x, double *y)
+{
+  int i, j;
+  for (i = 0; i < n; i++) {
+    y[i] = 0;
+    for (j = 0; j < n; j++) y[i] += (A[i * n + j] * x[j]);
+  }
+}
+
+/* blocktridiagLU on one process (blocktridiagLU.c:159-180 stage 1, :279-301
+ * stage 4), one system of m block rows; a, b, c, x are overwritten */
+static void bt_solve(int m, int n, double *a, double *b, double *c, double *x)
+{
+  int i, j, n2 = n * n;
+  double binv[NMAX * NMAX], factor[NMAX * NMAX], xt[NMAX], xp1[NMAX];
+  for (j = 0; j < n; j++) xp1[j] = 0.0;
+  for (i = 1; i < m; i++) {
+    bt_inv(n, b + (i - 1) * n2, binv);
+    bt_mul(n, a + i * n2, binv, factor);
+    bt_mulsub(n, b + i * n2, factor, c + (i - 1) * n2);
+    for (j = 0; j < n2; j++) a[i * n2 + j] = 0.0;
+    bt_mulsub(n, a + i * n2, factor, a + (i - 1) * n2);
+    bt_mvsub(n, x + i * n, factor, x + (i - 1) * n);
+  }
+  bt_inv(n, b + (m - 1) * n2, binv);
+  bt_mvsub(n, x + (m - 1) * n, a + (m - 1) * n2, x);
+  bt_mvsub(n, x + (m - 1) * n, c + (m - 1) * n2, xp1);
+  bt_mv(n, binv, x + (m - 1) * n, xt);
+  for (j = 0; j < n; j++) x[(m - 1) * n + j] = xt[j];
+  for (i = m - 2; i > -1; i--) {
+    bt_inv(n, b + i * n2, binv);
+    bt_mvsub(n, x + i * n, c + i * n2, x + (i + 1) * n);
+    bt_mvsub(n, x + i * n, a + i * n2, x);
+    bt_mv(n, binv, x + i * n, xt);
+    for (j = 0; j < n; j++) x[i * n + j] = xt[j];
+  }
+}
+
+/* One line: the four block systems (uL, uR, fL, fR), each row the CRWENO5
+ * relation of every characteristic field at interface k, written in the
+ * physical variables through L of the Roe average at k; WENO5 rows
+ * (B = L) at the two ends of the line. */
+static void recon_line_crweno_char(const orc_case *c, const double *fC, const double *uC, int base, int st,
+                                   int N, int dir, double *uLo, double *uRo, double *fLo, double *fRo,
+                                   size_t ostride)
+{
+  int nv = c->nvars, n2 = nv * nv, k, v, j, which, kind = weight_kind(c);
+  static const double t3 = 1.0 / 3.0, s6 = 1.0 / 6.0;
+  double *A = (double *)malloc(sizeof(double) * (size_t)(N + 1) * (3 * n2 + nv));
+  double *B = A + (size_t)(N + 1) * n2, *C = B + (size_t)(N + 1) * n2, *F = C + (size_t)(N + 1) * n2;
+  double L[NMAX * NMAX], R[NMAX * NMAX], ua[NMAX];
+  for (which = 0; which < 4; which++) { /* 0:uL 1:uR 2:fL 3:fR */
+    const double *src = (which < 2) ? uC : fC;
+    int upw = (which == 0 || which == 2) ? 1 : -1;
+    double *out = (which == 0) ? uLo : (which == 1) ? uRo : (which == 2) ? fLo : fRo;
+    for (k = 0; k <= N; k++) {
+      int q[5], bnd = (k == 0 || k == N);
+      if (upw > 0) { q[0] = k - 3; q[1] = k - 2; q[2] = k - 1; q[3] = k; q[4] = k + 1; }
+      else { q[0] = k + 2; q[1] = k + 1; q[2] = k; q[3] = k - 1; q[4] = k - 2; }
+      /* Roe average of the cells qm1 and qp1 (swapped for the right bias) */
+      if (c->model_id == 1) {
+        roeavg_euler1d(c, ua, LP(uC, q[2]), LP(uC, q[3]));
+        eig_euler1d(c, ua, NULL, L, R);
+      } else {
+        roeavg_ns2d(c, ua, LP(uC, q[2]), LP(uC, q[3]));
+        eig_ns2d(c, ua, dir, NULL, L, R);
+      }
+      for (v = 0; v < nv; v++) {
+        double s[5], w[3], cw[3], f1, f2, f3, ca, cb, cc;
+        int t;
+        for (t = 0; t < 5; t++) {
+          double acc = 0;
+          for (j = 0; j < nv; j++) acc += L[v * nv + j] * LP(src, q[t])[j];
+          s[t] = acc;
+        }
+        optimal_weights(c, k, N, cw);
+        if (c->no_limiting) { w[0] = cw[0]; w[1] = cw[1]; w[2] = cw[2]; }
+        else orc_weno_weights(kind, cw[0], cw[1], cw[2], s[0], s[1], s[2], s[3], s[4], c->eps, w);
+        if (bnd) {
+          f1 = (2 * s6) * s[0] - (7.0 * s6) * s[1] + (11.0 * s6) * s[2];
+          f2 = (-s6) * s[1] + (5.0 * s6) * s[2] + (2 * s6) * s[3];
+          f3 = (2 * s6) * s[2] + (5 * s6) * s[3] - (s6) * s[4];
+          for (j = 0; j < nv; j++) {
+            A[k * n2 + v * nv + j] = 0.0;
+            C[k * n2 + v * nv + j] = 0.0;
+            B[k * n2 + v * nv + j] = L[v * nv + j];
+          }
+        } else {
+          f1 = (s6) * (s[1] + 5 * s[2]);
+          f2 = (s6) * (5 * s[2] + s[3]);
+          f3 = (s6) * (s[2] + 5 * s[3]);
+          ca = ((2 * t3) * w[0] + (t3) * w[1]);
+          cb = ((t3) * w[0] + (2 * t3) * (w[1] + w[2]));
+          cc = ((t3) * w[2]);
+          for (j = 0; j < nv; j++) {
+            if (upw > 0) {
+              A[k * n2 + v * nv + j] = ca * L[v * nv + j];
+              B[k * n2 + v * nv + j] = cb * L[v * nv + j];
+              C[k * n2 + v * nv + j] = cc * L[v * nv + j];
+            } else {
+              C[k * n2 + v * nv + j] = ca * L[v * nv + j];
+              B[k * n2 + v * nv + j] = cb * L[v * nv + j];
+              A[k * n2 + v * nv + j] = cc * L[v * nv + j];
+            }
+          }
+        }
+        F[k * nv + v] = w[0] * f1 + w[1] * f2 + w[2] * f3;
+      }
+    }
+    bt_solve(N + 1, nv, A, B, C, F);
+    for (k = 0; k <= N; k++)
+      for (v = 0; v < nv; v++) out[k * ostride + v] = F[k * nv + v];
+  }
+  free(A);
+}
+
 /* Upwinding at the interfaces of one line.
  * NavierStokes2DUpwindRoe (NavierStokes2DUpwind.c:26-92),
  * Euler1DUpwindRoe (Euler1DUpwind.c:27-93), Euler1DUpwindRF (:107-190). */
@@ -1289,7 +1457,9 @@ static void hyperbolic(orc_case *c, const double *u, double *hyp)
       if (other >= 0) idx[other] = l;
       idx[d] = 0;
       base = gidx(c, idx);
-      if (c->is_char) recon_line_char(c, c->fluxC, u, base, st, N, d, uLl, uRl, fLl, fRl, (size_t)nv);
+      if (c->is_char && c->is_crweno)
+        recon_line_crweno_char(c, c->fluxC, u, base, st, N, d, uLl, uRl, fLl, fRl, (size_t)nv);
+      else if (c->is_char) recon_line_char(c, c->fluxC, u, base, st, N, d, uLl, uRl, fLl, fRl, (size_t)nv);
       else recon_line_components(c, c->fluxC, u, base, st, N, uLl, uRl, fLl, fRl, (size_t)nv);
       upwind_line(c, u, base, st, N, d, uLl, uRl, fLl, fRl, fIl, (size_t)nv);
       /* hyp += dxinv * (fI[k+1] - fI[k]) at the N interior points of the line */

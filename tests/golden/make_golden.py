@@ -36,6 +36,9 @@ from piafs_amd import decks  # noqa: E402
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "PIAFS")
 REF_TREE = "/root/reference"
 
+PERIODIC_2D = [("periodic", 0, 1, 0, 0, -1e99, 1e99), ("periodic", 0, -1, 0, 0, -1e99, 1e99),
+               ("periodic", 1, 1, -1e99, 1e99, 0, 0), ("periodic", 1, -1, -1e99, 1e99, 0, 0)]
+
 # name -> (builder, kwargs); every deck writes op.bin at the end
 CASES = {
     "sod_char_roe_ssprk3_N1024": ("sod_1d", dict(N=1024)),
@@ -113,6 +116,27 @@ CASES = {
                                                boundary=[("extrapolate", 0, 1, 0, 0), ("extrapolate", 0, -1, 0, 0),
                                                          ("sponge", 0, -1, 0.8, 1.0, 0.125, 0.0, 0.25)])),
     "sod_char_rusanov_N64": ("sod_1d", dict(N=64, n_iter=20, t_final=0.07, upwinding="rusanov")),
+    # characteristic CRWENO5 (block-tridiagonal line solves)
+    "sod_crweno_char_roe_N64": ("sod_1d", dict(N=64, n_iter=20, t_final=0.07, scheme="crweno5")),
+    "sod_crweno_char_rf_mapped_N48": ("sod_1d", dict(N=48, n_iter=15, t_final=0.07, scheme="crweno5",
+                                                     upwinding="rf-char", mapped=1)),
+    # 2D: the reference inverts the blocks without pivoting, and the first pivot
+    # ((gamma-1) e_k + a u_n)/(2 a^2) vanishes where the gas is at rest, so the
+    # 2D decks move in both directions everywhere
+    "ns2d_crweno_char_flow_20x16": ("ns2d_open_box", dict(nx=20, ny=16, n_iter=10, flow=(1.0, 0.3, 0.2, None),
+                                                          scheme="crweno5", interp="characteristic",
+                                                          boundary=PERIODIC_2D)),
+    "ns2d_chem_crweno_char_rf_flow_16x12": ("ns2d_open_box", dict(nx=16, ny=12, n_iter=8,
+                                                                  flow=(1.0, 0.25, 0.35, 0.7), scheme="crweno5",
+                                                                  interp="characteristic", upwinding="rf-char",
+                                                                  include_chemistry=True, Re=1000.0,
+                                                                  boundary=PERIODIC_2D)),
+    "ns2d_crweno_char_llf_mapped_inout_20x16": ("ns2d_open_box", dict(
+        nx=20, ny=16, n_iter=10, flow=(1.0, 0.3, 0.2, None), scheme="crweno5", interp="characteristic",
+        upwinding="llf-char", weights="mapped", Re=300.0, boundary=[
+            ("subsonic-inflow", 0, 1, 0, 0, -1e99, 1e99, 1.0, 0.3, 0.2),
+            ("subsonic-outflow", 0, -1, 0, 0, -1e99, 1e99, 0.7142857142857143),
+            ("periodic", 1, 1, -1e99, 1e99, 0, 0), ("periodic", 1, -1, -1e99, 1e99, 0, 0)])),
 }
 
 
