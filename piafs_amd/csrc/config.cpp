@@ -527,10 +527,6 @@ int Config::parse(const std::string &dir, std::string &err) {
     err = "nvars has to be " + std::to_string(expect) + " in solver.inp";
     return 1;
   }
-  if (characteristic && crweno) {
-    err = "characteristic interpolation is supported for weno5 on this path";
-    return 1;
-  }
   if (nvars > 64) {
     err = "nvars > 64 not supported";
     return 1;

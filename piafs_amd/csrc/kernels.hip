@@ -1170,6 +1170,340 @@ __global__ void __launch_bounds__(256) k_recon_char(CASE_ARGS) {
 #undef RCHAR
 }
 
+// ---------------------------------------------------------------------------
+// Characteristic CRWENO5: Interp1PrimFifthOrderCRWENOChar.c:96-280 with the
+// reference's block solver (blocktridiagLU.c:159-180 and :279-301 on one
+// process; _MatrixInvert_ etc., matops.h:16-130).  Each interface row ties
+// the characteristic fields through L of the Roe average at that interface,
+// so the unknowns are the physical interface values and every row is a block
+// row.  L's species block is the identity, and the reference's dense n x n
+// arithmetic then reduces exactly (up to the sign of zeros) to the flow block
+// plus one scalar chain per species; both are computed here in the
+// reference's operation order.
+// ---------------------------------------------------------------------------
+
+// block rows of every system at interface k of line l: one thread each
+template <int MODEL, int DIR, int KIND>
+__device__ __forceinline__ void cwc_rows(const CaseDev &C, const SweepGeom &S, int l, int k, const double *q,
+                                         const double *f, double *rows) {
+  constexpr int NF = MODEL == 1 ? 3 : 4;
+  const int nv = C.G.nv, nl = S.nlines, N = S.N;
+  const long pl = S.plane;
+  const double gam = C.P.gamma;
+  const double *ql = q + S.base + (long)l * S.line_stride, *fl = f + S.base + (long)l * S.line_stride;
+  double L[NF][NF];
+  if (MODEL == 1) {
+    double a0, a1, a2;
+    euler1d_roeavg(gam, ql[k - 1], ql[pl + k - 1], ql[2 * pl + k - 1], ql[k], ql[pl + k], ql[2 * pl + k], a0, a1,
+                   a2);
+    Eig3 E;
+    euler1d_eig(gam, a0, a1, a2, E);
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+#pragma unroll
+      for (int j = 0; j < NF; j++) L[i][j] = E.L[i][j];
+  } else {
+    // the reference averages (k-1, k) for the left bias and (k, k-1) for the
+    // right one; the Roe average is symmetric to the bit
+    const double uLc[4] = {ql[k - 1], ql[pl + k - 1], ql[2 * pl + k - 1], ql[3 * pl + k - 1]};
+    const double uRc[4] = {ql[k], ql[pl + k], ql[2 * pl + k], ql[3 * pl + k]};
+    double ua[4], L4[4][4], R4[4][4];
+    ns2d_roeavg(gam, uLc, uRc, ua);
+    ns2d_eigvecs<DIR>(gam, ua, L4, R4);
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+#pragma unroll
+      for (int j = 0; j < NF; j++) L[i][j] = L4[i][j];
+  }
+  const long vs = (long)nl * (N + 1), ii = (long)l * (N + 1) + k;
+#pragma unroll
+  for (int i = 0; i < NF; i++)
+#pragma unroll
+    for (int j = 0; j < NF; j++) rows[(long)(i * NF + j) * vs + ii] = L[i][j];
+  const bool bnd = (k == 0 || k == N);
+  // optimal weights: WENO5 at the physical ends, CRWENO5 inside
+  // (WENOFifthOrderCalculateWeights.c:846-862)
+  const double c1 = bnd ? 0.1 : 0.2, c2 = bnd ? 0.6 : 0.5, c3 = 0.3;
+  const double s6 = 1.0 / 6.0, t3 = 1.0 / 3.0;
+  const ReconParams &P = C.R;
+  auto row = [&](int sys, int v, double m3, double m2, double m1, double p1, double p2) {
+    double w1, w2, w3, f1, f2, f3, ca, cb, cc;
+    if (P.no_limiting) { w1 = c1; w2 = c2; w3 = c3; }
+    else weno_w<KIND>(c1, c2, c3, m3, m2, m1, p1, p2, P.eps, w1, w2, w3);
+    if (bnd) {  // WENO5 row, B = L
+      f1 = (2 * s6) * m3 - (7.0 * s6) * m2 + (11.0 * s6) * m1;
+      f2 = (-s6) * m2 + (5.0 * s6) * m1 + (2 * s6) * p1;
+      f3 = (2 * s6) * m1 + (5 * s6) * p1 - (s6)*p2;
+      ca = 0.0; cb = 1.0; cc = 0.0;
+    } else {
+      f1 = (s6) * (m2 + 5 * m1);
+      f2 = (s6) * (5 * m1 + p1);
+      f3 = (s6) * (m1 + 5 * p1);
+      const double wa = ((2 * t3) * w1 + (t3)*w2), wb = ((t3)*w1 + (2 * t3) * (w2 + w3)), wc = ((t3)*w3);
+      const bool left = !(sys & 1);  // the right bias swaps the sub- and super-diagonal coefficients
+      ca = left ? wa : wc;
+      cb = wb;
+      cc = left ? wc : wa;
+    }
+    const long b = (long)(NF * NF + (sys * nv + v) * 4) * vs + ii;
+    rows[b] = ca;
+    rows[b + vs] = cb;
+    rows[b + 2 * vs] = cc;
+    rows[b + 3 * vs] = w1 * f1 + w2 * f2 + w3 * f3;
+  };
+#pragma unroll
+  for (int src = 0; src < 2; src++) {  // 0: the solution (uL, uR), 1: the flux (fL, fR)
+    const double *a = src ? fl : ql;
+    double c[6][NF];  // characteristic values at k-3 .. k+2
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+      const int pos = k - 3 + p;
+      double x[NF];
+#pragma unroll
+      for (int j = 0; j < NF; j++) x[j] = a[j * pl + pos];
+#pragma unroll
+      for (int v = 0; v < NF; v++) {
+        double acc = L[v][0] * x[0];
+#pragma unroll
+        for (int j = 1; j < NF; j++) acc = acc + L[v][j] * x[j];
+        c[p][v] = acc;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NF; v++) {
+      row(2 * src, v, c[0][v], c[1][v], c[2][v], c[3][v], c[4][v]);
+      row(2 * src + 1, v, c[5][v], c[4][v], c[3][v], c[2][v], c[1][v]);
+    }
+    for (int m = NF; m < nv; m++) {
+      const double *am = a + m * pl + k;
+      row(2 * src, m, am[-3], am[-2], am[-1], am[0], am[1]);
+      row(2 * src + 1, m, am[2], am[1], am[0], am[-1], am[-2]);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_cw_rows(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const long nX = (long)(C.Sx.N + 1) * C.Sx.nlines;
+  const long nY = (C.G.ndims == 2) ? (long)(C.Sy.N + 1) * C.Sy.nlines : 0;
+  if (t >= nX + nY) return;
+  const int dir = t < nX ? 0 : 1;
+  const SweepGeom &S = dir ? C.Sy : C.Sx;
+  const long u = dir ? t - nX : t;
+  const int k = (int)(u % (S.N + 1)), l = (int)(u / (S.N + 1));
+  const double *q = dir ? C.UT : C.U, *f = dir ? C.flux_y : C.flux_x;
+  double *rows = C.cwr[dir];
+#define CWROWS(KIND)                                                        \
+  (C.model == 1 ? cwc_rows<1, 0, KIND>(C, S, l, k, q, f, rows)             \
+                : (dir ? cwc_rows<2, 1, KIND>(C, S, l, k, q, f, rows)      \
+                       : cwc_rows<2, 0, KIND>(C, S, l, k, q, f, rows)))
+  switch (C.R.kind) {
+    case 1: CWROWS(1); break;
+    case 2: CWROWS(2); break;
+    case 3: CWROWS(3); break;
+    default: CWROWS(0); break;
+  }
+#undef CWROWS
+}
+
+// _MatrixInvert_ (matops.h:96-130) on the flow block: Gauss-Jordan without
+// pivoting, in the reference's loop order
+template <int NF>
+__device__ __forceinline__ void cw_inv(const double A[NF][NF], double B[NF][NF]) {
+  double Ac[NF][NF];
+#pragma unroll
+  for (int i = 0; i < NF; i++)
+#pragma unroll
+    for (int j = 0; j < NF; j++) {
+      Ac[i][j] = A[i][j];
+      B[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+#pragma unroll
+  for (int i = 0; i < NF - 1; i++)
+#pragma unroll
+    for (int j = i + 1; j < NF; j++) {
+      const double f = Ac[j][i] / Ac[i][i];
+#pragma unroll
+      for (int k = i + 1; k < NF; k++) Ac[j][k] -= (f * Ac[i][k]);
+#pragma unroll
+      for (int k = 0; k < j; k++) B[j][k] -= (f * B[i][k]);
+    }
+#pragma unroll
+  for (int i = NF - 1; i >= 0; i--)
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      double sum = 0.0;
+#pragma unroll
+      for (int j = i + 1; j < NF; j++) sum += (Ac[i][j] * B[j][k]);
+      B[i][k] = (B[i][k] - sum) / Ac[i][i];
+    }
+}
+
+// One system of one line: chain 0 is the flow block, chain 1 + m' species
+// NF + m'.  Forward elimination row by row (the back-substitution data go to
+// cwb), then back substitution from the last row, writing the interface values.
+template <int NF>
+__device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, int dir, int l, int sys,
+                                          int chain) {
+  const int nv = C.G.nv, nl = S.nlines, N = S.N;
+  const long vs = (long)nl * (N + 1), ib = (long)l * (N + 1);
+  const long bs = 4L * nl * (N + 1), bb = ((long)sys * nl + l) * (N + 1);
+  const double *rows = C.cwr[dir];
+  double *back = C.cwb[dir];
+  double *out = (dir ? C.iy : C.ix)[sys];
+  auto rf = [&](int f, int k) { return rows[(long)f * vs + ib + k]; };
+  auto coef = [&](int v, int which, int k) { return rf(NF * NF + (sys * nv + v) * 4 + which, k); };
+  if (chain > 0) {  // a species: a scalar chain of the same arithmetic
+    const int m = NF + chain - 1, f0 = 2 * NF * NF + NF + 3 * (chain - 1);
+    double bp = 0.0, xp = 0.0, cprev = 0.0;
+    for (int k = 0; k <= N; k++) {
+      const double a = coef(m, 0, k), b = coef(m, 1, k), c = coef(m, 2, k), F = coef(m, 3, k);
+      if (k == 0) {
+        bp = b;
+        xp = F;
+      } else {
+        const double rb = 1.0 / bp, fac = a * rb;
+        back[(long)f0 * bs + bb + k - 1] = rb;
+        back[(long)(f0 + 1) * bs + bb + k - 1] = cprev;
+        back[(long)(f0 + 2) * bs + bb + k - 1] = xp;
+        bp = b - fac * cprev;
+        xp = F - fac * xp;
+      }
+      cprev = c;
+    }
+    double x = (1.0 / bp) * xp;
+    out[((long)m * nl + l) * (N + 1) + N] = x;
+    for (int k = N - 1; k >= 0; k--) {
+      const double rb = back[(long)f0 * bs + bb + k], c = back[(long)(f0 + 1) * bs + bb + k];
+      const double xq = back[(long)(f0 + 2) * bs + bb + k];
+      x = rb * (xq - c * x);
+      out[((long)m * nl + l) * (N + 1) + k] = x;
+    }
+    return;
+  }
+  double bp[NF][NF], xp[NF], cprev[NF][NF];
+  for (int k = 0; k <= N; k++) {
+    double L[NF][NF], A[NF][NF], Bm[NF][NF], Cm[NF][NF], F[NF];
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+#pragma unroll
+      for (int j = 0; j < NF; j++) L[i][j] = rf(i * NF + j, k);
+#pragma unroll
+    for (int v = 0; v < NF; v++) {
+      const double a = coef(v, 0, k), b = coef(v, 1, k), c = coef(v, 2, k);
+      F[v] = coef(v, 3, k);
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        A[v][j] = a * L[v][j];
+        Bm[v][j] = b * L[v][j];
+        Cm[v][j] = c * L[v][j];
+      }
+    }
+    if (k == 0) {
+#pragma unroll
+      for (int i = 0; i < NF; i++) {
+        xp[i] = F[i];
+#pragma unroll
+        for (int j = 0; j < NF; j++) bp[i][j] = Bm[i][j];
+      }
+    } else {
+      double binv[NF][NF], fac[NF][NF];
+      cw_inv<NF>(bp, binv);
+#pragma unroll
+      for (int i = 0; i < NF; i++)
+#pragma unroll
+        for (int j = 0; j < NF; j++) {
+          double acc = 0;
+#pragma unroll
+          for (int q = 0; q < NF; q++) acc += (A[i][q] * binv[q][j]);
+          fac[i][j] = acc;
+        }
+      // row k-1 for the back substitution
+#pragma unroll
+      for (int i = 0; i < NF; i++) {
+#pragma unroll
+        for (int j = 0; j < NF; j++) {
+          back[(long)(i * NF + j) * bs + bb + k - 1] = binv[i][j];
+          back[(long)(NF * NF + i * NF + j) * bs + bb + k - 1] = cprev[i][j];
+        }
+        back[(long)(2 * NF * NF + i) * bs + bb + k - 1] = xp[i];
+      }
+      double nx[NF];
+#pragma unroll
+      for (int i = 0; i < NF; i++) {
+#pragma unroll
+        for (int j = 0; j < NF; j++) {
+          double b = Bm[i][j];
+#pragma unroll
+          for (int q = 0; q < NF; q++) b -= (fac[i][q] * cprev[q][j]);
+          bp[i][j] = b;
+        }
+        double x = F[i];
+#pragma unroll
+        for (int q = 0; q < NF; q++) x -= (fac[i][q] * xp[q]);
+        nx[i] = x;
+      }
+#pragma unroll
+      for (int i = 0; i < NF; i++) xp[i] = nx[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NF; i++)
+#pragma unroll
+      for (int j = 0; j < NF; j++) cprev[i][j] = Cm[i][j];
+  }
+  // last row, then back substitution x_k = binv_k (x'_k - C_k x_{k+1})
+  double binv[NF][NF], x[NF];
+  cw_inv<NF>(bp, binv);
+#pragma unroll
+  for (int i = 0; i < NF; i++) {
+    double acc = 0;
+#pragma unroll
+    for (int j = 0; j < NF; j++) acc += (binv[i][j] * xp[j]);
+    x[i] = acc;
+  }
+#pragma unroll
+  for (int i = 0; i < NF; i++) out[((long)i * nl + l) * (N + 1) + N] = x[i];
+  for (int k = N - 1; k >= 0; k--) {
+    double y[NF];
+#pragma unroll
+    for (int i = 0; i < NF; i++) {
+      double r = back[(long)(2 * NF * NF + i) * bs + bb + k];
+#pragma unroll
+      for (int j = 0; j < NF; j++) r -= (back[(long)(NF * NF + i * NF + j) * bs + bb + k] * x[j]);
+      y[i] = r;
+    }
+#pragma unroll
+    for (int i = 0; i < NF; i++) {
+      double acc = 0;
+#pragma unroll
+      for (int j = 0; j < NF; j++) acc += (back[(long)(i * NF + j) * bs + bb + k] * y[j]);
+      x[i] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < NF; i++) out[((long)i * nl + l) * (N + 1) + k] = x[i];
+  }
+}
+
+// work items: (sweep, chain, system, line), the line fastest and the flow
+// chains first so that a wave does one kind of chain
+__host__ __device__ inline long cw_chains(const SweepGeom &S, int nv, int nf) { return 4L * S.nlines * (1 + nv - nf); }
+
+__global__ void __launch_bounds__(256) k_cw_solve(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const int nv = C.G.nv, nf = C.model == 1 ? 3 : 4;
+  const long nX = cw_chains(C.Sx, nv, nf), nY = (C.G.ndims == 2) ? cw_chains(C.Sy, nv, nf) : 0;
+  if (t >= nX + nY) return;
+  const int dir = t < nX ? 0 : 1;
+  const SweepGeom &S = dir ? C.Sy : C.Sx;
+  const long u = dir ? t - nX : t;
+  const long per = 4L * S.nlines;
+  const int chain = (int)(u / per), sys = (int)((u % per) / S.nlines), l = (int)(u % S.nlines);
+  if (nf == 3) cwc_solve<3>(C, S, dir, l, sys, chain);
+  else cwc_solve<4>(C, S, dir, l, sys, chain);
+}
+
 // Blocks of a sweep for the team-based reconstruction (each block holds
 // 256/team whole teams; a block never mixes sweeps).
 __host__ __device__ inline int recon_sweep_blocks(const SweepGeom &S, int nv, int paired) {
@@ -1932,9 +2266,13 @@ __global__ void __launch_bounds__(TB) k_weno_weights(int kind, double c1, double
 // host side: work decomposition and launchers
 // ===========================================================================
 
-static const char *kKNames[K_NCLASS] = {"stage_bc", "ghost", "flux", "recon_ends", "recon", "recon_char", "upwind",
-                                        "bflux", "qderiv", "viscflux", "assemble", "update"};
+static const char *kKNames[K_NCLASS] = {"stage_bc", "ghost",  "flux",    "recon_ends", "recon",
+                                        "recon_char", "cw_rows", "cw_solve", "upwind", "bflux",
+                                        "qderiv",   "viscflux", "assemble", "update"};
 const char *kclass_name(int k) { return kKNames[k]; }
+
+int cw_row_fields(int nflow, int nv) { return nflow * nflow + 16 * nv; }
+int cw_back_fields(int nflow, int nv) { return 2 * nflow * nflow + nflow + 3 * (nv - nflow); }
 
 bool team_shape(int N, int max_rows, int &team, int &rows) {
   if (max_rows < 1 || max_rows > kMaxRows) max_rows = kMaxRows;
@@ -1952,7 +2290,13 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
   blocks[K_STAGE] = two ? ((G.nx + 31) / 32) * ((G.ny + 31) / 32) * G.nv : nb((long)G.nx * G.nv);
   blocks[K_GHOST] = nb(ghost_points(G) * G.nv);
   blocks[K_FLUX] = nb((long)(c.Sx.N + 2 * c.Sx.g) * c.Sx.nlines + (two ? (long)(c.Sy.N + 2 * c.Sy.g) * c.Sy.nlines : 0));
-  if (c.R.characteristic) {
+  blocks[K_CWROWS] = blocks[K_CWSOLVE] = 0;
+  if (c.R.characteristic && c.R.crweno) {
+    const int nf = c.model == 1 ? 3 : 4;
+    blocks[K_ENDS] = blocks[K_RECON] = blocks[K_RCHAR] = 0;
+    blocks[K_CWROWS] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
+    blocks[K_CWSOLVE] = nb(cw_chains(c.Sx, G.nv, nf) + (two ? cw_chains(c.Sy, G.nv, nf) : 0));
+  } else if (c.R.characteristic) {
     blocks[K_ENDS] = 0;
     blocks[K_RECON] = 0;
     blocks[K_RCHAR] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
@@ -1988,6 +2332,8 @@ void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, void 
   run(k_recon_ends, K_ENDS, d, pl, s, st, mark, ctx);
   run(k_recon, K_RECON, d, pl, s, st, mark, ctx);
   run(k_recon_char, K_RCHAR, d, pl, s, st, mark, ctx);
+  run(k_cw_rows, K_CWROWS, d, pl, s, st, mark, ctx);
+  run(k_cw_solve, K_CWSOLVE, d, pl, s, st, mark, ctx);
   run(k_upwind, K_UPWIND, d, pl, s, st, mark, ctx);
   run(k_bflux, K_BFLUX, d, pl, s, st, mark, ctx);
   run(k_qderiv, K_QDERIV, d, pl, s, st, mark, ctx);

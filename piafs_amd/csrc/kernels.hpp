@@ -112,12 +112,15 @@ struct CaseDev {
   int cons;
   double dS[2];
   double *bflux, *tbi;
+  // characteristic CRWENO5: block rows of each sweep [field][line][k] and the
+  // back-substitution data of its four systems [field][system][line][k]
+  double *cwr[2], *cwb[2];
   int *status;   // bit 0: a zero pivot met in a line solve (tridiagLU.c:139-142 returns -1)
 };
 
 // Kernel classes launched per RK stage (plus the final update).
 enum KClass {
-  K_STAGE = 0, K_GHOST, K_FLUX, K_ENDS, K_RECON, K_RCHAR, K_UPWIND, K_BFLUX, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE,
+  K_STAGE = 0, K_GHOST, K_FLUX, K_ENDS, K_RECON, K_RCHAR, K_CWROWS, K_CWSOLVE, K_UPWIND, K_BFLUX, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE,
   K_NCLASS
 };
 const char *kclass_name(int k);
@@ -132,6 +135,13 @@ struct Plan {
 
 // Host side: blocks each case needs for each kernel class (0 = class unused).
 void case_blocks(const CaseDev &c, int blocks[K_NCLASS]);
+// doubles per interface of the characteristic CRWENO5 scratch of one sweep:
+// block rows (the flow block of L, then a, b, c coefficients and the right-hand
+// side of every variable of the four systems), back-substitution data (four
+// systems: flow block inverse, super-diagonal block and x'; per species the
+// reciprocal pivot, c and x')
+int cw_row_fields(int nflow, int nv);
+int cw_back_fields(int nflow, int nv);
 
 // Stage s of the RK step for every case in the plan: stage state + BCs,
 // fluxes, reconstruction, upwinding, divergence, viscous terms, assembly of

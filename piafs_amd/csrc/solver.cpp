@@ -76,6 +76,10 @@ size_t Case::plan_bytes(const Config &c) {
   size_t ix = nv * (size_t)(c.dim[0] + 1) * (c.ndims == 2 ? c.dim[1] : 1);
   size_t iy = c.ndims == 2 ? nv * (size_t)(c.dim[1] + 1) * c.dim[0] : 0;
   n += (paired_mode(c) ? 3 : 5) * (ix + iy);
+  if (c.characteristic && c.crweno) {
+    const int nf = c.model_id == MODEL_EULER1D ? 3 : 4;
+    n += (size_t)(cw_row_fields(nf, (int)nv) + 4 * cw_back_fields(nf, (int)nv)) * (ix + iy) / nv;
+  }
   if (c.model_id == MODEL_NS2D) n += 12 * (size_t)g.nxg * g.nyg;
   n += plane + c.x.size() + 4096;
   return n * sizeof(double);
@@ -206,6 +210,17 @@ int Case::setup(int dev, hipStream_t st) {
   D.bcvals = dalloc(zvals.size() + 1, st);
   D.imap = dalloc(G.plane, st);
   D.time = dalloc(1, st);
+  // characteristic CRWENO5 scratch (block rows and back-substitution data per sweep)
+  if (cfg.characteristic && cfg.crweno) {
+    const int nf = cfg.model_id == MODEL_EULER1D ? 3 : 4;
+    const SweepGeom *sw[2] = {&D.Sx, &D.Sy};
+    for (int d = 0; d < cfg.ndims; d++) {
+      const size_t ni = (size_t)(sw[d]->N + 1) * sw[d]->nlines;
+      D.cwr[d] = dalloc((size_t)cw_row_fields(nf, nv) * ni, st);
+      D.cwb[d] = dalloc((size_t)cw_back_fields(nf, nv) * 4 * ni, st);
+      if (!D.cwr[d] || !D.cwb[d]) return fail(-2, "device allocation failed");
+    }
+  }
   // conservation check (InitialSolution.c:115-135, TimePostStep.c:68-80)
   D.cons = (cfg.conservation_check == "yes") ? 1 : 0;
   if (D.cons) {

@@ -63,8 +63,7 @@ def test_reference_example_decks_parse(example):
     (lambda dk: dk.solver.__setitem__("time_scheme_type", "rk5"), "not a supported multi-stage"),
     (lambda dk: dk.physics.__setitem__("upwinding", "hllc"), "not a valid upwinding"),
     (lambda dk: dk.solver.__setitem__("par_space_scheme", "3"), "parabolic"),
-    (lambda dk: (dk.solver.__setitem__("hyp_space_scheme", "crweno5"),
-                 dk.solver.__setitem__("hyp_interp_type", "characteristic")), "supported for weno5"),
+    (lambda dk: dk.solver.__setitem__("hyp_interp_type", "upwind-char"), "not a supported interpolation"),
     (lambda dk: dk.boundary.__setitem__(0, ("thermal-wall", 0, 1, 0, 0, 0, 0)), "not supported"),
 ])
 def test_deck_errors_are_reported(mutate, msg):

@@ -91,7 +91,8 @@ FIXTURES = [
     "ns2d_chem_supersonic_jet_char_rf_16x20", "ns2d_chem_subsonic_jet_char_llf_16x20",
     "ns2d_dirichlet_ambivalent_sponge_rusanov_24x20", "riemann2d_char_rusanov_33_30",
     "riemann2d_comp_llf_crweno_33_30", "sod_slipwall_char_llf_N64", "sod_dirichlet_crweno_rusanov_N80",
-    "sod_sponge_comp_llf_N64", "sod_char_rusanov_N64",
+    "sod_sponge_comp_llf_N64", "sod_char_rusanov_N64", "sod_crweno_char_roe_N64", "sod_crweno_char_rf_mapped_N48",
+    "ns2d_crweno_char_flow_20x16", "ns2d_chem_crweno_char_rf_flow_16x12", "ns2d_crweno_char_llf_mapped_inout_20x16",
 ]
 
 
