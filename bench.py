@@ -229,6 +229,48 @@ def bench_serve(tmp, n_requests=24, rate=None, rate_frac=0.7, batched_rate=None)
     return out
 
 
+SCHEME_VARIANTS = [  # (name, hyp_space_scheme, hyp_interp_type, upwinding)
+    ("weno5_components_roe", "weno5", "components", "roe"),
+    ("weno5_characteristic_rf", "weno5", "characteristic", "rf-char"),
+    ("crweno5_components_llf", "crweno5", "components", "llf-char"),
+    ("crweno5_characteristic_roe", "crweno5", "characteristic", "roe"),
+]
+
+
+def bench_schemes(tmp, nx=512, steps=20, warmup=2):
+    """The interpolation / upwinding variants beyond the headline deck, on one
+    moving-flow 2D deck with chemistry (periodic box, uniform flow (0.3, 0.2)
+    plus a pressure bump: the reference's pivot-free block inverse of
+    characteristic CRWENO5 needs a moving gas).  Throughput and the per-class
+    kernel times of each variant."""
+    import piafs_amd
+    from piafs_amd import decks
+    periodic = [("periodic", 0, 1, 0, 0, -1e99, 1e99), ("periodic", 0, -1, 0, 0, -1e99, 1e99),
+                ("periodic", 1, 1, -1e99, 1e99, 0, 0), ("periodic", 1, -1, -1e99, 1e99, 0, 0)]
+    out = {"grid": [nx, nx], "steps": steps, "deck": "periodic box, chemistry (nvars 11), Re 1000, RK4, "
+           "flow (1, 0.3, 0.2) + 5 % pressure bump, dt 5e-4"}
+    for name, scheme, interp, upw in SCHEME_VARIANTS:
+        dk = decks.ns2d_open_box(nx=nx, ny=nx, n_iter=warmup + steps, dt=5e-4, flow=(1.0, 0.3, 0.2, None),
+                                 scheme=scheme, interp=interp, upwinding=upw, include_chemistry=True, Re=1000.0,
+                                 boundary=periodic)
+        d = os.path.join(tmp, "scheme_" + name)
+        dk.write(d)
+        with piafs_amd.Case(d, device=0) as c:
+            c.step(warmup)
+            c.sync()
+            c.profile(True)
+            t0 = time.perf_counter()
+            c.step(steps)
+            c.sync()
+            secs = time.perf_counter() - t0
+            kt = c.kernel_times()
+            c.profile(False)
+            finite = bool(np.isfinite(c.solution()).all())
+        out[name] = {"cell_updates_per_s": nx * nx * steps / secs, "ms_per_step": 1e3 * secs / steps,
+                     "kernel_ms": kt, "finite": finite}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,6 +286,7 @@ def main():
                     help="keep the default line-solve segment length (for profiling runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the ensemble leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-schemes", action="store_true", help="skip the interpolation / upwinding variants leg")
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--serve-requests", type=int, default=24)
@@ -338,6 +381,8 @@ def main():
         if not args.no_serve:
             br = line.get("batched", {}).get("batched_cell_updates_per_s")
             line["serve"] = bench_serve(tmp, n_requests=args.serve_requests, batched_rate=br)
+        if not args.no_schemes:
+            line["schemes"] = bench_schemes(tmp)
         if not args.no_cpu_baseline:
             line["cpu_baseline"] = cpu_baseline(tmp)
     print(json.dumps(line), flush=True)

@@ -1182,20 +1182,29 @@ __global__ void __launch_bounds__(256) k_recon_char(CASE_ARGS) {
 // reference's operation order.
 // ---------------------------------------------------------------------------
 
-// block rows of every system at interface k of line l: one thread each
+// block rows of every system at interface k of line l, one thread each, the
+// line index fastest: a wave reads the same position of consecutive lines,
+// contiguous in the other layout (the x sweep reads the transposed copy, the
+// y sweep the natural one), and computes the flux there as k_flux does
 template <int MODEL, int DIR, int KIND>
-__device__ __forceinline__ void cwc_rows(const CaseDev &C, const SweepGeom &S, int l, int k, const double *q,
-                                         const double *f, double *rows) {
+__device__ __forceinline__ void cwc_rows(const CaseDev &C, const SweepGeom &S, int l, int k, double *rows) {
   constexpr int NF = MODEL == 1 ? 3 : 4;
   const int nv = C.G.nv, nl = S.nlines, N = S.N;
-  const long pl = S.plane;
   const double gam = C.P.gamma;
-  const double *ql = q + S.base + (long)l * S.line_stride, *fl = f + S.base + (long)l * S.line_stride;
+  // state at position pos of line l
+  const double *q;
+  long o0, ostr, pl;
+  if (MODEL == 1) {
+    q = C.U; o0 = S.base + (long)l * S.line_stride; ostr = 1; pl = S.plane;
+  } else {
+    const SweepGeom &O = DIR ? C.Sx : C.Sy;
+    q = DIR ? C.U : C.UT; o0 = O.base + l; ostr = O.line_stride; pl = O.plane;
+  }
+  auto at = [&](int v, int pos) { return q[(long)v * pl + o0 + (long)pos * ostr]; };
   double L[NF][NF];
   if (MODEL == 1) {
     double a0, a1, a2;
-    euler1d_roeavg(gam, ql[k - 1], ql[pl + k - 1], ql[2 * pl + k - 1], ql[k], ql[pl + k], ql[2 * pl + k], a0, a1,
-                   a2);
+    euler1d_roeavg(gam, at(0, k - 1), at(1, k - 1), at(2, k - 1), at(0, k), at(1, k), at(2, k), a0, a1, a2);
     Eig3 E;
     euler1d_eig(gam, a0, a1, a2, E);
 #pragma unroll
@@ -1205,8 +1214,8 @@ __device__ __forceinline__ void cwc_rows(const CaseDev &C, const SweepGeom &S, i
   } else {
     // the reference averages (k-1, k) for the left bias and (k, k-1) for the
     // right one; the Roe average is symmetric to the bit
-    const double uLc[4] = {ql[k - 1], ql[pl + k - 1], ql[2 * pl + k - 1], ql[3 * pl + k - 1]};
-    const double uRc[4] = {ql[k], ql[pl + k], ql[2 * pl + k], ql[3 * pl + k]};
+    const double uLc[4] = {at(0, k - 1), at(1, k - 1), at(2, k - 1), at(3, k - 1)};
+    const double uRc[4] = {at(0, k), at(1, k), at(2, k), at(3, k)};
     double ua[4], L4[4][4], R4[4][4];
     ns2d_roeavg(gam, uLc, uRc, ua);
     ns2d_eigvecs<DIR>(gam, ua, L4, R4);
@@ -1215,7 +1224,7 @@ __device__ __forceinline__ void cwc_rows(const CaseDev &C, const SweepGeom &S, i
 #pragma unroll
       for (int j = 0; j < NF; j++) L[i][j] = L4[i][j];
   }
-  const long vs = (long)nl * (N + 1), ii = (long)l * (N + 1) + k;
+  const long vs = (long)nl * (N + 1), ii = (long)k * nl + l;
 #pragma unroll
   for (int i = 0; i < NF; i++)
 #pragma unroll
@@ -1251,34 +1260,60 @@ __device__ __forceinline__ void cwc_rows(const CaseDev &C, const SweepGeom &S, i
     rows[b + 2 * vs] = cc;
     rows[b + 3 * vs] = w1 * f1 + w2 * f2 + w3 * f3;
   };
+  // flow state, flux and characteristic values at k-3 .. k+2
+  double cu[6][NF], cf[6][NF], vn[6];
 #pragma unroll
-  for (int src = 0; src < 2; src++) {  // 0: the solution (uL, uR), 1: the flux (fL, fR)
-    const double *a = src ? fl : ql;
-    double c[6][NF];  // characteristic values at k-3 .. k+2
+  for (int p = 0; p < 6; p++) {
+    const int pos = k - 3 + p;
+    double x[NF], fx[NF];
 #pragma unroll
-    for (int p = 0; p < 6; p++) {
-      const int pos = k - 3 + p;
-      double x[NF];
-#pragma unroll
-      for (int j = 0; j < NF; j++) x[j] = a[j * pl + pos];
-#pragma unroll
-      for (int v = 0; v < NF; v++) {
-        double acc = L[v][0] * x[0];
-#pragma unroll
-        for (int j = 1; j < NF; j++) acc = acc + L[v][j] * x[j];
-        c[p][v] = acc;
-      }
+    for (int j = 0; j < NF; j++) x[j] = at(j, pos);
+    if (MODEL == 1) {
+      const double rho = x[0], v = x[1] / rho, e = x[2], P_ = (e - 0.5 * rho * v * v) * (gam - 1.0);
+      fx[0] = rho * v;
+      fx[1] = rho * v * v + P_;
+      fx[2] = (e + P_) * v;
+      vn[p] = v;
+    } else {
+      const double rho = x[0], vx = (rho == 0) ? 0 : x[1] / rho, vy = (rho == 0) ? 0 : x[2] / rho, e = x[3];
+      const double vsq = (vx * vx) + (vy * vy), P_ = (e - 0.5 * rho * vsq) * (gam - 1.0);
+      const double v = DIR == 0 ? vx : vy;
+      fx[0] = rho * v;
+      if (DIR == 0) { fx[1] = rho * vx * vx + P_; fx[2] = rho * vx * vy; }
+      else { fx[1] = rho * vy * vx; fx[2] = rho * vy * vy + P_; }
+      fx[NF - 1] = (e + P_) * v;
+      vn[p] = v;
     }
 #pragma unroll
     for (int v = 0; v < NF; v++) {
-      row(2 * src, v, c[0][v], c[1][v], c[2][v], c[3][v], c[4][v]);
-      row(2 * src + 1, v, c[5][v], c[4][v], c[3][v], c[2][v], c[1][v]);
+      double au = L[v][0] * x[0], af = L[v][0] * fx[0];
+#pragma unroll
+      for (int j = 1; j < NF; j++) {
+        au = au + L[v][j] * x[j];
+        af = af + L[v][j] * fx[j];
+      }
+      cu[p][v] = au;
+      cf[p][v] = af;
     }
-    for (int m = NF; m < nv; m++) {
-      const double *am = a + m * pl + k;
-      row(2 * src, m, am[-3], am[-2], am[-1], am[0], am[1]);
-      row(2 * src + 1, m, am[2], am[1], am[0], am[-1], am[-2]);
+  }
+#pragma unroll
+  for (int v = 0; v < NF; v++) {
+    row(0, v, cu[0][v], cu[1][v], cu[2][v], cu[3][v], cu[4][v]);
+    row(1, v, cu[5][v], cu[4][v], cu[3][v], cu[2][v], cu[1][v]);
+    row(2, v, cf[0][v], cf[1][v], cf[2][v], cf[3][v], cf[4][v]);
+    row(3, v, cf[5][v], cf[4][v], cf[3][v], cf[2][v], cf[1][v]);
+  }
+  for (int m = NF; m < nv; m++) {  // passive scalars: identity rows, flux vn u_m
+    double x[6], fx[6];
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+      x[p] = at(m, k - 3 + p);
+      fx[p] = vn[p] * x[p];
     }
+    row(0, m, x[0], x[1], x[2], x[3], x[4]);
+    row(1, m, x[5], x[4], x[3], x[2], x[1]);
+    row(2, m, fx[0], fx[1], fx[2], fx[3], fx[4]);
+    row(3, m, fx[5], fx[4], fx[3], fx[2], fx[1]);
   }
 }
 
@@ -1291,13 +1326,12 @@ __global__ void __launch_bounds__(256) k_cw_rows(CASE_ARGS) {
   const int dir = t < nX ? 0 : 1;
   const SweepGeom &S = dir ? C.Sy : C.Sx;
   const long u = dir ? t - nX : t;
-  const int k = (int)(u % (S.N + 1)), l = (int)(u / (S.N + 1));
-  const double *q = dir ? C.UT : C.U, *f = dir ? C.flux_y : C.flux_x;
+  const int l = (int)(u % S.nlines), k = (int)(u / S.nlines);
   double *rows = C.cwr[dir];
-#define CWROWS(KIND)                                                        \
-  (C.model == 1 ? cwc_rows<1, 0, KIND>(C, S, l, k, q, f, rows)             \
-                : (dir ? cwc_rows<2, 1, KIND>(C, S, l, k, q, f, rows)      \
-                       : cwc_rows<2, 0, KIND>(C, S, l, k, q, f, rows)))
+#define CWROWS(KIND)                                                \
+  (C.model == 1 ? cwc_rows<1, 0, KIND>(C, S, l, k, rows)           \
+                : (dir ? cwc_rows<2, 1, KIND>(C, S, l, k, rows)    \
+                       : cwc_rows<2, 0, KIND>(C, S, l, k, rows)))
   switch (C.R.kind) {
     case 1: CWROWS(1); break;
     case 2: CWROWS(2); break;
@@ -1347,12 +1381,14 @@ template <int NF>
 __device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, int dir, int l, int sys,
                                           int chain) {
   const int nv = C.G.nv, nl = S.nlines, N = S.N;
-  const long vs = (long)nl * (N + 1), ib = (long)l * (N + 1);
-  const long bs = 4L * nl * (N + 1), bb = ((long)sys * nl + l) * (N + 1);
+  // rows [field][k][line], back-substitution data [field][system][k][line]
+  const long vs = (long)nl * (N + 1);
+  const long bs = 4L * nl * (N + 1), bb = (long)sys * (N + 1) * nl + l;
   const double *rows = C.cwr[dir];
   double *back = C.cwb[dir];
   double *out = (dir ? C.iy : C.ix)[sys];
-  auto rf = [&](int f, int k) { return rows[(long)f * vs + ib + k]; };
+  auto rf = [&](int f, int k) { return rows[(long)f * vs + (long)k * nl + l]; };
+  auto bk = [&](int f, int k) -> double & { return back[(long)f * bs + bb + (long)k * nl]; };
   auto coef = [&](int v, int which, int k) { return rf(NF * NF + (sys * nv + v) * 4 + which, k); };
   if (chain > 0) {  // a species: a scalar chain of the same arithmetic
     const int m = NF + chain - 1, f0 = 2 * NF * NF + NF + 3 * (chain - 1);
@@ -1364,9 +1400,9 @@ __device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, 
         xp = F;
       } else {
         const double rb = 1.0 / bp, fac = a * rb;
-        back[(long)f0 * bs + bb + k - 1] = rb;
-        back[(long)(f0 + 1) * bs + bb + k - 1] = cprev;
-        back[(long)(f0 + 2) * bs + bb + k - 1] = xp;
+        bk(f0, k - 1) = rb;
+        bk(f0 + 1, k - 1) = cprev;
+        bk(f0 + 2, k - 1) = xp;
         bp = b - fac * cprev;
         xp = F - fac * xp;
       }
@@ -1375,8 +1411,7 @@ __device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, 
     double x = (1.0 / bp) * xp;
     out[((long)m * nl + l) * (N + 1) + N] = x;
     for (int k = N - 1; k >= 0; k--) {
-      const double rb = back[(long)f0 * bs + bb + k], c = back[(long)(f0 + 1) * bs + bb + k];
-      const double xq = back[(long)(f0 + 2) * bs + bb + k];
+      const double rb = bk(f0, k), c = bk(f0 + 1, k), xq = bk(f0 + 2, k);
       x = rb * (xq - c * x);
       out[((long)m * nl + l) * (N + 1) + k] = x;
     }
@@ -1424,10 +1459,10 @@ __device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, 
       for (int i = 0; i < NF; i++) {
 #pragma unroll
         for (int j = 0; j < NF; j++) {
-          back[(long)(i * NF + j) * bs + bb + k - 1] = binv[i][j];
-          back[(long)(NF * NF + i * NF + j) * bs + bb + k - 1] = cprev[i][j];
+          bk(i * NF + j, k - 1) = binv[i][j];
+          bk(NF * NF + i * NF + j, k - 1) = cprev[i][j];
         }
-        back[(long)(2 * NF * NF + i) * bs + bb + k - 1] = xp[i];
+        bk(2 * NF * NF + i, k - 1) = xp[i];
       }
       double nx[NF];
 #pragma unroll
@@ -1468,16 +1503,16 @@ __device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, 
     double y[NF];
 #pragma unroll
     for (int i = 0; i < NF; i++) {
-      double r = back[(long)(2 * NF * NF + i) * bs + bb + k];
+      double r = bk(2 * NF * NF + i, k);
 #pragma unroll
-      for (int j = 0; j < NF; j++) r -= (back[(long)(NF * NF + i * NF + j) * bs + bb + k] * x[j]);
+      for (int j = 0; j < NF; j++) r -= (bk(NF * NF + i * NF + j, k) * x[j]);
       y[i] = r;
     }
 #pragma unroll
     for (int i = 0; i < NF; i++) {
       double acc = 0;
 #pragma unroll
-      for (int j = 0; j < NF; j++) acc += (back[(long)(i * NF + j) * bs + bb + k] * y[j]);
+      for (int j = 0; j < NF; j++) acc += (bk(i * NF + j, k) * y[j]);
       x[i] = acc;
     }
 #pragma unroll
