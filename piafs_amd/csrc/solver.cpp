@@ -838,11 +838,15 @@ int piafs_solve(piafs_case *pc, const char *dir, int verbose, double *seconds) {
     if (iter == 0 && output()) return case_fail(pc, 2);
     const double t_start = c->t;
     bool screen = ((iter + 1) % cf.screen_op_iter == 0);
+    const auto t_it = std::chrono::steady_clock::now();
     int rc = screen && verbose ? piafs_step_diag(pc, 1, cfl.data(), norm.data()) : piafs_step(pc, 1);
     if (rc) return rc;
     steps++;
     stage_time = t_start + cf.c[cf.nstages - 1] * cf.dt;
-    if (screen && verbose) printf("iter=%7d  t=%1.3E  CFL=%1.3E  norm=%1.4E\n", iter + 1, c->t, cfl[0], norm[0]);
+    if (screen && verbose) {  // TimePrintStep's line; the wall time is the host's view of an asynchronous step
+      const double wc = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_it).count();
+      printf("iter=%7d  t=%1.3E  CFL=%1.3E  norm=%1.4E  wctime: %1.1E (s)  \n", iter + 1, c->t, cfl[0], norm[0], wc);
+    }
     if (((iter + 1) % cf.file_op_iter == 0) && ((iter + 1) < cf.n_iter) && output()) return case_fail(pc, 2);
   }
   int rc = piafs_sync(pc);

@@ -290,6 +290,60 @@ def ns2d_open_box(nx: int = 24, ny: int = 20, n_iter: int = 20, dt: float = 0.00
     return Deck(solver, boundary, physics, [x, y], u, weno, chem, imap)
 
 
+def example_2d_initial(name: str, d: str, NI: int, NJ: int, gamma: float = 1.4) -> None:
+    """initial.inp of Examples/2D_NavierStokes/<name> (Riemann2DCase4 or
+    RadialExpansionWave) as the example's aux/init.c writes it: ASCII with
+    "%lf " (six decimals), x then y, then each variable with x fastest."""
+    X = np.zeros((NJ, NI))
+    Y = np.zeros((NJ, NI))
+    if name == "Riemann2DCase4":
+        dx, dy = 1.0 / (NI - 1), 1.0 / (NJ - 1)
+        x = np.array([-0.5 + i * dx for i in range(NI)])
+        y = np.array([-0.5 + j * dy for j in range(NJ)])
+    elif name == "RadialExpansionWave":
+        dx, dy = 8.0 / (NI - 1), 8.0 / (NJ - 1)
+        x = np.array([-4.0 + i * dx for i in range(NI)])
+        y = np.array([-4.0 + j * dy for j in range(NJ)])
+    else:
+        raise ValueError(name)
+    u = np.zeros((4, NJ, NI))
+    for j in range(NJ):
+        for i in range(NI):
+            xi, yj = x[i], y[j]
+            if name == "Riemann2DCase4":
+                if xi < 0 and yj < 0:
+                    rho, P, uu, vv = 1.1, 1.1, 0.8939, 0.8939
+                elif xi >= 0 and yj < 0:
+                    rho, P, uu, vv = 0.5065, 0.35, 0.0, 0.8939
+                elif xi < 0 and yj >= 0:
+                    rho, P, uu, vv = 0.5065, 0.35, 0.8939, 0.0
+                else:
+                    rho, P, uu, vv = 1.1, 1.1, 0.0, 0.0
+            else:
+                r = math.sqrt(xi * xi + yj * yj)
+                if r < 0.5:
+                    q = 0.0
+                elif r < 1.5:
+                    with np.errstate(divide="ignore"):  # r = 0.5 divides by zero, as C does (tanh(-inf))
+                        arg = np.float64(r - 1) / np.float64(0.25 - (r - 1) * (r - 1))
+                    q = (1.0 / gamma) * (1.0 + math.tanh(arg))
+                else:
+                    q = 2.0 / gamma
+                uu, vv = (0.0, 0.0) if r == 0 else (xi * q / r, yj * q / r)
+                a = 1.0 - (gamma - 1.0) * q / 2.0
+                rho = gamma * math.exp((2 / (gamma - 1.0)) * math.log(a))
+                P = rho * a * a / gamma
+            u[0, j, i] = rho
+            u[1, j, i] = rho * uu
+            u[2, j, i] = rho * vv
+            u[3, j, i] = P / (gamma - 1.0) + 0.5 * rho * (uu * uu + vv * vv)
+    with open(os.path.join(d, "initial.inp"), "w") as f:
+        f.write("".join("%f " % v for v in x) + "\n")
+        f.write("".join("%f " % v for v in y) + "\n")
+        for k in range(4):
+            f.write("".join("%f " % v for v in u[k].reshape(-1)) + "\n")
+
+
 def read_binary_op(path: str):
     """Read a WriteBinary file (src/IOFunctions/WriteBinary.c:37-93):
     int32 ndims, int32 nvars, int32 dim[ndims], f64 x[sum dim], f64 u[prod dim * nvars]."""

@@ -288,6 +288,61 @@ def outputs():
         json.dump(meta, f, indent=1)
 
 
+EXAMPLES_2D = {"Riemann2DCase4": (201, 201), "RadialExpansionWave": (65, 65)}
+DECK_FILES = ("solver.inp", "boundary.inp", "physics.inp", "weno.inp")
+
+
+def read_tecplot(path):
+    """A WriteTecplot2D file (WriteTecplot2D.c:22-80): the point data, x fastest."""
+    rows = [l.split() for l in open(path).read().splitlines()[2:] if l.strip()]
+    return np.array([[float(t) for t in r[4:]] for r in rows])
+
+
+def screen_rows(log):
+    """The screen lines of TimePrintStep: iteration, time, CFL, norm."""
+    out = []
+    for line in log.splitlines():
+        m = re.match(r"\s*iter=\s*(\d+)\s+t=(\S+)\s+CFL=(\S+)\s+norm=(\S+)", line)
+        if m:
+            out.append([int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(4))])
+    return out
+
+
+def examples_2d():
+    """Run the shipped 2D example decks (their .inp files, the initial state of
+    their aux/init.c) through the reference; keep the decks, the screen rows,
+    conservation.dat and a reduction of the final solution file."""
+    meta = json.load(open(os.path.join(HERE, "examples.json")))
+    arrays = {}
+    for ex, (ni, nj) in EXAMPLES_2D.items():
+        src = os.path.join(REF_TREE, "Examples", "2D_NavierStokes", ex)
+        dst = os.path.join(HERE, "examples", ex)
+        os.makedirs(dst, exist_ok=True)
+        for fn in DECK_FILES:
+            shutil.copy(os.path.join(src, fn), os.path.join(dst, fn))
+        with tempfile.TemporaryDirectory() as d:
+            for fn in DECK_FILES:
+                shutil.copy(os.path.join(src, fn), os.path.join(d, fn))
+            decks.example_2d_initial(ex, d, ni, nj)
+            r = subprocess.run([REF_BIN], cwd=d, capture_output=True, text=True)
+            if r.returncode:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            ops = sorted(f for f in os.listdir(d) if f.startswith("op_") and f.endswith(".dat"))
+            U = read_tecplot(os.path.join(d, ops[-1])).reshape(nj, ni, 4)
+            cons = [float(v) for v in open(os.path.join(d, "conservation.dat")).read().split()[5:]]
+        arrays[ex + "_norm"] = np.sqrt((U.reshape(-1, 4) ** 2).sum(0))
+        arrays[ex + "_sub"] = U[::4, ::4, :].copy()
+        arrays[ex + "_rowsum"], arrays[ex + "_rowabs"] = U.sum(1), np.abs(U).sum(1)
+        arrays[ex + "_colsum"], arrays[ex + "_colabs"] = U.sum(0), np.abs(U).sum(0)
+        meta[ex] = {"rows": screen_rows(r.stdout), "files": ops, "conservation": cons, "size": [ni, nj],
+                    "source": f"Examples/2D_NavierStokes/{ex} (decks; initial state of aux/init.c), "
+                              "run by the reference built from source"}
+        print(ex, len(meta[ex]["rows"]), "screen rows,", len(ops), "solution files")
+    np.savez_compressed(os.path.join(HERE, "examples2d_ref.npz"), **arrays)
+    with open(os.path.join(HERE, "examples.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 2 and sys.argv[1] == "--config2-full":
         config2_full(sys.argv[2])
@@ -295,6 +350,8 @@ if __name__ == "__main__":
         config5_nochem(sys.argv[2])
     elif len(sys.argv) > 1 and sys.argv[1] == "--outputs":
         outputs()
+    elif len(sys.argv) > 1 and sys.argv[1] == "--examples-2d":
+        examples_2d()
     else:
         ref_runs()
         example_logs()

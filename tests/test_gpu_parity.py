@@ -136,6 +136,50 @@ def test_shipped_logs(examples, example):
         assert _close_in_last_digit(norm[it - 1], nm, "%1.4E"), (it, norm[it - 1], nm)
 
 
+@pytest.mark.parametrize("example", ["Riemann2DCase4", "RadialExpansionWave"])
+def test_shipped_2d_examples_through_the_driver(examples, example):
+    """A shipped 2D example as a user runs it: its own decks (characteristic
+    WENO5 with mapped weights, rf-char, SSPRK3, mirror BCs, conservation check,
+    Tecplot output every 25 / 10 steps) and the initial state its aux/init.c
+    writes, through the command-line driver.  Against the reference's run of
+    the same directory: every screen line to its printed digits, the file
+    names, the conservation errors, and the final solution (norms, a stride-4
+    subsample, row and column sums)."""
+    import shutil
+    import subprocess
+    ref = examples[example]
+    ni, nj = ref["size"]
+    z = np.load(os.path.join(GOLDEN, "examples2d_ref.npz"))
+    with tempfile.TemporaryDirectory() as d:
+        for fn in ("solver.inp", "boundary.inp", "physics.inp", "weno.inp"):
+            shutil.copy(os.path.join(GOLDEN, "examples", example, fn), os.path.join(d, fn))
+        decks.example_2d_initial(example, d, ni, nj)
+        r = subprocess.run([os.path.join(ROOT, "piafs_amd", "bin", "piafs_amd"), d], capture_output=True,
+                           text=True, timeout=600)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        ops = sorted(f for f in os.listdir(d) if f.startswith("op_") and f.endswith(".dat"))
+        assert ops == ref["files"]
+        lines = [l.split() for l in open(os.path.join(d, ops[-1])).read().splitlines()[2:] if l.strip()]
+        U = np.array([[float(t) for t in l[4:]] for l in lines]).reshape(nj, ni, 4)
+        cons = [float(v) for v in open(os.path.join(d, "conservation.dat")).read().split()[5:]]
+    rows = {}
+    for line in r.stdout.splitlines():
+        t = line.split()
+        if t and t[0].startswith("iter="):
+            rows[int(line.split("iter=")[1].split()[0])] = (float(line.split("CFL=")[1].split()[0]),
+                                                            float(line.split("norm=")[1].split()[0]))
+    for it, _, cf, nm in ref["rows"]:
+        assert _close_in_last_digit(rows[it][0], cf, "%1.3E"), (it, rows[it][0], cf)
+        assert _close_in_last_digit(rows[it][1], nm, "%1.4E"), (it, rows[it][1], nm)
+    np.testing.assert_allclose(cons, ref["conservation"], rtol=0, atol=1e-11)
+    norm = np.sqrt((U.reshape(-1, 4) ** 2).sum(0))
+    assert np.all(np.abs(norm - z[example + "_norm"]) <= 1e-12 * z[example + "_norm"])
+    sub = U[::4, ::4, :].reshape(-1, 4)
+    assert np.all(_rel(sub, z[example + "_sub"].reshape(-1, 4)) <= TOL_STEPS)
+    assert np.all(_sum_err(U.sum(1), z[example + "_rowsum"], z[example + "_rowabs"]) <= TOL_STEPS)
+    assert np.all(_sum_err(U.sum(0), z[example + "_colsum"], z[example + "_colabs"]) <= TOL_STEPS)
+
+
 def test_vortex_errors_match_errors_dat(examples):
     ref = examples["InviscidVortexConvection"]["errors"]
     dk = decks.isentropic_vortex(N=60, n_iter=400, dt=0.05)
