@@ -234,6 +234,7 @@ SCHEME_VARIANTS = [  # (name, hyp_space_scheme, hyp_interp_type, upwinding)
     ("weno5_characteristic_rf", "weno5", "characteristic", "rf-char"),
     ("crweno5_components_llf", "crweno5", "components", "llf-char"),
     ("crweno5_characteristic_roe", "crweno5", "characteristic", "roe"),
+    ("upw5_components_roe", "upw5", "components", "roe"),
 ]
 
 

@@ -31,7 +31,7 @@ typedef struct {
   double dt, t;          /* fixed step size and current simulation time */
   double gamma, Re, Pr;  /* after the photochemistry overrides */
   int    model;          /* 1 = euler1d, 2 = navierstokes2d */
-  int    scheme;         /* 0 = weno5, 1 = crweno5 */
+  int    scheme;         /* 0 = weno5, 1 = crweno5, 2 = upw5 */
   int    characteristic; /* 1 = characteristic interpolation */
   int    include_chem;
   size_t device_bytes;   /* HBM held by this case */

@@ -7,7 +7,7 @@
  * order as the reference so that gcc -O2 builds agree bit for bit.
  *
  * Supported (the hot path): models euler1d / navierstokes2d (+ photochemistry),
- * hyp_space_scheme weno5 / crweno5, hyp_interp_type components and
+ * hyp_space_scheme weno5 / crweno5 / upw5, hyp_interp_type components and
  * characteristic (both models; characteristic CRWENO5 with the block solver), WENO weights JS / mapped / borges(Z) /
  * yc and no_limiting, upwinding roe / rf-char / llf-char / rusanov (both
  * models), every boundary type of boundary.inp (periodic, extrapolate,
@@ -57,7 +57,7 @@ struct orc_case {
   /* weno.inp (WENOInitialize.c:46-123) */
   int    mapped, borges, yc, no_limiting;
   double eps;
-  int    is_crweno, is_char, par4;
+  int    is_crweno, is_upw5, is_char, par4;
   /* RK tableau (TimeExplicitRKInitialize.c:20-95) */
   int    nstages;
   double A[16], b[4], c[4];
@@ -471,8 +471,10 @@ int orc_load(const char *dir, orc_case **out)
   if (!strcmp(c->par, "4")) c->par4 = 1;
   else if (!strcmp(c->par, "2")) c->par4 = 0;
   else { orc_free(c); return 5; }
+  c->is_upw5 = 0;
   if (!strcmp(c->hyp, "weno5")) c->is_crweno = 0;
   else if (!strcmp(c->hyp, "crweno5")) c->is_crweno = 1;
+  else if (!strcmp(c->hyp, "upw5")) { c->is_crweno = 0; c->is_upw5 = 1; }
   else { orc_free(c); return 5; }
   if (strcmp(c->interp, "characteristic") && strcmp(c->interp, "components")) { orc_free(c); return 5; }
   c->is_char = (c->nvars > 1 && !strcmp(c->interp, "characteristic"));
@@ -482,7 +484,8 @@ int orc_load(const char *dir, orc_case **out)
     kw tab[] = {{"mapped", 'i', &c->mapped}, {"borges", 'i', &c->borges}, {"yc", 'i', &c->yc},
                 {"no_limiting", 'i', &c->no_limiting}, {"epsilon", 'd', &c->eps}, {"p", 'd', &p},
                 {"rc", 'd', &rc}, {"xi", 'd', &xi}, {"tol", 'd', &tol}};
-    if (read_kw_file(dir, "weno.inp", tab, 9)) { orc_free(c); return 6; }
+    /* upw5 reads no weno.inp (InitializeSolvers.c:141-148) */
+    if (!c->is_upw5 && read_kw_file(dir, "weno.inp", tab, 9)) { orc_free(c); return 6; }
   }
   if (rk_init(c)) { orc_free(c); return 7; }
   /* physics (Euler1DInitialize.c:58-170, NavierStokes2DInitialize.c:68-235) */
@@ -953,6 +956,18 @@ static int weight_kind(const orc_case *c)
   return 0;
 }
 
+/* fifth-order linear upwind value, Interp1PrimFifthOrderUpwind.c:92-96 and
+ * :138-142 (the same formula on characteristic values in
+ * Interp1PrimFifthOrderUpwindChar.c:174-182) */
+static double upw5(double m3, double m2, double m1, double p1, double p2)
+{
+  static const double one_by_thirty = 1.0 / 30.0, thirteen_by_sixty = 13.0 / 60.0,
+                      fortyseven_by_sixty = 47.0 / 60.0, twentyseven_by_sixty = 27.0 / 60.0,
+                      one_by_twenty = 1.0 / 20.0;
+  return one_by_thirty * m3 - thirteen_by_sixty * m2 + fortyseven_by_sixty * m1 + twentyseven_by_sixty * p1
+         - one_by_twenty * p2;
+}
+
 /* optimal weights (WENOFifthOrderCalculateWeights.c:202-220) */
 static void optimal_weights(const orc_case *c, int k, int N, double *cw)
 {
@@ -1024,6 +1039,10 @@ static void recon_line_components(const orc_case *c, const double *fC, const dou
         } else {
           m3 = LP(src, k + 2)[v]; m2 = LP(src, k + 1)[v]; m1 = LP(src, k)[v];
           p1 = LP(src, k - 1)[v]; p2 = LP(src, k - 2)[v];
+        }
+        if (c->is_upw5) {
+          out[k * ostride + v] = upw5(m3, m2, m1, p1, p2);
+          continue;
         }
         optimal_weights(c, k, N, cw);
         if (c->no_limiting) { w[0] = cw[0]; w[1] = cw[1]; w[2] = cw[2]; }
@@ -1098,6 +1117,10 @@ static void recon_line_char(const orc_case *c, const double *fC, const double *u
           double acc = 0;
           for (j = 0; j < nv; j++) acc += L[v * nv + j] * LP(src, q[t])[j];
           s[t] = acc;
+        }
+        if (c->is_upw5) {
+          fch[v] = upw5(s[0], s[1], s[2], s[3], s[4]);
+          continue;
         }
         optimal_weights(c, k, N, cw);
         if (c->no_limiting) { w[0] = cw[0]; w[1] = cw[1]; w[2] = cw[2]; }

@@ -61,6 +61,7 @@ struct Engine {
   bool owns_cases = true;
   int device = 0;
   hipStream_t stream = nullptr;
+  StageFork fork;              // side stream of the viscous branch (null: one stream)
   CaseDev *d_cases = nullptr;
   int *d_status = nullptr;  // one status word per member case
   Plan plan;

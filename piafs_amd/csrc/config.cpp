@@ -314,6 +314,7 @@ int Config::parse(const std::string &dir, std::string &err) {
   }
   if (hyp_scheme == "weno5") crweno = false;
   else if (hyp_scheme == "crweno5") crweno = true;
+  else if (hyp_scheme == "upw5") upw5 = true;
   else {
     err = hyp_scheme + " is not a supported spatial interpolation scheme on this path";
     return 1;
@@ -334,8 +335,10 @@ int Config::parse(const std::string &dir, std::string &err) {
       else return false;
       return true;
     };
-    if (read_keywords(join(dir, "weno.inp"), on_key, err) == 1) return 1;
-    weight_kind = yc ? W_YC : borges ? W_Z : mapped ? W_MAPPED : W_JS;
+    // upw5 has no weights and the reference reads no weno.inp for it
+    // (InitializeSolvers.c:141-148 calls no WENOInitialize)
+    if (!upw5 && read_keywords(join(dir, "weno.inp"), on_key, err) == 1) return 1;
+    weight_kind = upw5 ? W_UPW5 : yc ? W_YC : borges ? W_Z : mapped ? W_MAPPED : W_JS;
   }
   // RK tableau
   {

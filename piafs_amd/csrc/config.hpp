@@ -15,7 +15,8 @@ enum BCType {
   BC_SUBSONIC_INFLOW, BC_SUBSONIC_OUTFLOW, BC_SUBSONIC_AMBIVALENT, BC_SUPERSONIC_INFLOW, BC_SUPERSONIC_OUTFLOW,
   BC_NTYPES
 };
-enum WeightKind { W_JS = 0, W_MAPPED = 1, W_Z = 2, W_YC = 3 };
+// W_UPW5: not a weight; the linear fifth-order upwind interpolation (upw5)
+enum WeightKind { W_JS = 0, W_MAPPED = 1, W_Z = 2, W_YC = 3, W_UPW5 = 4 };
 enum Upwind { UPW_ROE = 0, UPW_RF = 1, UPW_LLF = 2, UPW_RUSANOV = 3 };
 
 struct Boundary {
@@ -67,7 +68,7 @@ struct Config {
   double eps = 1e-6;
   int weight_kind = W_JS;
   // selections
-  bool crweno = false, characteristic = false, par4 = false;
+  bool crweno = false, upw5 = false, characteristic = false, par4 = false;
   // RK tableau (TimeExplicitRKInitialize.c:20-95)
   int nstages = 1;
   double A[16] = {0}, b[4] = {0}, c[4] = {0};

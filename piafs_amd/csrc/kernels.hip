@@ -92,9 +92,17 @@ __device__ __forceinline__ void crweno5_cands(double m2, double m1, double p1, d
   f3 = (s6)*m1 + (5 * s6) * p1;
 }
 
+// Linear fifth-order upwind interpolation (upw5),
+// Interp1PrimFifthOrderUpwind.c:92-96 and :138-142, in the reference's
+// operation order (sums left to right).
+__device__ __forceinline__ double upw5_value(double m3, double m2, double m1, double p1, double p2) {
+  const double c30 = 1.0 / 30.0, c13 = 13.0 / 60.0, c47 = 47.0 / 60.0, c27 = 27.0 / 60.0, c20 = 1.0 / 20.0;
+  return c30 * m3 - c13 * m2 + c47 * m1 + c27 * p1 - c20 * p2;
+}
 
 // Coefficients of row k of one line system: A, B, C and the right-hand side
-// (for WENO5 the right-hand side is the interface value itself).
+// (for WENO5 and upw5, KIND 4, the right-hand side is the interface value
+// itself).
 template <int CR, int KIND>
 __device__ __forceinline__ void row_coeffs(int k, int N, bool left, const ReconParams &R, double w0, double w1,
                                            double w2, double w3, double w4, double w5, double &Ak, double &Bk,
@@ -102,6 +110,11 @@ __device__ __forceinline__ void row_coeffs(int k, int N, bool left, const ReconP
   double m3, m2, m1, p1, p2;
   if (left) { m3 = w0; m2 = w1; m1 = w2; p1 = w3; p2 = w4; }
   else { m3 = w5; m2 = w4; m1 = w3; p1 = w2; p2 = w1; }
+  if (KIND == 4) {
+    Ak = 0.0; Bk = 1.0; Ck = 0.0;
+    Rk = upw5_value(m3, m2, m1, p1, p2);
+    return;
+  }
   const bool bnd = (k == 0) || (k == N);
   double c1, c2, c3;
   if (CR && !bnd) { c1 = 0.2; c2 = 0.5; c3 = 0.3; }
@@ -1046,6 +1059,10 @@ __device__ __forceinline__ void recon_char_e1d(const SweepGeom &S, int nv, const
     }
     double fch[3];
     for (int vv = 0; vv < 3; vv++) {
+      if (KIND == 4) {
+        fch[vv] = upw5_value(s[0][vv], s[1][vv], s[2][vv], s[3][vv], s[4][vv]);
+        continue;
+      }
       double w1, w2, w3;
       if (R.no_limiting) { w1 = 0.1; w2 = 0.6; w3 = 0.3; }
       else weno_w<KIND>(0.1, 0.6, 0.3, s[0][vv], s[1][vv], s[2][vv], s[3][vv], s[4][vv], R.eps, w1, w2, w3);
@@ -1059,6 +1076,10 @@ __device__ __forceinline__ void recon_char_e1d(const SweepGeom &S, int nv, const
     for (int m = 3; m < nv; m++) {
       const double m3 = at(src, m, pos[0]), m2 = at(src, m, pos[1]), m1 = at(src, m, pos[2]);
       const double p1 = at(src, m, pos[3]), p2 = at(src, m, pos[4]);
+      if (KIND == 4) {
+        out[((long)m * nl + l) * (N + 1) + k] = upw5_value(m3, m2, m1, p1, p2);
+        continue;
+      }
       double w1, w2, w3, f1, f2, f3;
       if (R.no_limiting) { w1 = 0.1; w2 = 0.6; w3 = 0.3; }
       else weno_w<KIND>(0.1, 0.6, 0.3, m3, m2, m1, p1, p2, R.eps, w1, w2, w3);
@@ -1092,6 +1113,7 @@ __device__ __forceinline__ void recon_char_ns2d(const SweepGeom &S, int nv, cons
   const long vs = (long)nl * (N + 1), ii = (long)l * (N + 1) + k;
   const double s6 = 1.0 / 6.0;
   auto weno = [&](double m3, double m2, double m1, double p1, double p2) {
+    if (KIND == 4) return upw5_value(m3, m2, m1, p1, p2);
     double w1, w2, w3;
     if (P.no_limiting) { w1 = 0.1; w2 = 0.6; w3 = 0.3; }
     else weno_w<KIND>(0.1, 0.6, 0.3, m3, m2, m1, p1, p2, P.eps, w1, w2, w3);
@@ -1152,6 +1174,7 @@ __global__ void __launch_bounds__(256) k_recon_char(CASE_ARGS) {
       case 1: recon_char_e1d<1>(S, nv, C.R, gam, l, k, C.U, C.flux_x, C.ix); break;
       case 2: recon_char_e1d<2>(S, nv, C.R, gam, l, k, C.U, C.flux_x, C.ix); break;
       case 3: recon_char_e1d<3>(S, nv, C.R, gam, l, k, C.U, C.flux_x, C.ix); break;
+      case 4: recon_char_e1d<4>(S, nv, C.R, gam, l, k, C.U, C.flux_x, C.ix); break;
       default: recon_char_e1d<0>(S, nv, C.R, gam, l, k, C.U, C.flux_x, C.ix); break;
     }
     return;
@@ -1165,6 +1188,7 @@ __global__ void __launch_bounds__(256) k_recon_char(CASE_ARGS) {
     case 1: RCHAR(1); break;
     case 2: RCHAR(2); break;
     case 3: RCHAR(3); break;
+    case 4: RCHAR(4); break;
     default: RCHAR(0); break;
   }
 #undef RCHAR
@@ -1562,6 +1586,7 @@ __global__ void __launch_bounds__(256) k_recon_ends(CASE_ARGS) {
     case 1: cr ? recon_end<1, 1>(C, dir, sys, e) : recon_end<0, 1>(C, dir, sys, e); break;
     case 2: cr ? recon_end<1, 2>(C, dir, sys, e) : recon_end<0, 2>(C, dir, sys, e); break;
     case 3: cr ? recon_end<1, 3>(C, dir, sys, e) : recon_end<0, 3>(C, dir, sys, e); break;
+    case 4: recon_end<0, 4>(C, dir, sys, e); break;
     default: cr ? recon_end<1, 0>(C, dir, sys, e) : recon_end<0, 0>(C, dir, sys, e); break;
   }
 }
@@ -1585,6 +1610,7 @@ __global__ void __launch_bounds__(256) k_recon(CASE_ARGS) {
     case 1: cr ? recon_team<1, 1>(C, dir, sys, active, lane, red, stash) : recon_team<0, 1>(C, dir, sys, active, lane, red, stash); break;
     case 2: cr ? recon_team<1, 2>(C, dir, sys, active, lane, red, stash) : recon_team<0, 2>(C, dir, sys, active, lane, red, stash); break;
     case 3: cr ? recon_team<1, 3>(C, dir, sys, active, lane, red, stash) : recon_team<0, 3>(C, dir, sys, active, lane, red, stash); break;
+    case 4: recon_team<0, 4>(C, dir, sys, active, lane, red, stash); break;
     default: cr ? recon_team<1, 0>(C, dir, sys, active, lane, red, stash) : recon_team<0, 0>(C, dir, sys, active, lane, red, stash); break;
   }
 }
@@ -2359,20 +2385,35 @@ static void run(KernFn k, int cls, const CaseDev *d, const Plan &pl, int s, hipS
   if (mark) mark(ctx, cls, false);
 }
 
+// k_qderiv and k_viscflux read only the stage state (after k_ghost) and
+// write only Qd and FV, which nothing before k_assemble reads; the
+// reconstruction / upwinding chain neither reads nor writes Qd or FV.
 void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, void (*mark)(void *, int, bool),
-                  void *ctx) {
+                  void *ctx, const StageFork *fk) {
   run(k_stage, K_STAGE, d, pl, s, st, mark, ctx);
   run(k_ghost, K_GHOST, d, pl, s, st, mark, ctx);
   run(k_flux, K_FLUX, d, pl, s, st, mark, ctx);
   run(k_recon_ends, K_ENDS, d, pl, s, st, mark, ctx);
+  const bool fork = fk && fk->side && !mark && (pl.nblocks[K_QDERIV] > 0 || pl.nblocks[K_VISCFLUX] > 0);
+  if (fork) {
+    hipEventRecord(fk->fork, st);
+    hipStreamWaitEvent(fk->side, fk->fork, 0);
+    run(k_qderiv, K_QDERIV, d, pl, s, fk->side, nullptr, nullptr);
+    run(k_viscflux, K_VISCFLUX, d, pl, s, fk->side, nullptr, nullptr);
+    hipEventRecord(fk->join, fk->side);
+  }
   run(k_recon, K_RECON, d, pl, s, st, mark, ctx);
   run(k_recon_char, K_RCHAR, d, pl, s, st, mark, ctx);
   run(k_cw_rows, K_CWROWS, d, pl, s, st, mark, ctx);
   run(k_cw_solve, K_CWSOLVE, d, pl, s, st, mark, ctx);
   run(k_upwind, K_UPWIND, d, pl, s, st, mark, ctx);
   run(k_bflux, K_BFLUX, d, pl, s, st, mark, ctx);
-  run(k_qderiv, K_QDERIV, d, pl, s, st, mark, ctx);
-  run(k_viscflux, K_VISCFLUX, d, pl, s, st, mark, ctx);
+  if (fork) {
+    hipStreamWaitEvent(st, fk->join, 0);
+  } else {
+    run(k_qderiv, K_QDERIV, d, pl, s, st, mark, ctx);
+    run(k_viscflux, K_VISCFLUX, d, pl, s, st, mark, ctx);
+  }
   run(k_assemble, K_ASSEMBLE, d, pl, s, st, mark, ctx);
 }
 

@@ -143,11 +143,21 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]);
 int cw_row_fields(int nflow, int nv);
 int cw_back_fields(int nflow, int nv);
 
+// A second stream for the branch of a stage that does not depend on the
+// reconstruction: the viscous derivatives and fluxes (HBM-bound) run beside
+// the reconstruction (FP64-VALU-bound).  The branch forks from the main
+// stream before the reconstruction and joins it before the assembly.
+struct StageFork {
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
 // Stage s of the RK step for every case in the plan: stage state + BCs,
 // fluxes, reconstruction, upwinding, divergence, viscous terms, assembly of
-// Udot[s].  Times: stage time = *time + c[s]*dt of each case.
+// Udot[s].  Times: stage time = *time + c[s]*dt of each case.  With `fk`
+// (and no per-class timing marks) the viscous branch runs on fk->side.
 void launch_stage(const CaseDev *d_cases, const Plan &plan, int s, hipStream_t st, void (*mark)(void *, int, bool),
-                  void *mark_ctx);
+                  void *mark_ctx, const StageFork *fk = nullptr);
 // u += dt*b_s*Udot_s for every stage, time += dt.
 void launch_update(const CaseDev *d_cases, const Plan &plan, hipStream_t st);
 

@@ -352,6 +352,13 @@ int Engine::init(int dev) {
   device = dev;
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  // PIAFS_AMD_ONE_STREAM=1 keeps every launch of a step on one stream
+  const char *one = getenv("PIAFS_AMD_ONE_STREAM");
+  if (!(one && one[0] == '1')) {
+    HIPCHK(hipStreamCreateWithFlags(&fork.side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&fork.fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&fork.join, hipEventDisableTiming));
+  }
   return 0;
 }
 
@@ -446,7 +453,8 @@ void Engine::mark(int cls, bool begin) {
 void Engine::enqueue_step() {
   int ns = 0;
   for (Case *c : cases) ns = std::max(ns, c->cfg.nstages);
-  for (int s = 0; s < ns; s++) launch_stage(d_cases, plan, s, stream, profiling ? mark_cb : nullptr, this);
+  for (int s = 0; s < ns; s++)
+    launch_stage(d_cases, plan, s, stream, profiling ? mark_cb : nullptr, this, fork.side ? &fork : nullptr);
   if (profiling) mark(K_UPDATE, true);
   launch_update(d_cases, plan, stream);
   if (profiling) mark(K_UPDATE, false);
@@ -576,8 +584,12 @@ Engine::~Engine() {
     hipEventDestroy(e.first);
     hipEventDestroy(e.second);
   }
+  if (fork.side) hipStreamSynchronize(fork.side);
   if (owns_cases)
     for (Case *c : cases) delete c;
+  if (fork.side) hipStreamDestroy(fork.side);
+  if (fork.fork) hipEventDestroy(fork.fork);
+  if (fork.join) hipEventDestroy(fork.join);
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -658,7 +670,7 @@ void fill_info(const Case *c, piafs_info *info) {
   info->Re = c->cfg.Re;
   info->Pr = c->cfg.Pr;
   info->model = c->cfg.model_id;
-  info->scheme = c->cfg.crweno ? 1 : 0;
+  info->scheme = c->cfg.crweno ? 1 : c->cfg.upw5 ? 2 : 0;
   info->characteristic = c->cfg.characteristic ? 1 : 0;
   info->include_chem = c->cfg.include_chem ? 1 : 0;
   info->device_bytes = c->bytes;

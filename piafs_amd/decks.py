@@ -344,6 +344,157 @@ def example_2d_initial(name: str, d: str, NI: int, NJ: int, gamma: float = 1.4) 
             f.write("".join("%f " % v for v in u[k].reshape(-1)) + "\n")
 
 
+def _write_binary_initial(d: str, x, y, U) -> None:
+    """initial.inp in the binary layout the examples' aux/init.c write
+    (ReadArray.c): f64 x[NI], f64 y[NJ], then U point-major (x fastest, the
+    nvars components of a point contiguous)."""
+    with open(os.path.join(d, "initial.inp"), "wb") as f:
+        np.asarray(x, dtype=np.float64).tofile(f)
+        np.asarray(y, dtype=np.float64).tofile(f)
+        np.ascontiguousarray(U, dtype=np.float64).tofile(f)
+
+
+def _read_chem_keys(path: str) -> dict:
+    """The begin/end keyword file chemistry.inp as the jets' aux/init.C read
+    it: a number after each known keyword, one skipped token after any other."""
+    known = {"lambda_UV", "theta", "f_CO2", "f_O3", "Ptot", "Ti", "Lz", "z_mm", "nz"}
+    toks = open(path).read().split()
+    out, k = {}, 1
+    while k < len(toks) and toks[k] != "end":
+        w = toks[k]
+        if w in known:
+            out[w] = float(toks[k + 1])
+        k += 2
+    return out
+
+
+def example_jet_initial(name: str, d: str, NI: int, NJ: int) -> None:
+    """initial.inp of Examples/2D_NavierStokes/Uniform_{Supersonic,Subsonic}_Jet
+    as their aux/init.C writes it (binary): the reference scales of the O2/O3
+    photochemistry (L_ref = 1/k_g, v_ref the sound speed, P_ref = rho_ref
+    v_ref^2), a uniform gas of 1e7 Pa moving at 530 m/s along y on
+    [-5, 5] mm x [0, 2] mm (supersonic) or [0, 10] mm (subsonic), and the
+    species ratios O2 = 1, O3 = n_O3/n_O2, CO2 = n_CO2/n_O2 in each of the
+    z_i + 1 layers.  The chemistry.inp of `d` supplies lambda_UV, theta, the
+    fractions, Lz, z_mm and nz, as it does for the C program."""
+    if name not in ("Uniform_Supersonic_Jet", "Uniform_Subsonic_Jet"):
+        raise ValueError(name)
+    sup = name == "Uniform_Supersonic_Jet"
+    pi = 4.0 * math.atan(1.0)
+    NA, kB = 6.02214076e23, 1.380649e-23
+    M_O2, M_CO2 = 0.032, 0.044
+    Cp_O2, Cv_O2, Cp_CO2, Cv_CO2 = 918.45, 650.0, 842.86, 654.5
+    ck = _read_chem_keys(os.path.join(d, "chemistry.inp"))
+    lambda_UV = ck.get("lambda_UV", 2.48e-7)
+    theta = ck.get("theta", 0.17 * pi / 180)
+    f_CO2, f_O3 = ck.get("f_CO2", 0.0), ck.get("f_O3", 0.05)
+    if sup:
+        Ptot, Ti = 101325.0, 288.0
+    else:
+        Ptot = 1.0e7
+        Ti = Ptot / (kB * 1.0e26)
+    Lz, z_mm, nz = ck.get("Lz", 0.03), ck.get("z_mm", 0.0), int(ck.get("nz", 22))
+    kUV = 2 * pi / lambda_UV
+    kg = 2 * kUV * math.sin(theta)
+    f_O2 = 1.0 - f_CO2
+    R = NA * kB / (f_O2 * M_O2 + f_CO2 * M_CO2)
+    gamma = (f_O2 * Cp_O2 + f_CO2 * Cp_CO2) / (f_O2 * Cv_O2 + f_CO2 * Cv_CO2)
+    n_O2 = f_O2 * Ptot / (kB * Ti)
+    n_O3 = f_O3 * Ptot / (kB * Ti)
+    n_CO2 = f_CO2 * Ptot / (kB * Ti)
+    L_ref = 1.0 / kg
+    rho_ref = Ptot / (R * Ti)
+    v_ref = math.sqrt(gamma * R * Ti)
+    P_ref = rho_ref * v_ref * v_ref
+    nz = int(math.ceil(z_mm * 0.001 * nz / Lz)) + 1
+    xmin, xmax = -0.005 / L_ref, 0.005 / L_ref
+    ymin, ymax = 0.000 / L_ref, (0.002 if sup else 0.010) / L_ref
+    if sup:
+        rho0 = (f_O2 * 1.0e26 * M_O2 / NA + f_CO2 * 1.0e26 * M_CO2 / NA) / rho_ref
+    else:
+        rho0 = (n_O2 * M_O2 / NA + n_CO2 * M_CO2 / NA) / rho_ref
+    p0 = 1.0e7 / P_ref
+    u0, v0 = 0.0, 530 / v_ref
+    dx = (xmax - xmin) / (NI - 1.0)
+    dy = (ymax - ymin) / (NJ - 1.0)
+    x = np.array([xmin + i * dx for i in range(NI)])
+    y = np.array([j * dy for j in range(NJ)])
+    nvars = 4 + 7 * nz
+    U = np.zeros((NJ * NI, nvars))
+    U[:, 0] = rho0
+    U[:, 1] = rho0 * u0
+    U[:, 2] = rho0 * v0
+    U[:, 3] = p0 / (gamma - 1.0) + 0.5 * rho0 * (u0 * u0 + v0 * v0)
+    for iz in range(nz):
+        U[:, 4 + 7 * iz + 0] = 1.0
+        U[:, 4 + 7 * iz + 1] = n_O3 / n_O2
+        U[:, 4 + 7 * iz + 6] = n_CO2 / n_O2
+    _write_binary_initial(d, x, y, U)
+
+
+def example_wall_initial(name: str, d: str, NI: int, NJ: int) -> None:
+    """initial.inp (binary) of Examples/2D_NavierStokes/LidDrivenCavity (unit
+    square, rho = 1, p = 1/gamma, the rotating start u = Minf (y - 1/2),
+    v = -Minf (x - 1/2) with Minf = 0.1) or FlatPlateLaminar/UniformGrid
+    ([-0.25, 1] x [0, 0.25], rho = 1, p = 1/1.4, u = 0.3), as their aux/init.c
+    write it."""
+    gamma = 1.4
+    if name == "LidDrivenCavity":
+        dx, dy = 1.0 / (NI - 1.0), 1.0 / (NJ - 1.0)
+        x = np.array([i * dx for i in range(NI)])
+        y = np.array([j * dy for j in range(NJ)])
+        X, Y = np.meshgrid(x, y, indexing="xy")
+        Minf = 0.1
+        u = Minf * (Y - 0.5)
+        v = -(Minf * (X - 0.5))
+        P = 1.0 / gamma
+    elif name == "FlatPlateLaminar":
+        xmin, xmax, ymin, ymax = -0.25, 1.0, 0.0, 0.25
+        dx, dy = (xmax - xmin) / (NI - 1), (ymax - ymin) / (NJ - 1)
+        x = np.array([xmin + i * dx for i in range(NI)])
+        y = np.array([ymin + j * dy for j in range(NJ)])
+        u = np.full((NJ, NI), 0.3)
+        v = np.zeros((NJ, NI))
+        P = 1.0 / 1.4
+    else:
+        raise ValueError(name)
+    rho = np.ones((NJ, NI))
+    U = np.stack([rho, rho * u, rho * v, P / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)], axis=-1)
+    _write_binary_initial(d, x, y, U.reshape(NJ * NI, 4))
+
+
+def shipped_example_initial(name: str, d: str, NI: int, NJ: int) -> None:
+    """initial.inp of a shipped 2D example, written as its aux/init program
+    writes it (the deck files, chemistry.inp included, already in `d`)."""
+    if name in ("Riemann2DCase4", "RadialExpansionWave"):
+        example_2d_initial(name, d, NI, NJ)
+    elif name in ("Uniform_Supersonic_Jet", "Uniform_Subsonic_Jet"):
+        example_jet_initial(name, d, NI, NJ)
+    else:
+        example_wall_initial(name, d, NI, NJ)
+
+
+def copy_example_deck(src: str, dst: str, files, solver_set: dict | None = None,
+                      solver_drop=()) -> None:
+    """Copy an example's input files, changing solver.inp keywords (e.g. a
+    shorter n_iter) and dropping the lines of `solver_drop`."""
+    os.makedirs(dst, exist_ok=True)
+    for fn in files:
+        text = open(os.path.join(src, fn)).read()
+        if fn == "solver.inp" and (solver_set or solver_drop):
+            out = []
+            for line in text.splitlines():
+                t = line.split()
+                if t and t[0] in solver_drop:
+                    continue
+                if t and solver_set and t[0] in solver_set:
+                    line = "  %s %s" % (t[0], solver_set[t[0]])
+                out.append(line)
+            text = "\n".join(out) + "\n"
+        with open(os.path.join(dst, fn), "w") as f:
+            f.write(text)
+
+
 def read_binary_op(path: str):
     """Read a WriteBinary file (src/IOFunctions/WriteBinary.c:37-93):
     int32 ndims, int32 nvars, int32 dim[ndims], f64 x[sum dim], f64 u[prod dim * nvars]."""

@@ -90,6 +90,23 @@ def test_every_reference_fixture_deck_is_accepted(ref_runs):
             assert piafs_amd.estimate_bytes(d) > 0, name
 
 
+def test_shipped_example_decks_are_accepted(examples):
+    """The shipped example directories kept under tests/golden/examples (1D
+    Euler, and 2D Navier-Stokes with the initial state of their aux/init
+    programs: upw5 and walls, photochemistry jets, the flat plate) parse and
+    size on this path."""
+    import shutil
+    base = os.path.join(os.path.dirname(__file__), "golden", "examples")
+    for ex in sorted(os.listdir(base)):
+        with tempfile.TemporaryDirectory() as d:
+            for fn in os.listdir(os.path.join(base, ex)):
+                shutil.copy(os.path.join(base, ex, fn), os.path.join(d, fn))
+            if not os.path.exists(os.path.join(d, "initial.inp")):
+                ni, nj = examples[ex]["size"]
+                decks.shipped_example_initial(ex, d, ni, nj)
+            assert piafs_amd.estimate_bytes(d) > 0, ex
+
+
 def test_missing_initial_solution_is_an_error():
     dk = decks.sod_1d(N=32)
     with tempfile.TemporaryDirectory() as d:

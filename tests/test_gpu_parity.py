@@ -93,6 +93,8 @@ FIXTURES = [
     "riemann2d_comp_llf_crweno_33_30", "sod_slipwall_char_llf_N64", "sod_dirichlet_crweno_rusanov_N80",
     "sod_sponge_comp_llf_N64", "sod_char_rusanov_N64", "sod_crweno_char_roe_N64", "sod_crweno_char_rf_mapped_N48",
     "ns2d_crweno_char_flow_20x16", "ns2d_chem_crweno_char_rf_flow_16x12", "ns2d_crweno_char_llf_mapped_inout_20x16",
+    "sod_upw5_char_roe_N64", "sod_upw5_comp_rf_N64", "ns2d_chem_upw5_comp_roe_20x16",
+    "ns2d_upw5_char_llf_flow_20x16", "ns2d_upw5_cavity_re400_20x20",
 ]
 
 
@@ -136,46 +138,78 @@ def test_shipped_logs(examples, example):
         assert _close_in_last_digit(norm[it - 1], nm, "%1.4E"), (it, norm[it - 1], nm)
 
 
-@pytest.mark.parametrize("example", ["Riemann2DCase4", "RadialExpansionWave"])
-def test_shipped_2d_examples_through_the_driver(examples, example):
-    """A shipped 2D example as a user runs it: its own decks (characteristic
-    WENO5 with mapped weights, rf-char, SSPRK3, mirror BCs, conservation check,
-    Tecplot output every 25 / 10 steps) and the initial state its aux/init.c
-    writes, through the command-line driver.  Against the reference's run of
-    the same directory: every screen line to its printed digits, the file
-    names, the conservation errors, and the final solution (norms, a stride-4
-    subsample, row and column sums)."""
-    import shutil
-    import subprocess
-    ref = examples[example]
-    ni, nj = ref["size"]
-    z = np.load(os.path.join(GOLDEN, "examples2d_ref.npz"))
-    with tempfile.TemporaryDirectory() as d:
-        for fn in ("solver.inp", "boundary.inp", "physics.inp", "weno.inp"):
-            shutil.copy(os.path.join(GOLDEN, "examples", example, fn), os.path.join(d, fn))
-        decks.example_2d_initial(example, d, ni, nj)
-        r = subprocess.run([os.path.join(ROOT, "piafs_amd", "bin", "piafs_amd"), d], capture_output=True,
-                           text=True, timeout=600)
-        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-        ops = sorted(f for f in os.listdir(d) if f.startswith("op_") and f.endswith(".dat"))
-        assert ops == ref["files"]
-        lines = [l.split() for l in open(os.path.join(d, ops[-1])).read().splitlines()[2:] if l.strip()]
-        U = np.array([[float(t) for t in l[4:]] for l in lines]).reshape(nj, ni, 4)
-        cons = [float(v) for v in open(os.path.join(d, "conservation.dat")).read().split()[5:]]
+SHIPPED_2D = ["Riemann2DCase4", "RadialExpansionWave", "LidDrivenCavity", "Uniform_Supersonic_Jet",
+              "Uniform_Subsonic_Jet", "FlatPlateLaminar"]
+
+
+def _run_shipped(example, d, ni, nj, solver_set=None):
+    """Copy the stored deck of a shipped example into d (optionally changing
+    solver.inp keywords), write its initial state, run the command-line
+    driver there; returns the driver's output and the files it made."""
+    src = os.path.join(GOLDEN, "examples", example)
+    decks.copy_example_deck(src, d, sorted(os.listdir(src)), solver_set)
+    decks.shipped_example_initial(example, d, ni, nj)
+    before = set(os.listdir(d))
+    r = subprocess.run([os.path.join(ROOT, "piafs_amd", "bin", "piafs_amd"), d], capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r.stdout, sorted(set(os.listdir(d)) - before)
+
+
+def _screen_rows(out):
     rows = {}
-    for line in r.stdout.splitlines():
+    for line in out.splitlines():
         t = line.split()
         if t and t[0].startswith("iter="):
             rows[int(line.split("iter=")[1].split()[0])] = (float(line.split("CFL=")[1].split()[0]),
                                                             float(line.split("norm=")[1].split()[0]))
+    return rows
+
+
+@pytest.mark.parametrize("example", SHIPPED_2D)
+def test_shipped_2d_examples_through_the_driver(examples, example):
+    """A shipped 2D example as a user runs it: its own decks and the initial
+    state its aux/init program writes, through the command-line driver.
+    Riemann2DCase4 / RadialExpansionWave: characteristic WENO5, mapped
+    weights, rf-char, SSPRK3, mirror BCs, conservation check, Tecplot output.
+    LidDrivenCavity: upw5, no-slip walls with a moving lid, Re 3200.  The
+    jets: O2/O3 photochemistry, characteristic WENO5, rf-char, supersonic
+    inflow / outflow and subsonic inflow / extrapolation in y, periodic in x,
+    numbered binary output and species files (the subsonic jet through its
+    photolysis pulse).  FlatPlateLaminar: WENO5 components, Roe, slip and
+    no-slip walls, subsonic inflow / outflow, Re 1e5.  Against the
+    reference's run of the same directory (a shortened n_iter): every screen
+    line to its printed digits, the names of the files written, the
+    conservation errors, and the final solution (norms, a subsample, row and
+    column sums)."""
+    ref = examples[example]
+    ni, nj = ref["size"]
+    z = np.load(os.path.join(GOLDEN, "examples2d_ref.npz"))
+    with tempfile.TemporaryDirectory() as d:
+        out, made = _run_shipped(example, d, ni, nj)
+        assert made == ref["files"]
+        ops = [f for f in made if f.startswith("op") and not f.startswith("op_species")]
+        last = os.path.join(d, ops[-1])
+        if last.endswith(".dat"):
+            lines = [l.split() for l in open(last).read().splitlines()[2:] if l.strip()]
+            U = np.array([[float(t) for t in l[4:]] for l in lines]).reshape(nj, ni, -1)
+        else:
+            U = decks.read_binary_op(last)[4].reshape(nj, ni, -1)
+        cons = ([float(v) for v in open(os.path.join(d, "conservation.dat")).read().split()[5:]]
+                if ref["conservation"] is not None else None)
+    rows = _screen_rows(out)
+    assert sorted(rows) == [row[0] for row in ref["rows"]]
     for it, _, cf, nm in ref["rows"]:
         assert _close_in_last_digit(rows[it][0], cf, "%1.3E"), (it, rows[it][0], cf)
         assert _close_in_last_digit(rows[it][1], nm, "%1.4E"), (it, rows[it][1], nm)
-    np.testing.assert_allclose(cons, ref["conservation"], rtol=0, atol=1e-11)
-    norm = np.sqrt((U.reshape(-1, 4) ** 2).sum(0))
+    if cons is not None:
+        np.testing.assert_allclose(cons, ref["conservation"], rtol=0, atol=1e-11)
+    nv = U.shape[-1]
+    norm = np.sqrt((U.reshape(-1, nv) ** 2).sum(0))
     assert np.all(np.abs(norm - z[example + "_norm"]) <= 1e-12 * z[example + "_norm"])
-    sub = U[::4, ::4, :].reshape(-1, 4)
-    assert np.all(_rel(sub, z[example + "_sub"].reshape(-1, 4)) <= TOL_STEPS)
+    st = ref["stride"]
+    sub = U[::st, ::st, :].reshape(-1, nv)
+    assert np.all(_rel(sub, z[example + "_sub"].reshape(-1, nv)) <= TOL_STEPS)
     assert np.all(_sum_err(U.sum(1), z[example + "_rowsum"], z[example + "_rowabs"]) <= TOL_STEPS)
     assert np.all(_sum_err(U.sum(0), z[example + "_colsum"], z[example + "_colabs"]) <= TOL_STEPS)
 

@@ -41,6 +41,8 @@ def _run_oracle(dk, n=None):
     "riemann2d_comp_llf_crweno_33_30", "sod_slipwall_char_llf_N64", "sod_dirichlet_crweno_rusanov_N80",
     "sod_sponge_comp_llf_N64", "sod_char_rusanov_N64", "sod_crweno_char_roe_N64", "sod_crweno_char_rf_mapped_N48",
     "ns2d_crweno_char_flow_20x16", "ns2d_chem_crweno_char_rf_flow_16x12", "ns2d_crweno_char_llf_mapped_inout_20x16",
+    "sod_upw5_char_roe_N64", "sod_upw5_comp_rf_N64", "ns2d_chem_upw5_comp_roe_20x16",
+    "ns2d_upw5_char_llf_flow_20x16", "ns2d_upw5_cavity_re400_20x20",
 ])
 def test_oracle_matches_reference_bitwise(ref_runs, name):
     data, meta = ref_runs
@@ -140,9 +142,10 @@ def test_free_stream_preserved():
     assert np.abs(r).max() < 1e-13
 
 
-@pytest.mark.parametrize("scheme", ["weno5", "crweno5"])
+@pytest.mark.parametrize("scheme", ["weno5", "crweno5", "upw5"])
 def test_linear_schemes_fifth_order(scheme):
-    """With no_limiting the reconstructions are the linear fifth-order schemes:
+    """With no_limiting the reconstructions are the linear fifth-order schemes
+    (upw5 is one by construction):
     the advection RHS of a density wave converges at order ~5 in the middle half
     of the line (CRWENO's lines end in explicit WENO5 rows even on a periodic
     grid, which is fourth order at the two ends and decays geometrically)."""
@@ -158,6 +161,29 @@ def test_linear_schemes_fifth_order(scheme):
         errs.append(np.abs(r[N // 4:3 * N // 4, 0] - exact[N // 4:3 * N // 4]).max())
     order = np.log2(errs[0] / errs[1])
     assert order > 4.5, (errs, order)
+
+
+@pytest.mark.parametrize("interp", ["components", "characteristic"])
+def test_upw5_is_weno5_at_optimal_weights(interp):
+    """The optimal weights (0.1, 0.6, 0.3) combine the three WENO5 candidate
+    stencils into the single five-point upwind stencil (2, -13, 47, 27, -3)/60,
+    so upw5 and WENO5 with no_limiting give the same right-hand side up to
+    rounding, on any data: a wrong coefficient or a reversed bias in either
+    fails this."""
+    dk = decks.sod_1d(N=48, n_iter=1, scheme="upw5", interp=interp)
+    rng = np.random.default_rng(11)
+    u = dk.u * (1.0 + 0.05 * rng.uniform(-1, 1, dk.u.shape))
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        r_upw = orc.OCase(d).rhs(u, 0.0)
+    dk.solver["hyp_space_scheme"] = "weno5"
+    dk.weno["no_limiting"] = 1
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        r_weno = orc.OCase(d).rhs(u, 0.0)
+    scale = np.abs(r_weno).max(axis=0)
+    assert np.all(np.abs(r_upw - r_weno).max(axis=0) <= 1e-12 * scale)
+    assert not np.array_equal(r_upw, r_weno)  # different arithmetic, same polynomial
 
 
 def test_chemistry_constants_from_formulas():
