@@ -9,7 +9,8 @@ through the total pressure), O2/O3 photochemistry with a Gaussian beam map,
 RK4 with dt = 0.01.  A step is one RK4 time step (four right-hand sides);
 inputs are resident in HBM before timing starts.
 
-Prints one JSON line (see README of the benchmark contract in DESIGN.md).
+Prints one JSON line: metric, value, unit, n_gpus, steps, warmup, ms_per_step, the
+workload, the roofline of the dominant kernel and the CPU baseline.
 """
 from __future__ import annotations
 
@@ -67,7 +68,7 @@ def measured_traffic(nx):
 
 def cpu_baseline(tmp, nx_sample=256, steps=10):
     """Reference PIAFS (built from source, serial, 1 core) on a bounded sample of
-    the same workload: the config deck at nx_sample^2 with the same dx, dt and
+    the same workload: the benchmark deck at nx_sample^2 with the same dx, dt and
     physics.  Falls back to the CPU restatement where the reference binary is
     absent."""
     from piafs_amd import decks
@@ -76,7 +77,7 @@ def cpu_baseline(tmp, nx_sample=256, steps=10):
     dk.write(d)
     ref = os.path.join(ROOT, "oracle", "_ref", "PIAFS")
     cells = nx_sample * nx_sample * steps
-    sample = (f"config-2 deck restricted to {nx_sample}x{nx_sample} (same dx, dt, physics), "
+    sample = (f"the 1024^2 chemistry deck restricted to {nx_sample}x{nx_sample} (same dx, dt, physics), "
               f"{steps} RK4 steps")
     if os.path.exists(ref):
         r = subprocess.run([ref], cwd=d, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
@@ -279,8 +280,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=None,
                     help="grid side (same domain 10 pi, dx = 10 pi / nx; dt 0.01 * 1024 / nx)")
-    ap.add_argument("--config", type=int, default=2, choices=(2, 5),
-                    help="2: 1024^2, dt 0.01 (default); 5: 4096^2, dt 0.0025, 50 steps")
+    ap.add_argument("--grid", type=int, default=1024, choices=(1024, 4096),
+                    help="1024: 1024^2, dt 0.01 (default); 4096: 4096^2, dt 0.0025, 50 steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
@@ -307,15 +308,15 @@ def main():
         print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "launches": 10}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only (nothing here uses a second GPU); "
+        print("bench.py: piafs_amd runs on one GPU (nothing here uses a second GPU); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import piafs_amd
     from piafs_amd import decks
 
-    nx = args.nx or (4096 if args.config == 5 else 1024)
-    if args.config == 5 and args.steps == 200:
+    nx = args.nx or args.grid
+    if args.grid == 4096 and args.steps == 200:
         args.steps = 50
     dt = 0.01 * 1024 / nx   # same acoustic CFL on the same domain
     with tempfile.TemporaryDirectory() as tmp:
@@ -367,7 +368,7 @@ def main():
                                  "achieved": step_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": step_gbs / HBM_PEAK_GBS}
         if args.parity and nx == 1024 and args.warmup + args.steps == 200:
-            fx = os.path.join(ROOT, "tests", "golden", "config2_1024_ref.npz")
+            fx = os.path.join(ROOT, "tests", "golden", "ns2d_chem_crweno_1024_200steps_ref.npz")
             if os.path.exists(fx):
                 z = np.load(fx)
                 U = case.solution().reshape(nx, nx, nv)

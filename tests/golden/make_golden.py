@@ -14,7 +14,7 @@ Two kinds of fixtures:
    LaxShockTube} and the error norms of
    Examples/2D_NavierStokes/InviscidVortexConvection/errors.dat.
 
-Usage: python tests/golden/make_golden.py [--config2-full RUN_DIR | --config5-nochem RUN_DIR]
+Usage: python tests/golden/make_golden.py [--ns2d-1024-full RUN_DIR | --ns2d-4096-nochem RUN_DIR]
 """
 from __future__ import annotations
 
@@ -56,7 +56,7 @@ CASES = {
     "ns2d_chem_2layer_16x16": ("ns2d_photochem", dict(nx=16, ny=16, n_iter=5, z_mm=1.5)),
     "ns2d_nochem_mapped_20x16": ("ns2d_photochem", dict(nx=20, ny=16, n_iter=8, include_chemistry=False,
                                                         weights="mapped")),
-    "ns2d_chem_weno_mixed_cfg3_17x31": ("ns2d_photochem", dict(nx=17, ny=31, n_iter=6, scheme="weno5",
+    "ns2d_chem_weno_mixed_17x31": ("ns2d_photochem", dict(nx=17, ny=31, n_iter=6, scheme="weno5",
                                                                amplitude=1.7, Re=300.0, dx=10.0 * np.pi / 1024 * 8)),
     "vortex_crweno_mapped_60_400": ("isentropic_vortex", dict(N=60, n_iter=400, dt=0.05)),
     # characteristic WENO5 and rf-char upwinding in 2D
@@ -216,7 +216,7 @@ def example_logs():
         json.dump(meta, f, indent=1)
 
 
-def config2_full(run_dir):
+def ns2d_1024_full(run_dir):
     """Reduce the 1024^2 / 200-step reference run to a small fixture: per
     variable norms, row and column sums, and a stride-32 subsample."""
     _, nv, dims, x, u = decks.read_binary_op(os.path.join(run_dir, "op.bin"))
@@ -231,16 +231,16 @@ def config2_full(run_dir):
             if m:
                 cfl, norm = float(m.group(1)), float(m.group(2))
     np.savez_compressed(
-        os.path.join(HERE, "config2_1024_ref.npz"),
+        os.path.join(HERE, "ns2d_chem_crweno_1024_200steps_ref.npz"),
         l1=np.abs(U).sum(axis=(0, 1)), l2=np.sqrt((U * U).sum(axis=(0, 1))), linf=np.abs(U).max(axis=(0, 1)),
         colsum=U.sum(axis=0), rowsum=U.sum(axis=1), sub=U[::32, ::32, :].copy(),
         colabs=np.abs(U).sum(axis=0), rowabs=np.abs(U).sum(axis=1),
         cfl_200=np.float64(cfl), norm_200=np.float64(norm),
     )
-    print("config2 fixture written", U.shape)
+    print("1024^2 fixture written", U.shape)
 
 
-def config5_nochem(run_dir):
+def ns2d_4096_nochem(run_dir):
     """Reduce the 4096^2 chemistry-off 10-step reference run (lines of 4097
     interfaces: the longest team solves) to norms, sums and a stride-128
     subsample."""
@@ -248,11 +248,11 @@ def config5_nochem(run_dir):
     nx, ny = dims
     U = u.reshape(ny, nx, nv)
     np.savez_compressed(
-        os.path.join(HERE, "config5_4096_nochem_ref.npz"),
+        os.path.join(HERE, "ns2d_nochem_crweno_4096_10steps_ref.npz"),
         l2=np.sqrt((U * U).sum(axis=(0, 1))), colsum=U.sum(axis=0), rowsum=U.sum(axis=1),
         colabs=np.abs(U).sum(axis=0), rowabs=np.abs(U).sum(axis=1), sub=U[::128, ::128, :].copy(),
     )
-    print("config5 chemistry-off fixture written", U.shape)
+    print("4096^2 chemistry-off fixture written", U.shape)
 
 
 # Decks whose complete file output (solution, species, errors.dat and the
@@ -402,10 +402,10 @@ def examples_2d(names=None):
 
 
 if __name__ == "__main__":
-    if len(sys.argv) > 2 and sys.argv[1] == "--config2-full":
-        config2_full(sys.argv[2])
-    elif len(sys.argv) > 2 and sys.argv[1] == "--config5-nochem":
-        config5_nochem(sys.argv[2])
+    if len(sys.argv) > 2 and sys.argv[1] == "--ns2d-1024-full":
+        ns2d_1024_full(sys.argv[2])
+    elif len(sys.argv) > 2 and sys.argv[1] == "--ns2d-4096-nochem":
+        ns2d_4096_nochem(sys.argv[2])
     elif len(sys.argv) > 1 and sys.argv[1] == "--outputs":
         outputs()
     elif len(sys.argv) > 1 and sys.argv[1] == "--add":

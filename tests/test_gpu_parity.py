@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C ABI) with the CPU restatement and the
 reference fixtures.  Tolerances: FP64 relative L2 per variable <= 1e-11 after
-the stated step counts (the north-star bar); single RHS evaluations and
+the stated step counts; single RHS evaluations and
 operator calls <= 1e-12 (differences come only from FMA contraction)."""
 import os
 import subprocess
@@ -84,7 +84,7 @@ FIXTURES = [
     "sod_char_roe_N128", "sod_char_mapped_N96", "sod_rfchar_mapped_rk44_N101", "sine_crweno_comp_N128",
     "sine_weno_comp_N100", "ns2d_chem_crweno_js_24x20", "ns2d_chem_weno_z_16x24", "ns2d_chem_par2_yc_16x16",
     "ns2d_chem_mapped_nolimit_16x16", "ns2d_chem_pulse_24x20", "ns2d_chem_2layer_16x16",
-    "ns2d_nochem_mapped_20x16", "ns2d_chem_weno_mixed_cfg3_17x31", "ns2d_chem_crweno_48x40",
+    "ns2d_nochem_mapped_20x16", "ns2d_chem_weno_mixed_17x31", "ns2d_chem_crweno_48x40",
     "sod_char_roe_ssprk3_N1024", "ns2d_chem_char_rf_24x20", "ns2d_chem_char_roe_mapped_20x16",
     "ns2d_chem_comp_rf_crweno_20x24", "ns2d_nochem_char_rf_z_16x20", "riemann2d_char_rf_41_40",
     "riemann2d_comp_roe_crweno_33_30", "ns2d_walls_inout_re200_24x20", "ns2d_walls_overlap_crweno_llf_20x24",
@@ -242,7 +242,7 @@ def test_write_binary_layout_and_cli(ref_runs):
     assert np.all(_rel(u, data[name]) <= TOL_STEPS)
 
 
-def test_config1_full_size(ref_runs):
+def test_sod_1024_full_run(ref_runs):
     """1D Sod, N = 1024, WENO5 characteristic + Roe, SSPRK3, 901 steps to t = 0.2."""
     data, meta = ref_runs
     dk = build_deck(meta["sod_char_roe_ssprk3_N1024"])
@@ -254,11 +254,11 @@ def test_config1_full_size(ref_runs):
     assert np.all(_rel(got, data["sod_char_roe_ssprk3_N1024"]) <= TOL_STEPS)
 
 
-@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "config2_1024_ref.npz")),
-                    reason="config-2 reference fixture not generated")
-def test_config2_full_size():
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "ns2d_chem_crweno_1024_200steps_ref.npz")),
+                    reason="1024^2 reference fixture not generated")
+def test_ns2d_chem_1024_full_run():
     """2D NS + photochemistry, 1024^2, CRWENO5, RK4, 200 steps vs the reference run."""
-    z = np.load(os.path.join(GOLDEN, "config2_1024_ref.npz"))
+    z = np.load(os.path.join(GOLDEN, "ns2d_chem_crweno_1024_200steps_ref.npz"))
     dk = decks.ns2d_photochem(nx=1024, n_iter=200)
     with tempfile.TemporaryDirectory() as d:
         dk.write(d)
@@ -276,13 +276,13 @@ def test_config2_full_size():
     assert np.all(_sum_err(U.sum(axis=1), z["rowsum"], z["rowabs"]) <= TOL_STEPS)
 
 
-@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "config5_4096_nochem_ref.npz")),
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "ns2d_nochem_crweno_4096_10steps_ref.npz")),
                     reason="4096^2 reference fixture not generated")
 def test_4096_lines_nochem():
     """4096^2 (lines of 4097 interfaces, 256-lane teams spanning four waves),
     CRWENO5 + viscous, chemistry off, RK4 dt 0.0025, 10 steps vs the reference
     run."""
-    z = np.load(os.path.join(GOLDEN, "config5_4096_nochem_ref.npz"))
+    z = np.load(os.path.join(GOLDEN, "ns2d_nochem_crweno_4096_10steps_ref.npz"))
     dk = decks.ns2d_photochem(nx=4096, n_iter=10, dt=0.0025, include_chemistry=False, Re=1000.0)
     with tempfile.TemporaryDirectory() as d:
         dk.write(d)

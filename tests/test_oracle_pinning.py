@@ -32,7 +32,7 @@ def _run_oracle(dk, n=None):
     "sod_char_roe_N128", "sod_char_mapped_N96", "sod_rfchar_mapped_rk44_N101", "sine_crweno_comp_N128",
     "sine_weno_comp_N100", "ns2d_chem_crweno_js_24x20", "ns2d_chem_weno_z_16x24", "ns2d_chem_par2_yc_16x16",
     "ns2d_chem_mapped_nolimit_16x16", "ns2d_chem_pulse_24x20", "ns2d_chem_2layer_16x16",
-    "ns2d_nochem_mapped_20x16", "ns2d_chem_weno_mixed_cfg3_17x31", "ns2d_chem_crweno_48x40",
+    "ns2d_nochem_mapped_20x16", "ns2d_chem_weno_mixed_17x31", "ns2d_chem_crweno_48x40",
     "sod_char_roe_ssprk3_N1024", "ns2d_chem_char_rf_24x20", "ns2d_chem_char_roe_mapped_20x16",
     "ns2d_chem_comp_rf_crweno_20x24", "ns2d_nochem_char_rf_z_16x20", "riemann2d_char_rf_41_40",
     "riemann2d_comp_roe_crweno_33_30", "ns2d_walls_inout_re200_24x20", "ns2d_walls_overlap_crweno_llf_20x24",
@@ -187,7 +187,7 @@ def test_upw5_is_weno5_at_optimal_weights(interp):
 
 
 def test_chemistry_constants_from_formulas():
-    """Re and Pr of the config-2 deck follow the reference-scale formulas
+    """Re and Pr of the 1024^2 chemistry deck follow the reference-scale formulas
     (ChemistryInitialize.c:289-314, NavierStokes2DInitialize.c:204-210)."""
     dk = decks.ns2d_photochem(nx=8, ny=8, n_iter=1)
     with tempfile.TemporaryDirectory() as d:
