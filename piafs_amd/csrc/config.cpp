@@ -179,8 +179,13 @@ int Config::parse(const std::string &dir, std::string &err) {
   npts_int = 1;
   npts_g = 1;
   for (int d = 0; d < ndims; d++) {
-    if (dim[d] < 5) {
-      err = "grid too small";
+    // the reference needs 4 points per dimension for its one rank
+    // (ComputeProcessorDistribution, ReadInputs.c:39-49, _MIN_GRID_PTS_PER_PROC_
+    // of basic.h:42); a periodic or mirrored ghost layer reads as many
+    // interior points as there are ghosts
+    if (dim[d] < 4 || dim[d] < ghosts) {
+      err = "grid too small: " + std::to_string(dim[d]) + " points along dimension " + std::to_string(d) +
+            " (at least 4, and at least the " + std::to_string(ghosts) + " ghost points)";
       return 1;
     }
     npts_int *= dim[d];

@@ -463,6 +463,190 @@ def example_wall_initial(name: str, d: str, NI: int, NJ: int) -> None:
     _write_binary_initial(d, x, y, U.reshape(NJ * NI, 4))
 
 
+def _write_ascii_rows(path: str, rows, fmt: str) -> None:
+    """Rows of numbers as the examples' aux/init programs print them: each
+    value followed by a space, one row per line."""
+    with open(path, "w") as f:
+        for r in rows:
+            f.write("".join(fmt % v for v in np.asarray(r).reshape(-1)) + "\n")
+
+
+def _photochem_scales(d: str, Ptot: float = 101325.0, Ti: float = 288.0):
+    """The reference scales the photochemistry examples' aux/init programs
+    derive (O2/CO2 gas, k_g = 2 k_UV sin(theta)), with lambda_UV, theta, the
+    fractions, Ptot, Ti, Lz, z_mm and nz taken from chemistry.inp in d."""
+    pi = 4.0 * math.atan(1.0)
+    NA, kB = 6.02214076e23, 1.380649e-23
+    M_O2, M_CO2 = 0.032, 0.044
+    ck = _read_chem_keys(os.path.join(d, "chemistry.inp"))
+    Ptot, Ti = ck.get("Ptot", Ptot), ck.get("Ti", Ti)
+    f_CO2, f_O3 = ck.get("f_CO2", 0.0), ck.get("f_O3", 0.05)
+    Lz, z_mm, nz = ck.get("Lz", 0.03), ck.get("z_mm", 0.0), int(ck.get("nz", 22))
+    f_O2 = 1.0 - f_CO2
+    R = NA * kB / (f_O2 * M_O2 + f_CO2 * M_CO2)
+    gamma = (f_O2 * 918.45 + f_CO2 * 842.86) / (f_O2 * 650.0 + f_CO2 * 654.5)
+    n_O2 = f_O2 * Ptot / (kB * Ti)
+    n_O3 = f_O3 * Ptot / (kB * Ti)
+    n_CO2 = f_CO2 * Ptot / (kB * Ti)
+    rho_ref = Ptot / (R * Ti)
+    v_ref = math.sqrt(gamma * R * Ti)
+    P_ref = rho_ref * v_ref * v_ref
+    rho0 = (n_O2 * M_O2 / NA + n_CO2 * M_CO2 / NA) / rho_ref
+    nz = int(math.ceil(z_mm * 0.001 * nz / Lz)) + 1
+    return dict(pi=pi, gamma=gamma, rho0=rho0, p0=Ptot / P_ref, nz=nz, O3=n_O3 / n_O2, CO2=n_CO2 / n_O2)
+
+
+def example_beam_initial(name: str, d: str, NI: int, NJ: int = 1) -> None:
+    """Examples/1D_Euler/BeamInducedDensityMod (initial.inp in ASCII "%lf")
+    and Examples/2D_NavierStokes/BeamInducedDensityMod_1D_x (binary, with the
+    beam map imap.inp of its aux/imap.C: 1 + cos(x - xmin)), as their aux
+    programs write them: gas at rest at Ptot on five acoustic periods
+    [-5 pi, 5 pi], species O2 = 1, O3 = n_O3/n_O2, CO2 = n_CO2/n_O2 in each of
+    the z_i + 1 layers."""
+    c = _photochem_scales(d)
+    pi, g, rho0, p0, nz = c["pi"], c["gamma"], c["rho0"], c["p0"], c["nz"]
+    l_aw = 2 * pi / 1
+    N_period_x = 5.0
+    xmin, xmax = -N_period_x / 2 * l_aw, N_period_x / 2 * l_aw
+    if name == "BeamInducedDensityMod":
+        dx = (xmax - xmin) / (NI - 1)
+        x = np.array([xmin + i * dx for i in range(NI)])
+        rows = [x, np.full(NI, rho0), np.full(NI, rho0 * 0.0), np.full(NI, p0 / (g - 1.0) + 0.5 * rho0 * 0.0 * 0.0)]
+        for _ in range(nz):
+            rows += [np.full(NI, 1.0), np.full(NI, c["O3"])] + [np.zeros(NI)] * 4 + [np.full(NI, c["CO2"])]
+        _write_ascii_rows(os.path.join(d, "initial.inp"), rows, "%f ")
+        return
+    if name != "BeamInducedDensityMod_1D_x":
+        raise ValueError(name)
+    dx = (xmax - xmin) / (NI - 1.0)
+    x = np.array([xmin + i * dx for i in range(NI)])
+    y = np.array([j * dx for j in range(NJ)])
+    nvars = 4 + 7 * nz
+    U = np.zeros((NJ * NI, nvars))
+    U[:, 0] = rho0
+    U[:, 3] = p0 / (g - 1.0) + 0.5 * rho0 * (0.0 * 0.0 + 0.0 * 0.0)
+    for iz in range(nz):
+        U[:, 4 + 7 * iz] = 1.0
+        U[:, 4 + 7 * iz + 1] = c["O3"]
+        U[:, 4 + 7 * iz + 6] = c["CO2"]
+    _write_binary_initial(d, x, y, U)
+    ival = np.tile(np.array([1.0 + 1.0 * math.cos((xi - xmin) * (1 - 0.0 * (xi - xmin))) for xi in x]), NJ)
+    with open(os.path.join(d, "imap.inp"), "wb") as f:
+        x.tofile(f)
+        y.tofile(f)
+        ival.tofile(f)
+
+
+def example_pressure_initial(name: str, d: str, NI: int, NJ: int = 1) -> None:
+    """Examples/1D_Euler/PressurePerturbation (ASCII "%lf") and
+    Examples/2D_NavierStokes/PressurePerturbation_1D_x (binary), as their
+    aux/init programs write them: rho = 1, p = 1 + exp(-x^2 / (2 sigma^2))
+    with sigma = 5 / (2 sqrt(2 ln 2)), at rest, on ten periods [-10 pi, 10 pi]."""
+    pi, gamma = 4.0 * math.atan(1.0), 1.4
+    l_aw = 2 * pi / 1
+    N_period_x = 10.0
+    xmin, xmax = -N_period_x / 2 * l_aw, N_period_x / 2 * l_aw
+    sigma = 5.0 / (2.0 * math.sqrt(2.0 * math.log(2.0)))
+    if name == "PressurePerturbation":
+        dx = (xmax - xmin) / (NI - 1)
+    elif name == "PressurePerturbation_1D_x":
+        dx = (xmax - xmin) / (NI - 1.0)
+    else:
+        raise ValueError(name)
+    x = np.array([xmin + i * dx for i in range(NI)])
+    pres = np.array([1.0 + 1.0 * 1.0 * math.exp(-0.5 * (xi / sigma) * (xi / sigma)) for xi in x])
+    if name == "PressurePerturbation":
+        e = pres / (gamma - 1.0) + 0.5 * 1.0 * 0.0 * 0.0
+        _write_ascii_rows(os.path.join(d, "initial.inp"), [x, np.ones(NI), np.zeros(NI), e], "%f ")
+        return
+    y = np.array([j * dx for j in range(NJ)])
+    U = np.zeros((NJ, NI, 4))
+    U[:, :, 0] = 1.0
+    U[:, :, 3] = pres / (gamma - 1.0) + 0.5 * 1.0 * (0.0 * 0.0 + 0.0 * 0.0)
+    _write_binary_initial(d, x, y, U.reshape(-1, 4))
+
+
+def _solver_keys(d: str) -> dict:
+    toks = open(os.path.join(d, "solver.inp")).read().split()
+    return {toks[k]: toks[k + 1] for k in range(len(toks) - 1)}
+
+
+def example_sine_initial(name: str, d: str, NI: int, NJ: int = 1) -> None:
+    """Examples/1D_Euler/DensitySineWave (`DensitySineWave_1D`) and
+    Examples/2D_NavierStokes/DensitySineWave (`DensitySineWave_2D`): exact.inp at
+    t = n_iter dt and initial.inp, both ASCII "%1.16E", as their aux/init
+    programs write them (rho = 1 + 0.1 sin(2 pi (x - t)) [cos(2 pi (y - t))],
+    u = v = 1, p = 1/gamma)."""
+    pi, gamma = 4.0 * math.atan(1.0), 1.4
+    sk = _solver_keys(d)
+    tf = float(int(sk["n_iter"])) * float(sk["dt"])
+    if name == "DensitySineWave_1D":
+        dx = 1.0 / NI
+        x = np.array([i * dx for i in range(NI)])
+        for fn, t in (("exact.inp", tf), ("initial.inp", None)):
+            rho = np.array([1.0 + (0.1 * math.sin(2 * pi * (xi - 1.0 * t)) if t is not None
+                                   else 0.1 * math.sin(2 * pi * xi)) for xi in x])
+            e = 1.0 / gamma / (gamma - 1.0) + 0.5 * rho * 1.0 * 1.0
+            _write_ascii_rows(os.path.join(d, fn), [x, rho, rho * 1.0, e], "%1.16E ")
+        return
+    if name != "DensitySineWave_2D":
+        raise ValueError(name)
+    dx, dy = 1.0 / NI, 1.0 / NJ
+    x = np.array([i * dx for i in range(NI)])
+    y = np.array([j * dy for j in range(NJ)])
+    for fn, t in (("exact.inp", tf), ("initial.inp", 0.0)):
+        rho = np.array([[1.0 + 0.1 * math.sin(2 * pi * (x[i] - 1.0 * t)) * math.cos(2 * pi * (y[j] - 1.0 * t))
+                         for i in range(NI)] for j in range(NJ)])
+        P = 1.0 / gamma
+        e = P / (gamma - 1.0) + 0.5 * rho * (1.0 * 1.0 + 1.0 * 1.0)
+        _write_ascii_rows(os.path.join(d, fn), [x, y, rho, rho * 1.0, rho * 1.0, e], "%1.16E ")
+
+
+def example_vortex_initial(name: str, d: str, NI: int, NJ: int) -> None:
+    """Examples/2D_NavierStokes/InviscidVortexConvection: initial.inp and
+    exact.inp (the vortex advected by u_inf t, t = n_iter dt folded into one
+    period of 20) in binary, as its aux/init.c writes them."""
+    if name != "InviscidVortexConvection":
+        raise ValueError(name)
+    pi, G = 4.0 * math.atan(1.0), 1.4
+    sk = _solver_keys(d)
+    tf = float(int(sk["n_iter"])) * float(sk["dt"])
+    while tf > 20:
+        tf -= 20
+    dx, dy = 10.0 / NI, 10.0 / NJ
+    x = np.array([i * dx for i in range(NI)])
+    y = np.array([j * dy for j in range(NJ)])
+    b = 0.5
+
+    def state(x0, y0):
+        U = np.zeros((NJ, NI, 4))
+        for i in range(NI):
+            for j in range(NJ):
+                rx, ry = x[i] - x0, y[j] - y0
+                if rx < -5:
+                    rx += 10
+                elif rx > 5:
+                    rx -= 10
+                rsq = rx * rx + ry * ry
+                rho = math.exp((1.0 / (G - 1.0)) * math.log(1.0 - ((G - 1.0) * b * b) / (8.0 * G * pi * pi)
+                                                               * math.exp(1.0 - rsq)))
+                P = math.exp(G * math.log(rho))
+                du = -b / (2.0 * pi) * math.exp(0.5 * (1.0 - rsq)) * ry
+                dv = b / (2.0 * pi) * math.exp(0.5 * (1.0 - rsq)) * rx
+                u, v = 0.5 + du, 0.0 + dv
+                U[j, i] = (rho, rho * u, rho * v, P / (G - 1.0) + 0.5 * rho * (u * u + v * v))
+        return U.reshape(-1, 4)
+
+    _write_binary_initial(d, x, y, state(5.0, 5.0))
+    x0 = 5.0 + tf * 0.5
+    if x0 > 10:
+        x0 -= 10
+    with open(os.path.join(d, "exact.inp"), "wb") as f:
+        x.tofile(f)
+        y.tofile(f)
+        state(x0, 5.0).tofile(f)
+
+
 def shipped_example_initial(name: str, d: str, NI: int, NJ: int) -> None:
     """initial.inp of a shipped 2D example, written as its aux/init program
     writes it (the deck files, chemistry.inp included, already in `d`)."""
@@ -470,6 +654,14 @@ def shipped_example_initial(name: str, d: str, NI: int, NJ: int) -> None:
         example_2d_initial(name, d, NI, NJ)
     elif name in ("Uniform_Supersonic_Jet", "Uniform_Subsonic_Jet"):
         example_jet_initial(name, d, NI, NJ)
+    elif name.startswith("BeamInducedDensityMod"):
+        example_beam_initial(name, d, NI, NJ)
+    elif name.startswith("PressurePerturbation"):
+        example_pressure_initial(name, d, NI, NJ)
+    elif name.startswith("DensitySineWave"):
+        example_sine_initial(name, d, NI, NJ)
+    elif name == "InviscidVortexConvection":
+        example_vortex_initial(name, d, NI, NJ)
     else:
         example_wall_initial(name, d, NI, NJ)
 
@@ -504,6 +696,24 @@ def read_binary_op(path: str):
         x = np.fromfile(f, dtype=np.float64, count=int(dims.sum()))
         u = np.fromfile(f, dtype=np.float64, count=int(np.prod(dims)) * int(nvars))
     return int(ndims), int(nvars), [int(d) for d in dims], x, u.reshape(-1, int(nvars))
+
+
+def read_solution_file(path: str, nv: int) -> np.ndarray:
+    """A solution file the reference writes, as (npoints, nvars): binary
+    (WriteBinary.c:37-93), or text / Tecplot (WriteText.c, WriteTecplot2D.c:
+    one row per point, x fastest, the nvars values last; header lines are
+    skipped)."""
+    if path.endswith(".bin"):
+        return read_binary_op(path)[4].reshape(-1, nv)
+    rows = []
+    for line in open(path):
+        try:
+            vals = [float(v) for v in line.split()]
+        except ValueError:
+            continue
+        if len(vals) >= nv:
+            rows.append(vals[-nv:])
+    return np.array(rows)
 
 
 def isentropic_vortex(N: int = 60, n_iter: int = 400, dt: float = 0.05, scheme: str = "crweno5",

@@ -65,6 +65,8 @@ def test_reference_example_decks_parse(example):
     (lambda dk: dk.solver.__setitem__("par_space_scheme", "3"), "parabolic"),
     (lambda dk: dk.solver.__setitem__("hyp_interp_type", "upwind-char"), "not a supported interpolation"),
     (lambda dk: dk.boundary.__setitem__(0, ("thermal-wall", 0, 1, 0, 0, 0, 0)), "not supported"),
+    (lambda dk: (dk.solver.__setitem__("size", (16, 3)), setattr(dk, "x", [dk.x[0], dk.x[1][:3]]),
+                 setattr(dk, "u", dk.u[:48]), setattr(dk, "imap", dk.imap[:48])), "grid too small"),
 ])
 def test_deck_errors_are_reported(mutate, msg):
     dk = decks.ns2d_photochem(nx=16, ny=16, n_iter=1)

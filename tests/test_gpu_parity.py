@@ -95,6 +95,7 @@ FIXTURES = [
     "ns2d_crweno_char_flow_20x16", "ns2d_chem_crweno_char_rf_flow_16x12", "ns2d_crweno_char_llf_mapped_inout_20x16",
     "sod_upw5_char_roe_N64", "sod_upw5_comp_rf_N64", "ns2d_chem_upw5_comp_roe_20x16",
     "ns2d_upw5_char_llf_flow_20x16", "ns2d_upw5_cavity_re400_20x20",
+    "ns2d_chem_crweno_16x4", "ns2d_chem_char_rf_24x4", "ns2d_crweno_char_flow_16x4",
 ]
 
 
@@ -138,8 +139,10 @@ def test_shipped_logs(examples, example):
         assert _close_in_last_digit(norm[it - 1], nm, "%1.4E"), (it, norm[it - 1], nm)
 
 
-SHIPPED_2D = ["Riemann2DCase4", "RadialExpansionWave", "LidDrivenCavity", "Uniform_Supersonic_Jet",
-              "Uniform_Subsonic_Jet", "FlatPlateLaminar"]
+SHIPPED = ["Riemann2DCase4", "RadialExpansionWave", "LidDrivenCavity", "Uniform_Supersonic_Jet",
+           "Uniform_Subsonic_Jet", "FlatPlateLaminar", "BeamInducedDensityMod_1D_x", "PressurePerturbation_1D_x",
+           "DensitySineWave_2D", "InviscidVortexConvection", "BeamInducedDensityMod", "PressurePerturbation",
+           "DensitySineWave_1D"]
 
 
 def _run_shipped(example, d, ni, nj, solver_set=None):
@@ -166,10 +169,11 @@ def _screen_rows(out):
     return rows
 
 
-@pytest.mark.parametrize("example", SHIPPED_2D)
-def test_shipped_2d_examples_through_the_driver(examples, example):
-    """A shipped 2D example as a user runs it: its own decks and the initial
-    state its aux/init program writes, through the command-line driver.
+@pytest.mark.parametrize("example", SHIPPED)
+def test_shipped_examples_through_the_driver(examples, example):
+    """A shipped example as a user runs it: its own decks and the initial
+    state (and exact solution or beam map) its aux programs write, through the
+    command-line driver.
     Riemann2DCase4 / RadialExpansionWave: characteristic WENO5, mapped
     weights, rf-char, SSPRK3, mirror BCs, conservation check, Tecplot output.
     LidDrivenCavity: upw5, no-slip walls with a moving lid, Re 3200.  The
@@ -177,7 +181,12 @@ def test_shipped_2d_examples_through_the_driver(examples, example):
     inflow / outflow and subsonic inflow / extrapolation in y, periodic in x,
     numbered binary output and species files (the subsonic jet through its
     photolysis pulse).  FlatPlateLaminar: WENO5 components, Roe, slip and
-    no-slip walls, subsonic inflow / outflow, Re 1e5.  Against the
+    no-slip walls, subsonic inflow / outflow, Re 1e5.  The beam-induced
+    density modulation with photochemistry (2D, and 1D Euler with three
+    z-layers), the pressure perturbations (conservation check), the density
+    sine waves (text output, errors against exact.inp) and the isentropic
+    vortex (CRWENO5, SSPRK3, Tecplot output, errors and conservation).
+    Against the
     reference's run of the same directory (a shortened n_iter): every screen
     line to its printed digits, the names of the files written, the
     conservation errors, and the final solution (norms, a subsample, row and
@@ -189,14 +198,12 @@ def test_shipped_2d_examples_through_the_driver(examples, example):
         out, made = _run_shipped(example, d, ni, nj)
         assert made == ref["files"]
         ops = [f for f in made if f.startswith("op") and not f.startswith("op_species")]
-        last = os.path.join(d, ops[-1])
-        if last.endswith(".dat"):
-            lines = [l.split() for l in open(last).read().splitlines()[2:] if l.strip()]
-            U = np.array([[float(t) for t in l[4:]] for l in lines]).reshape(nj, ni, -1)
-        else:
-            U = decks.read_binary_op(last)[4].reshape(nj, ni, -1)
+        nv = len(z[example + "_norm"])
+        U = decks.read_solution_file(os.path.join(d, ops[-1]), nv).reshape(nj, ni, nv)
         cons = ([float(v) for v in open(os.path.join(d, "conservation.dat")).read().split()[5:]]
                 if ref["conservation"] is not None else None)
+        nd = 1 if nj == 1 else 2
+        errs = [float(v) for v in open(os.path.join(d, "errors.dat")).read().split()][2 * nd + 1:2 * nd + 4]
     rows = _screen_rows(out)
     assert sorted(rows) == [row[0] for row in ref["rows"]]
     for it, _, cf, nm in ref["rows"]:
@@ -204,7 +211,8 @@ def test_shipped_2d_examples_through_the_driver(examples, example):
         assert _close_in_last_digit(rows[it][1], nm, "%1.4E"), (it, rows[it][1], nm)
     if cons is not None:
         np.testing.assert_allclose(cons, ref["conservation"], rtol=0, atol=1e-11)
-    nv = U.shape[-1]
+    if "errors" in ref:  # L1, L2, Linf against the example's exact.inp (zeros without one)
+        np.testing.assert_allclose(errs, ref["errors"], rtol=1e-9, atol=1e-15)
     norm = np.sqrt((U.reshape(-1, nv) ** 2).sum(0))
     assert np.all(np.abs(norm - z[example + "_norm"]) <= 1e-12 * z[example + "_norm"])
     st = ref["stride"]
@@ -215,7 +223,7 @@ def test_shipped_2d_examples_through_the_driver(examples, example):
 
 
 def test_vortex_errors_match_errors_dat(examples):
-    ref = examples["InviscidVortexConvection"]["errors"]
+    ref = examples["InviscidVortexConvection_errors_dat"]["errors"]
     dk = decks.isentropic_vortex(N=60, n_iter=400, dt=0.05)
     with tempfile.TemporaryDirectory() as d:
         dk.write(d)

@@ -43,6 +43,7 @@ def _run_oracle(dk, n=None):
     "ns2d_crweno_char_flow_20x16", "ns2d_chem_crweno_char_rf_flow_16x12", "ns2d_crweno_char_llf_mapped_inout_20x16",
     "sod_upw5_char_roe_N64", "sod_upw5_comp_rf_N64", "ns2d_chem_upw5_comp_roe_20x16",
     "ns2d_upw5_char_llf_flow_20x16", "ns2d_upw5_cavity_re400_20x20",
+    "ns2d_chem_crweno_16x4", "ns2d_chem_char_rf_24x4", "ns2d_crweno_char_flow_16x4",
 ])
 def test_oracle_matches_reference_bitwise(ref_runs, name):
     data, meta = ref_runs
@@ -67,7 +68,7 @@ def test_oracle_matches_shipped_logs(examples, example):
 
 def test_oracle_vortex_errors_match_errors_dat(examples):
     """CRWENO5 + tridiagonal + SSPRK3 + Roe + mapped weights, 60^2, t = 20."""
-    ref = examples["InviscidVortexConvection"]["errors"]
+    ref = examples["InviscidVortexConvection_errors_dat"]["errors"]
     dk = decks.isentropic_vortex(N=60, n_iter=400, dt=0.05)
     u = _run_oracle(dk)
     e = decks.error_norms(u, dk.u)
