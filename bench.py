@@ -36,18 +36,22 @@ HBM_PEAK_GBS = 8000.0
 STEP_BYTES_PER_CELL = lambda nv, ns: 40 * nv * ns  # noqa: E731
 
 
-def kernel_algorithmic_bytes(name, nx, ny, nv, g, nstages, crweno):
+def kernel_algorithmic_bytes(name, nx, ny, nv, g, nstages, crweno, paired=True):
     """Compulsory bytes of one launch of a kernel class (its inputs read once
     and outputs written once).  One launch covers both sweeps (x and y) of an
     nx x ny grid; per sweep the counts below are for `lines` lines of N
-    interior points (the two sweeps have the same counts on a square grid)."""
+    interior points (the two sweeps have the same counts on a square grid).
+    paired: component-wise interpolation with Roe or Rusanov, where the
+    reconstruction writes only the two combinations 0.5 (uR - uL) and
+    0.5 (fL + fR) that the upwinding reads, instead of uL, uR, fL and fR."""
     lines, N = ny, nx
     pos = N + 2 * g
     sweeps = 2
-    if name == "recon":   # u and f at every position of every line; uL uR fL fR
-        return sweeps * 8 * (2 * nv * pos * lines + 4 * nv * (N + 1) * lines)
-    if name == "upwind":  # uL uR fL fR + 4 flow vars at the two cells; fI
-        return sweeps * 8 * (4 * nv * (N + 1) * lines + 2 * 4 * (N + 1) * lines + nv * (N + 1) * lines)
+    nio = 2 if paired else 4
+    if name == "recon":   # u and f at every position of every line; the interface arrays
+        return sweeps * 8 * (2 * nv * pos * lines + nio * nv * (N + 1) * lines)
+    if name == "upwind":  # the interface arrays + 4 flow vars at the two cells; fI
+        return sweeps * 8 * (nio * nv * (N + 1) * lines + 2 * 4 * (N + 1) * lines + nv * (N + 1) * lines)
     if name == "flux":
         return sweeps * 8 * 2 * nv * pos * lines
     if name == "stage_bc":  # u + up to one Udot read, U and UT written (ghosted)
@@ -354,7 +358,8 @@ def main():
                         "viscflux": ns, "assemble": ns, "update": 1}
             share = {k: v * per_step.get(k, 1) for k, v in ktimes.items()}
             dom = max(share.items(), key=lambda kv: kv[1])[0]
-            byts = kernel_algorithmic_bytes(dom, nx, nx, nv, g, inf.nstages, True)
+            paired = (not inf.characteristic) and dk.physics["upwinding"] in ("roe", "rusanov")
+            byts = kernel_algorithmic_bytes(dom, nx, nx, nv, g, inf.nstages, True, paired)
             ms = ktimes[dom]
             ach = byts / (ms * 1e-3) / 1e9 if byts else None
             line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -89,7 +89,7 @@ struct Engine {
   double time_class(int cls, int reps);
   int autotune(const int *cands, int ncand, int reps, int *best_rows, double *best_ms);
   int sync();
-  void mark(int cls, bool begin);
+  void mark(int cls, bool begin, hipStream_t st);
   int fail(int code, const std::string &msg);
   void release_plan();
 };

@@ -2377,29 +2377,29 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
 
 typedef void (*KernFn)(const CaseDev *, const int *, const int *, int);
 
-static void run(KernFn k, int cls, const CaseDev *d, const Plan &pl, int s, hipStream_t st,
-                void (*mark)(void *, int, bool), void *ctx) {
+static void run(KernFn k, int cls, const CaseDev *d, const Plan &pl, int s, hipStream_t st, MarkFn mark,
+                void *ctx) {
   if (pl.nblocks[cls] <= 0) return;
-  if (mark) mark(ctx, cls, true);
+  if (mark) mark(ctx, cls, true, st);
   hipLaunchKernelGGL(k, dim3(pl.nblocks[cls]), dim3(TB), 0, st, d, pl.blockcase[cls], pl.firstblock[cls], s);
-  if (mark) mark(ctx, cls, false);
+  if (mark) mark(ctx, cls, false, st);
 }
 
 // k_qderiv and k_viscflux read only the stage state (after k_ghost) and
 // write only Qd and FV, which nothing before k_assemble reads; the
 // reconstruction / upwinding chain neither reads nor writes Qd or FV.
-void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, void (*mark)(void *, int, bool),
-                  void *ctx, const StageFork *fk) {
+void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, MarkFn mark, void *ctx,
+                  const StageFork *fk) {
   run(k_stage, K_STAGE, d, pl, s, st, mark, ctx);
   run(k_ghost, K_GHOST, d, pl, s, st, mark, ctx);
   run(k_flux, K_FLUX, d, pl, s, st, mark, ctx);
   run(k_recon_ends, K_ENDS, d, pl, s, st, mark, ctx);
-  const bool fork = fk && fk->side && !mark && (pl.nblocks[K_QDERIV] > 0 || pl.nblocks[K_VISCFLUX] > 0);
+  const bool fork = fk && fk->side && (pl.nblocks[K_QDERIV] > 0 || pl.nblocks[K_VISCFLUX] > 0);
   if (fork) {
     hipEventRecord(fk->fork, st);
     hipStreamWaitEvent(fk->side, fk->fork, 0);
-    run(k_qderiv, K_QDERIV, d, pl, s, fk->side, nullptr, nullptr);
-    run(k_viscflux, K_VISCFLUX, d, pl, s, fk->side, nullptr, nullptr);
+    run(k_qderiv, K_QDERIV, d, pl, s, fk->side, mark, ctx);
+    run(k_viscflux, K_VISCFLUX, d, pl, s, fk->side, mark, ctx);
     hipEventRecord(fk->join, fk->side);
   }
   run(k_recon, K_RECON, d, pl, s, st, mark, ctx);

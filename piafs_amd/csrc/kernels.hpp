@@ -155,9 +155,11 @@ struct StageFork {
 // Stage s of the RK step for every case in the plan: stage state + BCs,
 // fluxes, reconstruction, upwinding, divergence, viscous terms, assembly of
 // Udot[s].  Times: stage time = *time + c[s]*dt of each case.  With `fk`
-// (and no per-class timing marks) the viscous branch runs on fk->side.
-void launch_stage(const CaseDev *d_cases, const Plan &plan, int s, hipStream_t st, void (*mark)(void *, int, bool),
-                  void *mark_ctx, const StageFork *fk = nullptr);
+// the viscous branch runs on fk->side.  `mark` (per-class timing) is called
+// before and after each launch with the stream the launch goes to.
+typedef void (*MarkFn)(void *ctx, int cls, bool begin, hipStream_t st);
+void launch_stage(const CaseDev *d_cases, const Plan &plan, int s, hipStream_t st, MarkFn mark, void *mark_ctx,
+                  const StageFork *fk = nullptr);
 // u += dt*b_s*Udot_s for every stage, time += dt.
 void launch_update(const CaseDev *d_cases, const Plan &plan, hipStream_t st);
 

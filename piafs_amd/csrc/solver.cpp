@@ -430,9 +430,11 @@ int Engine::build() {
   return 0;
 }
 
-static void mark_cb(void *ctx, int cls, bool begin) { ((Engine *)ctx)->mark(cls, begin); }
+static void mark_cb(void *ctx, int cls, bool begin, hipStream_t st) { ((Engine *)ctx)->mark(cls, begin, st); }
 
-void Engine::mark(int cls, bool begin) {
+// Events bracketing one launch of class cls, recorded on the stream the launch
+// goes to (the viscous branch runs on the side stream beside the main one).
+void Engine::mark(int cls, bool begin, hipStream_t st) {
   if (!profiling) return;
   if (begin) {
     if (ev_used >= ev.size()) {
@@ -443,9 +445,9 @@ void Engine::mark(int cls, bool begin) {
       ev_cls.push_back(cls);
     }
     ev_cls[ev_used] = cls;
-    hipEventRecord(ev[ev_used].first, stream);
+    hipEventRecord(ev[ev_used].first, st);
   } else {
-    hipEventRecord(ev[ev_used].second, stream);
+    hipEventRecord(ev[ev_used].second, st);
     ev_used++;
   }
 }
@@ -455,9 +457,9 @@ void Engine::enqueue_step() {
   for (Case *c : cases) ns = std::max(ns, c->cfg.nstages);
   for (int s = 0; s < ns; s++)
     launch_stage(d_cases, plan, s, stream, profiling ? mark_cb : nullptr, this, fork.side ? &fork : nullptr);
-  if (profiling) mark(K_UPDATE, true);
+  if (profiling) mark(K_UPDATE, true, stream);
   launch_update(d_cases, plan, stream);
-  if (profiling) mark(K_UPDATE, false);
+  if (profiling) mark(K_UPDATE, false, stream);
 }
 
 // Capture one step of the current member set into a graph (once per member set).
