@@ -81,6 +81,38 @@ int  piafs_sync(piafs_case *c);
  * SourceFunction). */
 int  piafs_rhs(piafs_case *c, const double *u_host, double t, double *rhs_host);
 
+/* Per-operator entry points: the pieces of one right-hand-side evaluation
+ * of the interior state u_host at time t (the case's own solution and time are
+ * left as they were).  "Ghosted" arrays hold every point of the grid with its
+ * ghost layers, x fastest, the nvars components of a point contiguous: the
+ * layout of HyPar::u (include/hypar.h:106-111).
+ *
+ * The state after the boundary fill.  Replaces ApplyBoundaryConditions
+ * (src/HyParFunctions/ApplyBoundaryConditions.c:31-54) as
+ * TimeRHSFunctionExplicit.c:45 calls it; corner points stay zero. */
+int  piafs_boundary_fill(piafs_case *c, const double *u_host, double t, double *u_ghosted);
+/* The physical flux along dimension dir at every position of every interior
+ * line along dir (ghosts along dir included; the other points are zero), after
+ * the boundary fill.  Replaces the FluxFunction argument, HyPar::FFunction =
+ * NavierStokes2DFlux (src/PhysicalModels/NavierStokes2D/NavierStokes2DFlux.c:24-52)
+ * / Euler1DFlux (src/PhysicalModels/Euler1D/Euler1DFlux.c:19), as
+ * src/HyParFunctions/HyperbolicFunction.c:84 calls it.  Ghosted layout. */
+int  piafs_flux(piafs_case *c, const double *u_host, double t, int dir, double *f_ghosted);
+/* The upwind interface flux fI of every interior line along dir, as
+ * [line][interface 0..N][var].  Replaces ReconstructHyperbolic
+ * (src/HyParFunctions/HyperbolicFunction.c:168-214: InterpolateInterfacesHyp
+ * for uL, uR, fL, fR, then the UpwindFunction argument, e.g.
+ * NavierStokes2DUpwindRoe NavierStokes2DUpwind.c:26-92). */
+int  piafs_interface_flux(piafs_case *c, const double *u_host, double t, int dir, double *fi);
+/* The three terms of the right-hand side at the interior points (WriteBinary
+ * order; any output may be NULL): hyp, the flux divergence that
+ * TimeRHSFunctionExplicit subtracts (HyperbolicFunction,
+ * src/HyParFunctions/HyperbolicFunction.c:34-127); par, the viscous term
+ * (ParabolicFunction = NavierStokes2DParabolicFunction.c:41-204; zero for
+ * euler1d); src, the photochemistry and sponge source (SourceFunction,
+ * src/HyParFunctions/SourceFunction.c:21-60).  rhs = -hyp + par + src. */
+int  piafs_rhs_terms(piafs_case *c, const double *u_host, double t, double *hyp, double *par, double *src);
+
 /* Write the current solution in the reference's binary layout.  Replaces
  * WriteBinary (src/IOFunctions/WriteBinary.c:37-93) as called from
  * OutputSolution (src/Simulation/OutputSolution.cpp:25). */

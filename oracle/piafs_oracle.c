@@ -74,6 +74,9 @@ struct orc_case {
   double *U[4], *Udot[4], *hyp_a, *par_a, *src_a, *fluxC, *fluxI;
   double *uL, *uR, *fL, *fR;
   double *Q, *QdX, *QdY, *FV, *FD;
+  /* optional capture of the interface fluxes of every line, per dimension
+   * ([line][interface][var]), set by orc_rhs_parts */
+  double *fi_cap[2];
 };
 
 /* ------------------------------------------------------------------ */
@@ -1485,6 +1488,7 @@ static void hyperbolic(orc_case *c, const double *u, double *hyp)
       else if (c->is_char) recon_line_char(c, c->fluxC, u, base, st, N, d, uLl, uRl, fLl, fRl, (size_t)nv);
       else recon_line_components(c, c->fluxC, u, base, st, N, uLl, uRl, fLl, fRl, (size_t)nv);
       upwind_line(c, u, base, st, N, d, uLl, uRl, fLl, fRl, fIl, (size_t)nv);
+      if (c->fi_cap[d]) memcpy(c->fi_cap[d] + (size_t)l * (N + 1) * nv, fIl, (size_t)(N + 1) * nv * sizeof(double));
       /* hyp += dxinv * (fI[k+1] - fI[k]) at the N interior points of the line */
       for (k = 0; k < N; k++) {
         int pp = base + k * st;
@@ -1785,6 +1789,38 @@ static void interior_copy(const orc_case *c, double *dst, const double *src, int
 
 void orc_get_solution(const orc_case *c, double *u_out) { interior_copy(c, u_out, c->u, 1); }
 void orc_set_solution(orc_case *c, const double *u_in) { interior_copy(c, c->u, u_in, 0); }
+
+/* The pieces of TimeRHSFunctionExplicit.c:29-77 at time t (see the header) */
+int orc_rhs_parts(orc_case *c, const double *u_in, double t, double *u_ghosted, double *flux0, double *flux1,
+                  double *fi0, double *fi1, double *hyp, double *par, double *src)
+{
+  size_t n = (size_t)c->npts * c->nvars, p;
+  int d;
+  double *w = (double *)calloc(n, sizeof(double));
+  double *fl[2] = {flux0, flux1};
+  interior_copy(c, w, u_in, 0);
+  apply_bcs(c, w);
+  if (u_ghosted) memcpy(u_ghosted, w, n * sizeof(double));
+  for (d = 0; d < c->ndims; d++) {
+    if (!fl[d]) continue;
+    for (p = 0; p < (size_t)c->npts; p++) {
+      if (c->model_id == 1) flux_euler1d(c, w + p * c->nvars, fl[d] + p * c->nvars);
+      else flux_ns2d(c, w + p * c->nvars, fl[d] + p * c->nvars, d);
+    }
+  }
+  c->fi_cap[0] = fi0;
+  c->fi_cap[1] = c->ndims > 1 ? fi1 : NULL;
+  hyperbolic(c, w, c->hyp_a);
+  c->fi_cap[0] = c->fi_cap[1] = NULL;
+  if (hyp) interior_copy(c, hyp, c->hyp_a, 1);
+  memset(c->par_a, 0, n * sizeof(double));
+  if (c->model_id == 2) parabolic_ns2d(c, w, c->par_a);
+  if (par) interior_copy(c, par, c->par_a, 1);
+  source(c, w, t, c->src_a);
+  if (src) interior_copy(c, src, c->src_a, 1);
+  free(w);
+  return 0;
+}
 
 int orc_rhs(orc_case *c, const double *u_in, double t, double *rhs_out)
 {

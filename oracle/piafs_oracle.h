@@ -58,6 +58,17 @@ void orc_set_solution(orc_case *c, const double *u_in);
  * (TimeRHSFunctionExplicit.c:29-77).  Output interior only, same order. */
 int  orc_rhs(orc_case *c, const double *u_in, double t, double *rhs_out);
 
+/* The pieces of the same evaluation, for unit parity of the per-operator
+ * entry points (any output may be NULL): the state after
+ * ApplyBoundaryConditions (ghosted, npoints-with-ghosts x nvars, AoS);
+ * the flux along dimensions 0 and 1 at every ghosted point (FFunction:
+ * NavierStokes2DFlux.c:24-52, Euler1DFlux.c:19); the upwind interface flux
+ * of every line along dimensions 0 and 1 ([line][interface][var], the fI of
+ * HyperbolicFunction.c:68-81); and the hyperbolic, parabolic and source terms
+ * at the interior points (WriteBinary order). */
+int  orc_rhs_parts(orc_case *c, const double *u_in, double t, double *u_ghosted, double *flux0, double *flux1,
+                   double *fi0, double *fi1, double *hyp, double *par, double *src);
+
 /* ---- function-level entry points used by unit tests ---- */
 
 /* tridiagLU serial path (tridiagLU.c:86-277): n rows, ns systems,

@@ -46,7 +46,8 @@ EXPORTS = [
     "piafs_batch_steps_done", "piafs_batch_download", "piafs_batch_info", "piafs_batch_last_error",
     "piafs_estimate_bytes", "piafs_autotune", "piafs_batch_autotune", "piafs_server_create", "piafs_server_destroy", "piafs_server_submit",
     "piafs_server_poll", "piafs_server_result", "piafs_server_release", "piafs_server_wait_all",
-    "piafs_server_stats", "piafs_server_last_error",
+    "piafs_server_stats", "piafs_server_last_error", "piafs_boundary_fill", "piafs_flux",
+    "piafs_interface_flux", "piafs_rhs_terms",
 ]
 
 
@@ -71,6 +72,10 @@ def lib():
     L.piafs_step_diag.argtypes = [vp, ip, vp, vp]
     L.piafs_sync.argtypes = [vp]
     L.piafs_rhs.argtypes = [vp, vp, ctypes.c_double, vp]
+    L.piafs_boundary_fill.argtypes = [vp, vp, ctypes.c_double, vp]
+    L.piafs_flux.argtypes = [vp, vp, ctypes.c_double, ip, vp]
+    L.piafs_interface_flux.argtypes = [vp, vp, ctypes.c_double, ip, vp]
+    L.piafs_rhs_terms.argtypes = [vp, vp, ctypes.c_double, vp, vp, vp]
     L.piafs_write_binary.argtypes = [vp, cp]
     L.piafs_solve.argtypes = [vp, cp, ip, dp]
     L.piafs_kernel_times.argtypes = [vp, ip, ctypes.POINTER(cp), dp, ctypes.POINTER(ctypes.c_int)]
@@ -182,6 +187,46 @@ class Case:
         out = np.empty(self.shape)
         self._check(self._L.piafs_rhs(self._h, _ptr(u), float(t), _ptr(out)))
         return out
+
+    def _ghosted_shape(self):
+        inf = self.info
+        g = inf.ghosts
+        npts = 1
+        for d in range(inf.ndims):
+            npts *= inf.dim[d] + 2 * g
+        return (npts, inf.nvars)
+
+    def boundary_fill(self, u, t: float) -> np.ndarray:
+        """The ghosted state after the boundary fill (piafs_boundary_fill)."""
+        u = _f64(u).reshape(self.shape)
+        out = np.empty(self._ghosted_shape())
+        self._check(self._L.piafs_boundary_fill(self._h, _ptr(u), float(t), _ptr(out)))
+        return out
+
+    def flux(self, u, t: float, dim: int) -> np.ndarray:
+        """The flux along dimension dim on the ghosted grid (piafs_flux)."""
+        u = _f64(u).reshape(self.shape)
+        out = np.empty(self._ghosted_shape())
+        self._check(self._L.piafs_flux(self._h, _ptr(u), float(t), int(dim), _ptr(out)))
+        return out
+
+    def interface_flux(self, u, t: float, dim: int) -> np.ndarray:
+        """The upwind interface flux of every line along dim, [line][interface][var]
+        (piafs_interface_flux)."""
+        u = _f64(u).reshape(self.shape)
+        inf = self.info
+        n = inf.dim[dim]
+        lines = inf.dim[1 - dim] if inf.ndims == 2 else 1
+        out = np.empty((lines, n + 1, inf.nvars))
+        self._check(self._L.piafs_interface_flux(self._h, _ptr(u), float(t), int(dim), _ptr(out)))
+        return out
+
+    def rhs_terms(self, u, t: float):
+        """hyp, par, src with rhs = -hyp + par + src (piafs_rhs_terms)."""
+        u = _f64(u).reshape(self.shape)
+        hyp, par, src = (np.empty(self.shape) for _ in range(3))
+        self._check(self._L.piafs_rhs_terms(self._h, _ptr(u), float(t), _ptr(hyp), _ptr(par), _ptr(src)))
+        return hyp, par, src
 
     def write_binary(self, path: str) -> None:
         self._check(self._L.piafs_write_binary(self._h, path.encode()))

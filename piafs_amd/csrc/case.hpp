@@ -85,7 +85,7 @@ struct Engine {
   int prepare();               // capture the step graph now (when graphs are used)
   void enqueue_step();         // eager launches of one full step
   int step(int n);             // n steps (graph replay when enabled)
-  int stage_only(int s);       // one stage (used for a single RHS evaluation)
+  int stage_only(int s, int last = K_NCLASS);  // one stage, or its classes up to `last`
   double time_class(int cls, int reps);
   int autotune(const int *cands, int ncand, int reps, int *best_rows, double *best_ms);
   int sync();

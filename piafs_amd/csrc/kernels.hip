@@ -2148,9 +2148,9 @@ __global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
     for (int v = 0; v < nv; v++) {
       const long a = (long)v * (G.nx + 1) + i;
       const double hyp = dxi * (C.fIx[a + 1] - C.fIx[a]);
-      udot[(long)v * G.plane + p] = -hyp;
+      udot[(long)v * G.plane + p] = (C.terms & 1) ? -hyp : 0.0;
     }
-    add_source(C, stage, i, 0, p, udot);
+    if (C.terms & 4) add_source(C, stage, i, 0, p, udot);
     return;
   }
   // 2D: a 32 x 32 tile of points, four per thread
@@ -2160,7 +2160,7 @@ __global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
   const int i0 = (id % ntx) * 32, j0 = (id / ntx) * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int i = i0 + tx;
-  const bool visc = C.P.viscous;
+  const bool visc = C.P.viscous && (C.terms & 2);
   double par[4][3];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
@@ -2189,14 +2189,14 @@ __global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
         const long a = ((long)v * G.ny + j) * (G.nx + 1) + i;
         double hyp = dxi * (C.fIx[a + 1] - C.fIx[a]);
         hyp = hyp + C.dxinv[(G.nx + 2 * G.g) + G.g + j] * (tl[tx][jl + 1] - tl[tx][jl]);
-        double r = -hyp;
+        double r = (C.terms & 1) ? -hyp : 0.0;
         if (visc && v >= 1 && v <= 3) r = r + par[q][v - 1];
         udot[(long)v * G.plane + (long)(j + G.gy) * G.nxg + (i + G.g)] = r;
       }
     }
     __syncthreads();
   }
-  if (C.P.chem || C.sponge) {
+  if ((C.P.chem || C.sponge) && (C.terms & 4)) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int j = j0 + ty + 8 * q;
@@ -2389,32 +2389,36 @@ static void run(KernFn k, int cls, const CaseDev *d, const Plan &pl, int s, hipS
 // write only Qd and FV, which nothing before k_assemble reads; the
 // reconstruction / upwinding chain neither reads nor writes Qd or FV.
 void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, MarkFn mark, void *ctx,
-                  const StageFork *fk) {
-  run(k_stage, K_STAGE, d, pl, s, st, mark, ctx);
-  run(k_ghost, K_GHOST, d, pl, s, st, mark, ctx);
-  run(k_flux, K_FLUX, d, pl, s, st, mark, ctx);
-  run(k_recon_ends, K_ENDS, d, pl, s, st, mark, ctx);
-  const bool fork = fk && fk->side && (pl.nblocks[K_QDERIV] > 0 || pl.nblocks[K_VISCFLUX] > 0);
+                  const StageFork *fk, int last) {
+  auto R = [&](KernFn k, int cls, hipStream_t on) {
+    if (cls <= last) run(k, cls, d, pl, s, on, mark, ctx);
+  };
+  R(k_stage, K_STAGE, st);
+  R(k_ghost, K_GHOST, st);
+  R(k_flux, K_FLUX, st);
+  R(k_recon_ends, K_ENDS, st);
+  const bool fork = fk && fk->side && last >= K_ASSEMBLE &&
+                    (pl.nblocks[K_QDERIV] > 0 || pl.nblocks[K_VISCFLUX] > 0);
   if (fork) {
     hipEventRecord(fk->fork, st);
     hipStreamWaitEvent(fk->side, fk->fork, 0);
-    run(k_qderiv, K_QDERIV, d, pl, s, fk->side, mark, ctx);
-    run(k_viscflux, K_VISCFLUX, d, pl, s, fk->side, mark, ctx);
+    R(k_qderiv, K_QDERIV, fk->side);
+    R(k_viscflux, K_VISCFLUX, fk->side);
     hipEventRecord(fk->join, fk->side);
   }
-  run(k_recon, K_RECON, d, pl, s, st, mark, ctx);
-  run(k_recon_char, K_RCHAR, d, pl, s, st, mark, ctx);
-  run(k_cw_rows, K_CWROWS, d, pl, s, st, mark, ctx);
-  run(k_cw_solve, K_CWSOLVE, d, pl, s, st, mark, ctx);
-  run(k_upwind, K_UPWIND, d, pl, s, st, mark, ctx);
-  run(k_bflux, K_BFLUX, d, pl, s, st, mark, ctx);
+  R(k_recon, K_RECON, st);
+  R(k_recon_char, K_RCHAR, st);
+  R(k_cw_rows, K_CWROWS, st);
+  R(k_cw_solve, K_CWSOLVE, st);
+  R(k_upwind, K_UPWIND, st);
+  R(k_bflux, K_BFLUX, st);
   if (fork) {
     hipStreamWaitEvent(st, fk->join, 0);
   } else {
-    run(k_qderiv, K_QDERIV, d, pl, s, st, mark, ctx);
-    run(k_viscflux, K_VISCFLUX, d, pl, s, st, mark, ctx);
+    R(k_qderiv, K_QDERIV, st);
+    R(k_viscflux, K_VISCFLUX, st);
   }
-  run(k_assemble, K_ASSEMBLE, d, pl, s, st, mark, ctx);
+  R(k_assemble, K_ASSEMBLE, st);
 }
 
 void launch_update(const CaseDev *d, const Plan &pl, hipStream_t st) { run(k_update, K_UPDATE, d, pl, 0, st, nullptr, nullptr); }

@@ -98,6 +98,7 @@ struct CaseDev {
   ReconParams R;
   int model, upwind, nstages;  // upwind: 0 Roe, 1 rf-char, 2 llf-char, 3 rusanov
   int paired;    // reconstruction writes 0.5*(uR-uL), 0.5*(fL+fR) only (Roe with component-wise interpolation)
+  int terms;     // terms k_assemble adds: bit 0 hyperbolic, bit 1 parabolic, bit 2 source (7 in a step)
   double dt, A[16], b[4], c[4];
   double *u, *U, *UT, *Udot[4];
   double *flux_x, *flux_y, *ix[4], *iy[4], *fIx, *fIy;
@@ -158,8 +159,10 @@ struct StageFork {
 // the viscous branch runs on fk->side.  `mark` (per-class timing) is called
 // before and after each launch with the stream the launch goes to.
 typedef void (*MarkFn)(void *ctx, int cls, bool begin, hipStream_t st);
+// With `last` < K_NCLASS only the classes up to `last` run (in launch order,
+// which is the order of KClass), for the per-operator entry points.
 void launch_stage(const CaseDev *d_cases, const Plan &plan, int s, hipStream_t st, MarkFn mark, void *mark_ctx,
-                  const StageFork *fk = nullptr);
+                  const StageFork *fk = nullptr, int last = K_NCLASS);
 // u += dt*b_s*Udot_s for every stage, time += dt.
 void launch_update(const CaseDev *d_cases, const Plan &plan, hipStream_t st);
 

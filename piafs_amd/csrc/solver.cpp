@@ -167,6 +167,7 @@ int Case::setup(int dev, hipStream_t st) {
     P.t_start_norm = ch.t_start_norm;
     P.t_pulse_norm = ch.t_pulse_norm;
   }
+  D.terms = 7;
   D.R.crweno = cfg.crweno ? 1 : 0;
   D.R.kind = cfg.weight_kind;
   D.R.no_limiting = cfg.no_limiting;
@@ -553,10 +554,10 @@ int Engine::autotune(const int *cands, int ncand, int reps, int *best_rows, doub
   return 0;
 }
 
-int Engine::stage_only(int s) {
+int Engine::stage_only(int s, int last) {
   HIPCHK(hipSetDevice(device));
   if (!built) return fail(-1, "the ensemble plan is not built");
-  launch_stage(d_cases, plan, s, stream, nullptr, nullptr);
+  launch_stage(d_cases, plan, s, stream, nullptr, nullptr, nullptr, last);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -762,6 +763,134 @@ int piafs_rhs(piafs_case *pc, const double *u, double t, double *rhs) {
   if ((rc = c->download(rhs, st, c->h.Udot[0]))) return case_fail(pc, rc);
   if (hipMemcpyAsync(c->h.u, c->uprev, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) return -1;
   return piafs_set_time(pc, t_saved);
+}
+
+}  // extern "C"
+
+// One evaluation of stage 0 at time t on the state u (interior, WriteBinary
+// order), up to kernel class `last`, then `body` reads what it needs; the
+// case's own solution and time are put back afterwards, as piafs_rhs does.
+template <class F>
+static int eval_pieces(piafs_case *pc, const double *u, double t, int last, int terms, F body) {
+  Case *c = pc->c;
+  hipStream_t st = pc->eng.stream;
+  const size_t n = (size_t)c->cfg.nvars * c->h.G.plane;
+  const double t_saved = c->t;
+  int rc = 0;
+  if (!u) {
+    pc->err = "state pointer is NULL";
+    return 1;
+  }
+  if (hipMemcpyAsync(c->uprev, c->h.u, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) return -1;
+  if ((rc = c->upload(u, st))) return case_fail(pc, rc);
+  if ((rc = piafs_set_time(pc, t))) return rc;
+  int *dterms = &pc->eng.d_cases[0].terms;
+  if (terms != 7 && hipMemcpy(dterms, &terms, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return -1;
+  // the stream is idle here (upload and set_time synchronise), so the
+  // blocking copies of the mask cannot race the kernels that read it
+  rc = pc->eng.stage_only(0, last);
+  if (!rc) rc = body();
+  const int all = 7;
+  hipStreamSynchronize(st);
+  if (terms != 7 && hipMemcpy(dterms, &all, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return -1;
+  if (rc) return case_fail(pc, rc);
+  if (hipMemcpyAsync(c->h.u, c->uprev, n * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess) return -1;
+  return piafs_set_time(pc, t_saved);
+}
+
+static int copy_to_host(Case *c, double *dst, const double *src, size_t n, hipStream_t st) {
+  const hipError_t e = hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, st);
+  const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(st) : e;
+  if (e2 != hipSuccess) {
+    c->err = std::string("device to host copy: ") + hipGetErrorString(e2);
+    return -1;
+  }
+  return 0;
+}
+
+// planar device array (nvars planes of the ghosted grid) -> host
+static int download_planes(Case *c, const double *src, std::vector<double> &buf, hipStream_t st) {
+  buf.resize((size_t)c->cfg.nvars * c->h.G.plane);
+  return copy_to_host(c, buf.data(), src, buf.size(), st);
+}
+
+extern "C" {
+
+int piafs_boundary_fill(piafs_case *pc, const double *u, double t, double *ug) {
+  Case *c = pc->c;
+  return eval_pieces(pc, u, t, K_GHOST, 7, [&]() {
+    std::vector<double> buf;
+    int rc = download_planes(c, c->h.U, buf, pc->eng.stream);
+    if (rc) return rc;
+    // the device planes are padded (plane >= nxg*nyg); the caller's array holds the nxg*nyg points
+    const Grid &G = c->h.G;
+    const long pl = G.plane, np = (long)G.nxg * G.nyg;
+    const int nv = c->cfg.nvars;
+    for (long p = 0; p < np; p++)
+      for (int v = 0; v < nv; v++) ug[(size_t)p * nv + v] = buf[(size_t)v * pl + p];
+    return 0;
+  });
+}
+
+int piafs_flux(piafs_case *pc, const double *u, double t, int dir, double *f) {
+  Case *c = pc->c;
+  if (dir < 0 || dir >= c->cfg.ndims) {
+    pc->err = "dir must be 0 .. ndims-1";
+    return 1;
+  }
+  return eval_pieces(pc, u, t, K_FLUX, 7, [&]() {
+    std::vector<double> buf;
+    int rc = download_planes(c, dir ? c->h.flux_y : c->h.flux_x, buf, pc->eng.stream);
+    if (rc) return rc;
+    const Grid &G = c->h.G;
+    const SweepGeom &S = dir ? c->h.Sy : c->h.Sx;
+    const int nv = c->cfg.nvars;
+    std::fill(f, f + (size_t)G.nxg * G.nyg * nv, 0.0);  // the caller's nxg*nyg points (the planes are padded)
+    for (int l = 0; l < S.nlines; l++)
+      for (int pos = -S.g; pos < S.N + S.g; pos++) {
+        const long b = S.base + (long)l * S.line_stride + pos;
+        const int i = dir ? l : pos, j = dir ? pos : l;  // point (i, j) of the natural ghosted grid
+        const long p = (long)(j + G.gy) * G.nxg + (i + G.g);
+        for (int v = 0; v < nv; v++) f[(size_t)p * nv + v] = buf[(size_t)v * S.plane + b];
+      }
+    return 0;
+  });
+}
+
+int piafs_interface_flux(piafs_case *pc, const double *u, double t, int dir, double *fi) {
+  Case *c = pc->c;
+  if (dir < 0 || dir >= c->cfg.ndims) {
+    pc->err = "dir must be 0 .. ndims-1";
+    return 1;
+  }
+  return eval_pieces(pc, u, t, K_UPWIND, 7, [&]() {
+    const SweepGeom &S = dir ? c->h.Sy : c->h.Sx;
+    const int nv = c->cfg.nvars, N = S.N, nl = S.nlines;
+    std::vector<double> buf((size_t)nv * nl * (N + 1));
+    const int rc = copy_to_host(c, buf.data(), dir ? c->h.fIy : c->h.fIx, buf.size(), pc->eng.stream);
+    if (rc) return rc;
+    for (int v = 0; v < nv; v++)
+      for (int l = 0; l < nl; l++)
+        for (int k = 0; k <= N; k++)
+          fi[((size_t)l * (N + 1) + k) * nv + v] = buf[((size_t)v * nl + l) * (N + 1) + k];
+    return 0;
+  });
+}
+
+int piafs_rhs_terms(piafs_case *pc, const double *u, double t, double *hyp, double *par, double *src) {
+  Case *c = pc->c;
+  double *out[3] = {hyp, par, src};
+  const size_t n = (size_t)c->cfg.npts_int * c->cfg.nvars;
+  for (int k = 0; k < 3; k++) {
+    if (!out[k]) continue;
+    const int rc = eval_pieces(pc, u, t, K_NCLASS, 1 << k, [&]() {
+      return c->download(out[k], pc->eng.stream, c->h.Udot[0]);
+    });
+    if (rc) return rc;
+    if (k == 0)  // k_assemble wrote -hyp
+      for (size_t q = 0; q < n; q++) hyp[q] = -hyp[q];
+  }
+  return 0;
 }
 
 int piafs_write_binary(piafs_case *pc, const char *path) {

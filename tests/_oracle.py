@@ -31,6 +31,7 @@ def lib():
         L.orc_get_solution.argtypes = [vp, vp]
         L.orc_set_solution.argtypes = [vp, vp]
         L.orc_rhs.argtypes = [vp, vp, ctypes.c_double, vp]
+        L.orc_rhs_parts.argtypes = [vp, vp, ctypes.c_double] + [vp] * 8
         L.orc_tridiag.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int]
         L.orc_weno_weights.argtypes = [ctypes.c_int] + [ctypes.c_double] * 9 + [vp]
         L.orc_fd4_line.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
@@ -76,6 +77,25 @@ class OCase:
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.shape)
         out = np.zeros(self.shape)
         self.L.orc_rhs(self.h, _p(u), float(t), _p(out))
+        return out
+
+    def rhs_parts(self, u, t):
+        """Boundary-filled ghosted state, fluxes, interface fluxes and the three
+        terms of one right-hand-side evaluation (orc_rhs_parts)."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.shape)
+        nd, nv = self.L.orc_ndims(self.h), self.shape[1]
+        g = self.L.orc_ghosts(self.h)
+        dims = [self.L.orc_dim(self.h, d) for d in range(nd)]
+        ng = int(np.prod([n + 2 * g for n in dims]))
+        other = lambda d: dims[1 - d] if nd == 2 else 1
+        out = {"u_ghosted": np.zeros((ng, nv)), "flux0": np.zeros((ng, nv)),
+               "flux1": np.zeros((ng, nv)) if nd == 2 else None,
+               "fi0": np.zeros((other(0), dims[0] + 1, nv)),
+               "fi1": np.zeros((other(1), dims[1] + 1, nv)) if nd == 2 else None,
+               "hyp": np.zeros(self.shape), "par": np.zeros(self.shape), "src": np.zeros(self.shape)}
+        ptr = lambda a: _p(a) if a is not None else None
+        self.L.orc_rhs_parts(self.h, _p(u), float(t), *[ptr(out[k]) for k in
+                             ("u_ghosted", "flux0", "flux1", "fi0", "fi1", "hyp", "par", "src")])
         return out
 
     def param(self, name):
