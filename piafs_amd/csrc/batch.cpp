@@ -78,12 +78,14 @@ int piafs_batch_destroy(piafs_batch *b) {
 }
 
 int piafs_batch_use_graph(piafs_batch *b, int on) {
+  if (!b) return 1;  // no batch
   b->eng.sync();
   b->eng.use_graph = (on != 0);
   return b->rebuild();
 }
 
 int piafs_batch_add(piafs_batch *b, const char *dir, int *slot) {
+  if (!b) return 1;  // no batch
   b->eng.sync();
   Case *c = new Case();
   int rc = c->init(dir ? dir : "", b->eng.device, b->eng.stream);
@@ -105,6 +107,7 @@ int piafs_batch_add(piafs_batch *b, const char *dir, int *slot) {
 }
 
 int piafs_batch_remove(piafs_batch *b, int k) {
+  if (!b) return 1;  // no batch
   Case *c = slot_case(b, k);
   if (!c) return 1;
   b->eng.sync();
@@ -115,6 +118,7 @@ int piafs_batch_remove(piafs_batch *b, int k) {
 }
 
 int piafs_batch_step(piafs_batch *b, int nsteps) {
+  if (!b) return 1;  // no batch
   int rc = b->eng.step(nsteps);
   if (rc) {
     b->err = b->eng.err;
@@ -126,12 +130,14 @@ int piafs_batch_step(piafs_batch *b, int nsteps) {
 }
 
 int piafs_batch_sync(piafs_batch *b) {
+  if (!b) return 1;  // no batch
   int rc = b->eng.sync();
   if (rc) b->err = b->eng.err;
   return rc;
 }
 
 int piafs_batch_download(piafs_batch *b, int k, double *u) {
+  if (!b) return 1;  // no batch
   Case *c = slot_case(b, k);
   if (!c) return 1;
   int rc = c->download(u, b->eng.stream);
@@ -140,6 +146,7 @@ int piafs_batch_download(piafs_batch *b, int k, double *u) {
 }
 
 int piafs_batch_info(piafs_batch *b, int k, piafs_info *info) {
+  if (!b) return 1;  // no batch
   Case *c = slot_case(b, k);
   if (!c || !info) return 1;
   fill_info(c, info);
@@ -147,6 +154,7 @@ int piafs_batch_info(piafs_batch *b, int k, piafs_info *info) {
 }
 
 int piafs_batch_autotune(piafs_batch *b, int *rows_per_lane, double *recon_ms) {
+  if (!b) return 1;  // no batch
   static const int cands[] = {4, 8, 12, 16};
   b->eng.sync();
   int rc = b->eng.autotune(cands, 4, 3, rows_per_lane, recon_ms);
@@ -155,6 +163,7 @@ int piafs_batch_autotune(piafs_batch *b, int *rows_per_lane, double *recon_ms) {
 }
 
 int piafs_batch_steps_done(piafs_batch *b, int k, int *steps) {
+  if (!b) return 1;  // no batch
   if (!slot_case(b, k) || !steps) return 1;
   *steps = b->steps[k];
   return 0;

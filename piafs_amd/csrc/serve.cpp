@@ -248,6 +248,7 @@ int piafs_server_create(int device, size_t hbm_budget, piafs_server **out) {
 }
 
 int piafs_server_submit(piafs_server *s, const char *dir, int *id) {
+  if (!s) return 1;  // no server
   auto r = std::make_unique<Request>();
   r->t_submit = now_s();
   r->c = std::make_unique<Case>();
@@ -285,6 +286,7 @@ int piafs_server_submit(piafs_server *s, const char *dir, int *id) {
 }
 
 int piafs_server_poll(piafs_server *s, int id, int *state, double *latency) {
+  if (!s) return 1;  // no server
   std::lock_guard<std::mutex> lk(s->mu);
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return 1;
@@ -296,6 +298,7 @@ int piafs_server_poll(piafs_server *s, int id, int *state, double *latency) {
 }
 
 int piafs_server_result(piafs_server *s, int id, double *u_host, piafs_info *info) {
+  if (!s) return 1;  // no server
   std::lock_guard<std::mutex> lk(s->mu);
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return 1;
@@ -318,6 +321,7 @@ int piafs_server_result(piafs_server *s, int id, double *u_host, piafs_info *inf
 }
 
 int piafs_server_release(piafs_server *s, int id) {
+  if (!s) return 1;  // no server
   std::lock_guard<std::mutex> lk(s->mu);
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return 1;
@@ -327,6 +331,7 @@ int piafs_server_release(piafs_server *s, int id) {
 }
 
 int piafs_server_wait_all(piafs_server *s) {
+  if (!s) return 1;  // no server
   std::unique_lock<std::mutex> lk(s->mu);
   s->cv.wait(lk, [&] {
     for (auto &kv : s->reqs)
@@ -337,6 +342,7 @@ int piafs_server_wait_all(piafs_server *s) {
 }
 
 int piafs_server_stats(piafs_server *s, size_t *bytes_in_use, int *running, int *queued) {
+  if (!s) return 1;  // no server
   std::lock_guard<std::mutex> lk(s->mu);
   if (bytes_in_use) *bytes_in_use = s->in_use;
   int run = 0, q = 0;

@@ -609,7 +609,12 @@ struct piafs_case {
   std::string err;
 };
 
-static std::string g_create_error;
+static thread_local std::string g_create_error;  // the calling thread's last failed create
+
+static int null_arg(piafs_case *pc, const char *what) {
+  pc->err = std::string(what) + " pointer is NULL";
+  return 1;
+}
 
 static int case_fail(piafs_case *pc, int rc) {
   pc->err = !pc->c->err.empty() ? pc->c->err : pc->eng.err;
@@ -683,22 +688,27 @@ void fill_info(const Case *c, piafs_info *info) {
 extern "C" {
 
 int piafs_get_info(const piafs_case *pc, piafs_info *info) {
-  if (!pc || !info) return 1;
+  if (!pc || !pc->c || !info) return 1;  // no case or no output
   fill_info(pc->c, info);
   return 0;
 }
 
 int piafs_upload_state(piafs_case *pc, const double *u) {
+  if (!pc || !pc->c) return 1;  // no case
+  if (!u) return null_arg(pc, "state");
   int rc = pc->c->upload(u, pc->eng.stream);
   return rc ? case_fail(pc, rc) : 0;
 }
 
 int piafs_download_state(piafs_case *pc, double *u) {
+  if (!pc || !pc->c) return 1;  // no case
+  if (!u) return null_arg(pc, "state");
   int rc = pc->c->download(u, pc->eng.stream);
   return rc ? case_fail(pc, rc) : 0;
 }
 
 int piafs_set_time(piafs_case *pc, double t) {
+  if (!pc || !pc->c) return 1;  // no case
   pc->c->t = t;
   if (hipMemcpyAsync(pc->c->h.time, &t, sizeof(double), hipMemcpyHostToDevice, pc->eng.stream) != hipSuccess ||
       hipStreamSynchronize(pc->eng.stream) != hipSuccess) {
@@ -709,16 +719,19 @@ int piafs_set_time(piafs_case *pc, double t) {
 }
 
 int piafs_step(piafs_case *pc, int n) {
+  if (!pc || !pc->c) return 1;  // no case
   int rc = pc->eng.step(n);
   return rc ? case_fail(pc, rc) : 0;
 }
 
 int piafs_sync(piafs_case *pc) {
+  if (!pc || !pc->c) return 1;  // no case
   int rc = pc->eng.sync();
   return rc ? case_fail(pc, rc) : 0;
 }
 
 int piafs_step_diag(piafs_case *pc, int n, double *cfl, double *norm) {
+  if (!pc || !pc->c) return 1;  // no case
   Case *c = pc->c;
   const Grid &G = c->h.G;
   hipStream_t st = pc->eng.stream;
@@ -751,6 +764,8 @@ int piafs_step_diag(piafs_case *pc, int n, double *cfl, double *norm) {
 }
 
 int piafs_rhs(piafs_case *pc, const double *u, double t, double *rhs) {
+  if (!pc || !pc->c) return 1;  // no case
+  if (!u || !rhs) return null_arg(pc, "state or output");
   Case *c = pc->c;
   hipStream_t st = pc->eng.stream;
   const size_t n = (size_t)c->cfg.nvars * c->h.G.plane;
@@ -817,6 +832,8 @@ static int download_planes(Case *c, const double *src, std::vector<double> &buf,
 extern "C" {
 
 int piafs_boundary_fill(piafs_case *pc, const double *u, double t, double *ug) {
+  if (!pc || !pc->c) return 1;  // no case
+  if (!ug) return null_arg(pc, "output");
   Case *c = pc->c;
   return eval_pieces(pc, u, t, K_GHOST, 7, [&]() {
     std::vector<double> buf;
@@ -833,6 +850,8 @@ int piafs_boundary_fill(piafs_case *pc, const double *u, double t, double *ug) {
 }
 
 int piafs_flux(piafs_case *pc, const double *u, double t, int dir, double *f) {
+  if (!pc || !pc->c) return 1;  // no case
+  if (!f) return null_arg(pc, "output");
   Case *c = pc->c;
   if (dir < 0 || dir >= c->cfg.ndims) {
     pc->err = "dir must be 0 .. ndims-1";
@@ -858,6 +877,8 @@ int piafs_flux(piafs_case *pc, const double *u, double t, int dir, double *f) {
 }
 
 int piafs_interface_flux(piafs_case *pc, const double *u, double t, int dir, double *fi) {
+  if (!pc || !pc->c) return 1;  // no case
+  if (!fi) return null_arg(pc, "output");
   Case *c = pc->c;
   if (dir < 0 || dir >= c->cfg.ndims) {
     pc->err = "dir must be 0 .. ndims-1";
@@ -878,6 +899,7 @@ int piafs_interface_flux(piafs_case *pc, const double *u, double t, int dir, dou
 }
 
 int piafs_rhs_terms(piafs_case *pc, const double *u, double t, double *hyp, double *par, double *src) {
+  if (!pc || !pc->c) return 1;  // no case
   Case *c = pc->c;
   double *out[3] = {hyp, par, src};
   const size_t n = (size_t)c->cfg.npts_int * c->cfg.nvars;
@@ -894,11 +916,14 @@ int piafs_rhs_terms(piafs_case *pc, const double *u, double t, double *hyp, doub
 }
 
 int piafs_write_binary(piafs_case *pc, const char *path) {
+  if (!pc || !pc->c) return 1;  // no case
+  if (!path) return null_arg(pc, "path");
   int rc = pc->c->write_binary(path, pc->eng.stream);
   return rc ? case_fail(pc, rc) : 0;
 }
 
 int piafs_kernel_times(piafs_case *pc, int enable, const char **names, double *ms, int *n) {
+  if (!pc || !pc->c) return 1;  // no case
   Engine &e = pc->eng;
   if (enable >= 0) {
     e.profiling = (enable != 0);
@@ -928,6 +953,7 @@ int piafs_kernel_times(piafs_case *pc, int enable, const char **names, double *m
 }
 
 int piafs_autotune(piafs_case *pc, int *rows_per_lane, double *recon_ms) {
+  if (!pc || !pc->c) return 1;  // no case
   static const int cands[] = {4, 8, 12, 16};
   int rc = pc->eng.autotune(cands, 4, 3, rows_per_lane, recon_ms);
   if (rc) pc->err = pc->eng.err;
@@ -946,6 +972,7 @@ int piafs_estimate_bytes(const char *dir, size_t *bytes) {
 }
 
 int piafs_solve(piafs_case *pc, const char *dir, int verbose, double *seconds) {
+  if (!pc || !pc->c) return 1;  // no case
   Case *c = pc->c;
   const Config &cf = c->cfg;
   const auto t_main = std::chrono::steady_clock::now();

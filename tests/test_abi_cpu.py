@@ -109,6 +109,28 @@ def test_shipped_example_decks_are_accepted(examples):
             assert piafs_amd.estimate_bytes(d) > 0, ex
 
 
+def test_null_handles_are_input_errors():
+    """Every entry point that takes a handle returns 1 (an input error) for a
+    NULL handle instead of dereferencing it; no GPU is touched."""
+    import ctypes
+    L = piafs_amd.lib()
+    d0 = ctypes.c_double(0.0)
+    calls = [
+        L.piafs_step(None, 1), L.piafs_sync(None), L.piafs_step_diag(None, 1, None, None),
+        L.piafs_upload_state(None, None), L.piafs_download_state(None, None), L.piafs_set_time(None, d0),
+        L.piafs_rhs(None, None, d0, None), L.piafs_rhs_terms(None, None, d0, None, None, None),
+        L.piafs_boundary_fill(None, None, d0, None), L.piafs_flux(None, None, d0, 0, None),
+        L.piafs_interface_flux(None, None, d0, 0, None), L.piafs_write_binary(None, None),
+        L.piafs_solve(None, None, 0, None), L.piafs_autotune(None, None, None),
+        L.piafs_batch_step(None, 1), L.piafs_batch_sync(None), L.piafs_batch_download(None, 0, None),
+        L.piafs_batch_add(None, None, None), L.piafs_batch_remove(None, 0),
+        L.piafs_server_submit(None, None, None), L.piafs_server_poll(None, 0, None, None),
+        L.piafs_server_wait_all(None), L.piafs_server_stats(None, None, None, None),
+    ]
+    assert calls == [1] * len(calls)
+    assert L.piafs_destroy(None) == 0
+
+
 def test_missing_initial_solution_is_an_error():
     dk = decks.sod_1d(N=32)
     with tempfile.TemporaryDirectory() as d:
