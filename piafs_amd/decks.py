@@ -687,6 +687,16 @@ def copy_example_deck(src: str, dst: str, files, solver_set: dict | None = None,
             f.write(text)
 
 
+def write_binary_op(path: str, ndims: int, dims, x_list, u) -> None:
+    """A solution file in WriteBinary's layout (src/IOFunctions/WriteBinary.c:37-93):
+    int32 ndims, int32 nvars, int32 dim[ndims], f64 x[sum dim], f64 u[prod dim * nvars]."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    with open(path, "wb") as f:
+        np.array([ndims, u.shape[1]] + list(dims), dtype=np.int32).tofile(f)
+        np.concatenate(x_list).astype(np.float64).tofile(f)
+        u.tofile(f)
+
+
 def read_binary_op(path: str):
     """Read a WriteBinary file (src/IOFunctions/WriteBinary.c:37-93):
     int32 ndims, int32 nvars, int32 dim[ndims], f64 x[sum dim], f64 u[prod dim * nvars]."""

@@ -222,6 +222,38 @@ def test_shipped_examples_through_the_driver(examples, example):
     assert np.all(_sum_err(U.sum(0), z[example + "_colsum"], z[example + "_colabs"]) <= TOL_STEPS)
 
 
+COMPARE = os.path.join(ROOT, "oracle", "_ref", "CompareSolutions")
+
+
+def _write_binary_op(path, dk, u):
+    dims = list(dk.solver["size"]) if dk.solver["ndims"] == 2 else [dk.solver["size"]]
+    decks.write_binary_op(path, dk.solver["ndims"], dims, dk.x, u)
+
+
+@pytest.mark.skipif(not os.path.exists(COMPARE), reason="the reference's CompareSolutions is not built")
+@pytest.mark.parametrize("name", ["ns2d_chem_crweno_48x40", "riemann2d_char_rf_41_40", "sod_char_roe_N128",
+                                  "ns2d_walls_inout_re200_24x20", "ns2d_crweno_char_flow_20x16"])
+def test_reference_compare_solutions_tool(ref_runs, name):
+    """The reference's own comparison tool (Extras/CompareSolutions.c, built
+    from source into oracle/_ref) on the driver's solution file against the
+    reference's run of the same deck: relative L1, L2 and Linf norms of the
+    difference, over all variables, within the parity bar."""
+    data, meta = ref_runs
+    dk = build_deck(meta[name])
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        with piafs_amd.Case(d) as c:
+            c.step(int(dk.solver["n_iter"]))
+            c.write_binary(os.path.join(d, "gpu.bin"))
+        _write_binary_op(os.path.join(d, "ref.bin"), dk, data[name])
+        with open(os.path.join(d, "compare.inp"), "w") as f:
+            f.write("gpu.bin ref.bin\n")
+        r = subprocess.run([COMPARE], cwd=d, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        l1, l2, linf = (float(v) for v in open(os.path.join(d, "compare.out")).read().split())
+    assert l1 <= TOL_STEPS and l2 <= TOL_STEPS and linf <= TOL_STEPS, (l1, l2, linf)
+
+
 def test_vortex_errors_match_errors_dat(examples):
     ref = examples["InviscidVortexConvection_errors_dat"]["errors"]
     dk = decks.isentropic_vortex(N=60, n_iter=400, dt=0.05)

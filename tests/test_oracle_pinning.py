@@ -187,6 +187,34 @@ def test_upw5_is_weno5_at_optimal_weights(interp):
     assert not np.array_equal(r_upw, r_weno)  # different arithmetic, same polynomial
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref",
+                                                   "CompareSolutions")),
+                    reason="the reference's CompareSolutions is not built")
+def test_reference_compare_tool_reports_a_planted_difference(ref_runs):
+    """Extras/CompareSolutions (built from source) reads solution files written
+    in WriteBinary's layout and reports zero for identical files, and for one
+    value changed by a relative 1e-9 an Linf of exactly |change| / max|u|."""
+    import json
+    import subprocess
+    data, meta = ref_runs
+    name = "ns2d_chem_crweno_48x40"
+    dk = build_deck(meta[name])
+    u = data[name]
+    exe = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "CompareSolutions")
+    up = u.copy()
+    up[100, 0] *= 1 + 1e-9
+    out = []
+    with tempfile.TemporaryDirectory() as d:
+        for fn, arr in (("a.bin", u), ("b.bin", up)):
+            decks.write_binary_op(os.path.join(d, fn), 2, list(dk.solver["size"]), dk.x, arr)
+        for pair in ("a.bin a.bin", "b.bin a.bin"):
+            open(os.path.join(d, "compare.inp"), "w").write(pair + "\n")
+            subprocess.run([exe], cwd=d, capture_output=True, check=True)
+            out.append([float(v) for v in open(os.path.join(d, "compare.out")).read().split()])
+    assert out[0] == [0.0, 0.0, 0.0]
+    assert out[1][2] == abs(up[100, 0] - u[100, 0]) / np.abs(u).max()
+
+
 def test_chemistry_constants_from_formulas():
     """Re and Pr of the 1024^2 chemistry deck follow the reference-scale formulas
     (ChemistryInitialize.c:289-314, NavierStokes2DInitialize.c:204-210)."""
