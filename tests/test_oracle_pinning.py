@@ -215,6 +215,39 @@ def test_reference_compare_tool_reports_a_planted_difference(ref_runs):
     assert out[1][2] == abs(up[100, 0] - u[100, 0]) / np.abs(u).max()
 
 
+@pytest.mark.parametrize("name", ["ns2d_chem_crweno_js_24x20", "ns2d_walls_inout_re200_24x20",
+                                  "ns2d_dirichlet_ambivalent_sponge_rusanov_24x20", "sod_char_roe_N128"])
+def test_oracle_rhs_parts_recombine(ref_runs, name):
+    """orc_rhs_parts, the oracle's pieces for the per-operator entry points:
+    the terms recombine bit for bit into orc_rhs (the reference's sum order,
+    TimeRHSFunctionExplicit.c:29-77), and the captured interface fluxes give
+    the hyperbolic term through hyp = sum_d dxinv_d (fI[k+1] - fI[k])
+    (HyperbolicFunction.c:34-127), which pins their [line][interface][var]
+    layout."""
+    _, meta = ref_runs
+    dk = build_deck(meta[name])
+    rng = np.random.default_rng(5)
+    u = dk.u * (1.0 + 1e-3 * rng.uniform(-1, 1, dk.u.shape))
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        c = orc.OCase(d)
+        P = c.rhs_parts(u, 0.45)
+        rhs = c.rhs(u, 0.45)
+    assert np.array_equal(((0.0 + -1.0 * P["hyp"]) + 1.0 * P["par"]) + 1.0 * P["src"], rhs)
+    nd = dk.solver["ndims"]
+    dims = list(dk.solver["size"]) if nd == 2 else [dk.solver["size"]]
+    nv = dk.u.shape[1]
+    # dxinv of the uniform grids these decks use, from the grid coordinates
+    inv = [1.0 / (dk.x[k][1] - dk.x[k][0]) for k in range(nd)]
+    hyp = np.zeros((dims[1] if nd == 2 else 1, dims[0], nv))
+    fi0 = P["fi0"]                                    # [y line][x interface][var]
+    hyp = hyp + inv[0] * (fi0[:, 1:, :] - fi0[:, :-1, :])
+    if nd == 2:
+        fi1 = P["fi1"]                                # [x line][y interface][var]
+        hyp = hyp + inv[1] * np.transpose(fi1[:, 1:, :] - fi1[:, :-1, :], (1, 0, 2))
+    assert np.allclose(hyp.reshape(-1, nv), P["hyp"], rtol=1e-12, atol=1e-12 * np.abs(P["hyp"]).max())
+
+
 def test_chemistry_constants_from_formulas():
     """Re and Pr of the 1024^2 chemistry deck follow the reference-scale formulas
     (ChemistryInitialize.c:289-314, NavierStokes2DInitialize.c:204-210)."""
