@@ -832,18 +832,24 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
       xin = xend;
     }
     if (!team_any(changed, T) || it > T) break;
-    if (bchg) {  // pivots of this segment derived again from the new entry
-      double c = cin;
-      bend = bin;
+    if (bchg) {  // pivots of this segment derived again from the new entry, until they repeat
+      const double bend_old = bend;
+      double c = cin, bb = bin;
+      bool live = true;
 #pragma unroll
       for (int i = 0; i < MR; i++) {
-        if (i < n) {
-          const double fac = A[i] / bend;
-          bend = B[i] - fac * c;
-          c = Cc[i];
-          Fk[i] = fac;
+        if (i < n && live) {
+          const double fac = A[i] / bb;
+          if (__double_as_longlong(fac) == __double_as_longlong(Fk[i])) {
+            live = false;  // same multiplier, so the same pivot here and in every later row
+          } else {
+            bb = B[i] - fac * c;
+            c = Cc[i];
+            Fk[i] = fac;
+          }
         }
       }
+      bend = live ? bb : bend_old;
     }
   }
   // b' and x' of every row, in the reference's arithmetic
@@ -875,13 +881,18 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
   double yin = is_last ? xlast : (plain_start ? 0.0 : Eb.a * xlast + Eb.b);
   for (int it = 0;; it++) {
     double xv = yin, dummy = 0.0;
+    bool live = true;
 #pragma unroll
     for (int i = MR - 1; i >= 0; i--) {
-      if (i < n) {
-        xv = (X[i] - Cc[i] * xv) / B[i];
-        Fk[i] = xv;
+      if (i < n && live) {
+        const double nx = (X[i] - Cc[i] * xv) / B[i];
+        // from the second pass on: a row that repeats its value leaves every row above unchanged
+        if (it > 0 && __double_as_longlong(nx) == __double_as_longlong(Fk[i])) live = false;
+        Fk[i] = nx;
+        xv = nx;
       }
     }
+    if (n > 0) xv = Fk[0];
     team_shift2(xv, dummy, lane, T, -1, red);
     const bool changed = has && !is_last && !(xv == yin);
     if (!is_last) yin = xv;
