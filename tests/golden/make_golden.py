@@ -215,6 +215,15 @@ def example_logs():
                 rows.append([int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(4))])
         meta[ex] = {"rows": rows, "source": f"Examples/1D_Euler/{ex}/output.log",
                     "ranks": 2 if ex == "LaxShockTube" else 1}
+        cons = os.path.join(src, "conservation.dat")
+        if os.path.exists(cons):  # N, ranks, dt, then the conservation error of each variable
+            meta[ex]["conservation_dat"] = [float(v) for v in open(cons).read().split()[3:]]
+            with tempfile.TemporaryDirectory() as d:  # the same deck through the serial reference
+                for fn in os.listdir(dst):
+                    shutil.copy(os.path.join(dst, fn), os.path.join(d, fn))
+                subprocess.run([REF_BIN], cwd=d, capture_output=True, check=True)
+                meta[ex]["conservation_dat_serial"] = [
+                    float(v) for v in open(os.path.join(d, "conservation.dat")).read().split()[3:]]
     # vortex errors.dat: n, n, ?, ?, dt, L1, L2, Linf, solver time, total time
     vals = open(os.path.join(REF_TREE, "Examples", "2D_NavierStokes", "InviscidVortexConvection",
                              "errors.dat")).read().split()
@@ -402,10 +411,20 @@ def examples_2d(names=None):
                  "stride": st, "solver_changes": {"set": spec.get("set", {}), "drop": list(spec.get("drop", ()))},
                  "source": f"Examples/{os.path.normpath(os.path.join('2D_NavierStokes', spec.get('src', ex)))} "
                            "(decks; initial state of its aux/init program), run by the reference built from source"}
+        if ex == "Riemann2DCase4":
+            # its shipped output.log (2 x 2 ranks) prints the same lines as the reference built here
+            rows = []
+            for line in open(os.path.join(src, "output.log")):
+                m = re.match(r"\s*Iteration:\s+(\d+)\s+Time:\s+(\S+)\s+Max CFL:\s+(\S+).*Norm:\s+(\S+)", line)
+                if m:
+                    rows.append([int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(4))])
+            entry["log_rows"] = rows
+            entry["log_source"] = f"Examples/2D_NavierStokes/{ex}/output.log"
         # LidDrivenCavity/output.log is not kept: it was written by an older code
         # (its header names HyPar, 2 x 2 ranks, PETSc), and the reference built
         # from source does not reproduce it (CFL 0.694 against 0.6896 and a step
-        # norm half as large at iteration 100)
+        # norm half as large at iteration 100); neither are the shipped logs of
+        # RadialExpansionWave and FlatPlateLaminar (norms differ in the third digit)
         meta[ex] = entry
         print(ex, len(entry["rows"]), "screen rows,", len(made), "files, errors", errs)
     np.savez_compressed(os.path.join(HERE, "examples2d_ref.npz"), **arrays)

@@ -209,6 +209,9 @@ def test_shipped_examples_through_the_driver(examples, example):
     for it, _, cf, nm in ref["rows"]:
         assert _close_in_last_digit(rows[it][0], cf, "%1.3E"), (it, rows[it][0], cf)
         assert _close_in_last_digit(rows[it][1], nm, "%1.4E"), (it, rows[it][1], nm)
+    for it, _, cf, nm in ref.get("log_rows", []):  # the example's own shipped output.log, where it has one
+        assert _close_in_last_digit(rows[it][0], cf, "%1.3E"), (it, rows[it][0], cf)
+        assert _close_in_last_digit(rows[it][1], nm, "%1.4E"), (it, rows[it][1], nm)
     if cons is not None:
         np.testing.assert_allclose(cons, ref["conservation"], rtol=0, atol=1e-11)
     if "errors" in ref:  # L1, L2, Linf against the example's exact.inp (zeros without one)
@@ -220,6 +223,26 @@ def test_shipped_examples_through_the_driver(examples, example):
     assert np.all(_rel(sub, z[example + "_sub"].reshape(-1, nv)) <= TOL_STEPS)
     assert np.all(_sum_err(U.sum(1), z[example + "_rowsum"], z[example + "_rowabs"]) <= TOL_STEPS)
     assert np.all(_sum_err(U.sum(0), z[example + "_colsum"], z[example + "_colabs"]) <= TOL_STEPS)
+
+
+@pytest.mark.parametrize("example", ["ShuOsherProblem", "LaxShockTube"])
+def test_shipped_1d_conservation_dat(examples, example):
+    """Examples/1D_Euler/{ShuOsherProblem,LaxShockTube}/conservation.dat: the
+    conservation error of each variable after the full run, through the
+    command-line driver (conservation_check yes in the shipped decks), bit
+    for bit equal to the serial reference's run of the same deck.  The
+    ShuOsherProblem file was written on one rank and equals that run; the
+    LaxShockTube file was written on two ranks, whose sums run in another
+    order, and agrees with it at roundoff level (2e-15)."""
+    d0 = os.path.join(GOLDEN, "examples", example)
+    with tempfile.TemporaryDirectory() as d:
+        decks.copy_example_deck(d0, d, sorted(os.listdir(d0)))
+        r = subprocess.run([os.path.join(ROOT, "piafs_amd", "bin", "piafs_amd"), d], capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        got = [float(v) for v in open(os.path.join(d, "conservation.dat")).read().split()[3:]]
+    assert got == examples[example]["conservation_dat_serial"], got
+    np.testing.assert_allclose(got, examples[example]["conservation_dat"], rtol=0, atol=2e-15)
 
 
 COMPARE = os.path.join(ROOT, "oracle", "_ref", "CompareSolutions")
