@@ -1428,8 +1428,16 @@ __device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, 
   if (chain > 0) {  // a species: a scalar chain of the same arithmetic
     const int m = NF + chain - 1, f0 = 2 * NF * NF + NF + 3 * (chain - 1);
     double bp = 0.0, xp = 0.0, cprev = 0.0;
+    // each row's coefficients are loaded one row ahead, so that their latency overlaps the chain
+    double na = coef(m, 0, 0), nb = coef(m, 1, 0), nc = coef(m, 2, 0), nF = coef(m, 3, 0);
     for (int k = 0; k <= N; k++) {
-      const double a = coef(m, 0, k), b = coef(m, 1, k), c = coef(m, 2, k), F = coef(m, 3, k);
+      const double a = na, b = nb, c = nc, F = nF;
+      if (k < N) {
+        na = coef(m, 0, k + 1);
+        nb = coef(m, 1, k + 1);
+        nc = coef(m, 2, k + 1);
+        nF = coef(m, 3, k + 1);
+      }
       if (k == 0) {
         bp = b;
         xp = F;
@@ -1445,24 +1453,48 @@ __device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, 
     }
     double x = (1.0 / bp) * xp;
     out[((long)m * nl + l) * (N + 1) + N] = x;
+    double nrb = N > 0 ? bk(f0, N - 1) : 0.0, ncp = N > 0 ? bk(f0 + 1, N - 1) : 0.0,
+           nxq = N > 0 ? bk(f0 + 2, N - 1) : 0.0;
     for (int k = N - 1; k >= 0; k--) {
-      const double rb = bk(f0, k), c = bk(f0 + 1, k), xq = bk(f0 + 2, k);
+      const double rb = nrb, c = ncp, xq = nxq;
+      if (k > 0) {
+        nrb = bk(f0, k - 1);
+        ncp = bk(f0 + 1, k - 1);
+        nxq = bk(f0 + 2, k - 1);
+      }
       x = rb * (xq - c * x);
       out[((long)m * nl + l) * (N + 1) + k] = x;
     }
     return;
   }
   double bp[NF][NF], xp[NF], cprev[NF][NF];
-  for (int k = 0; k <= N; k++) {
-    double L[NF][NF], A[NF][NF], Bm[NF][NF], Cm[NF][NF], F[NF];
+  // each row's block data is loaded one row ahead, so that its latency overlaps the chain
+  double nL[NF][NF], ncf[NF][4];
+  auto load_row = [&](int k) {
 #pragma unroll
     for (int i = 0; i < NF; i++)
 #pragma unroll
-      for (int j = 0; j < NF; j++) L[i][j] = rf(i * NF + j, k);
+      for (int j = 0; j < NF; j++) nL[i][j] = rf(i * NF + j, k);
+#pragma unroll
+    for (int v = 0; v < NF; v++)
+#pragma unroll
+      for (int w = 0; w < 4; w++) ncf[v][w] = coef(v, w, k);
+  };
+  load_row(0);
+  for (int k = 0; k <= N; k++) {
+    double L[NF][NF], cf[NF][4], A[NF][NF], Bm[NF][NF], Cm[NF][NF], F[NF];
+#pragma unroll
+    for (int i = 0; i < NF; i++) {
+#pragma unroll
+      for (int j = 0; j < NF; j++) L[i][j] = nL[i][j];
+#pragma unroll
+      for (int w = 0; w < 4; w++) cf[i][w] = ncf[i][w];
+    }
+    if (k < N) load_row(k + 1);
 #pragma unroll
     for (int v = 0; v < NF; v++) {
-      const double a = coef(v, 0, k), b = coef(v, 1, k), c = coef(v, 2, k);
-      F[v] = coef(v, 3, k);
+      const double a = cf[v][0], b = cf[v][1], c = cf[v][2];
+      F[v] = cf[v][3];
 #pragma unroll
       for (int j = 0; j < NF; j++) {
         A[v][j] = a * L[v][j];
@@ -1534,20 +1566,44 @@ __device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, 
   }
 #pragma unroll
   for (int i = 0; i < NF; i++) out[((long)i * nl + l) * (N + 1) + N] = x[i];
-  for (int k = N - 1; k >= 0; k--) {
-    double y[NF];
+  // the back-substitution data of row k-1 is loaded while row k is computed
+  double nbi[NF][NF], ncp[NF][NF], nxq[NF];
+  auto load_back = [&](int k) {
 #pragma unroll
     for (int i = 0; i < NF; i++) {
-      double r = bk(2 * NF * NF + i, k);
 #pragma unroll
-      for (int j = 0; j < NF; j++) r -= (bk(NF * NF + i * NF + j, k) * x[j]);
+      for (int j = 0; j < NF; j++) {
+        nbi[i][j] = bk(i * NF + j, k);
+        ncp[i][j] = bk(NF * NF + i * NF + j, k);
+      }
+      nxq[i] = bk(2 * NF * NF + i, k);
+    }
+  };
+  if (N > 0) load_back(N - 1);
+  for (int k = N - 1; k >= 0; k--) {
+    double bi[NF][NF], cp[NF][NF], xq[NF], y[NF];
+#pragma unroll
+    for (int i = 0; i < NF; i++) {
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        bi[i][j] = nbi[i][j];
+        cp[i][j] = ncp[i][j];
+      }
+      xq[i] = nxq[i];
+    }
+    if (k > 0) load_back(k - 1);
+#pragma unroll
+    for (int i = 0; i < NF; i++) {
+      double r = xq[i];
+#pragma unroll
+      for (int j = 0; j < NF; j++) r -= (cp[i][j] * x[j]);
       y[i] = r;
     }
 #pragma unroll
     for (int i = 0; i < NF; i++) {
       double acc = 0;
 #pragma unroll
-      for (int j = 0; j < NF; j++) acc += (bk(i * NF + j, k) * y[j]);
+      for (int j = 0; j < NF; j++) acc += (bi[i][j] * y[j]);
       x[i] = acc;
     }
 #pragma unroll
