@@ -1376,47 +1376,11 @@ __global__ void __launch_bounds__(256) k_cw_rows(CASE_ARGS) {
 #undef CWROWS
 }
 
-// _MatrixInvert_ (matops.h:96-130) on the flow block: Gauss-Jordan without
-// pivoting, in the reference's loop order
-template <int NF>
-__device__ __forceinline__ void cw_inv(const double A[NF][NF], double B[NF][NF]) {
-  double Ac[NF][NF];
-#pragma unroll
-  for (int i = 0; i < NF; i++)
-#pragma unroll
-    for (int j = 0; j < NF; j++) {
-      Ac[i][j] = A[i][j];
-      B[i][j] = (i == j) ? 1.0 : 0.0;
-    }
-#pragma unroll
-  for (int i = 0; i < NF - 1; i++)
-#pragma unroll
-    for (int j = i + 1; j < NF; j++) {
-      const double f = Ac[j][i] / Ac[i][i];
-#pragma unroll
-      for (int k = i + 1; k < NF; k++) Ac[j][k] -= (f * Ac[i][k]);
-#pragma unroll
-      for (int k = 0; k < j; k++) B[j][k] -= (f * B[i][k]);
-    }
-#pragma unroll
-  for (int i = NF - 1; i >= 0; i--)
-#pragma unroll
-    for (int k = 0; k < NF; k++) {
-      double sum = 0.0;
-#pragma unroll
-      for (int j = i + 1; j < NF; j++) sum += (Ac[i][j] * B[j][k]);
-      B[i][k] = (B[i][k] - sum) / Ac[i][i];
-    }
-}
-
-// One system of one line: chain 0 is the flow block, chain 1 + m' species
-// NF + m'.  Forward elimination row by row (the back-substitution data go to
-// cwb), then back substitution from the last row, writing the interface values.
-template <int NF>
-__device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, int dir, int l, int sys,
-                                          int chain) {
+// A species chain (chain >= 1): the scalar chain of the same arithmetic as the
+// flow block's, for species NF + chain - 1.
+__device__ __forceinline__ void cwc_species(const CaseDev &C, const SweepGeom &S, int dir, int l, int sys,
+                                           int chain, int NF) {
   const int nv = C.G.nv, nl = S.nlines, N = S.N;
-  // rows [field][k][line], back-substitution data [field][system][k][line]
   const long vs = (long)nl * (N + 1);
   const long bs = 4L * nl * (N + 1), bb = (long)sys * (N + 1) * nl + l;
   const double *rows = C.cwr[dir];
@@ -1425,209 +1389,199 @@ __device__ __forceinline__ void cwc_solve(const CaseDev &C, const SweepGeom &S, 
   auto rf = [&](int f, int k) { return rows[(long)f * vs + (long)k * nl + l]; };
   auto bk = [&](int f, int k) -> double & { return back[(long)f * bs + bb + (long)k * nl]; };
   auto coef = [&](int v, int which, int k) { return rf(NF * NF + (sys * nv + v) * 4 + which, k); };
-  if (chain > 0) {  // a species: a scalar chain of the same arithmetic
-    const int m = NF + chain - 1, f0 = 2 * NF * NF + NF + 3 * (chain - 1);
-    double bp = 0.0, xp = 0.0, cprev = 0.0;
-    // each row's coefficients are loaded one row ahead, so that their latency overlaps the chain
-    double na = coef(m, 0, 0), nb = coef(m, 1, 0), nc = coef(m, 2, 0), nF = coef(m, 3, 0);
-    for (int k = 0; k <= N; k++) {
-      const double a = na, b = nb, c = nc, F = nF;
-      if (k < N) {
-        na = coef(m, 0, k + 1);
-        nb = coef(m, 1, k + 1);
-        nc = coef(m, 2, k + 1);
-        nF = coef(m, 3, k + 1);
-      }
-      if (k == 0) {
-        bp = b;
-        xp = F;
-      } else {
-        const double rb = 1.0 / bp, fac = a * rb;
-        bk(f0, k - 1) = rb;
-        bk(f0 + 1, k - 1) = cprev;
-        bk(f0 + 2, k - 1) = xp;
-        bp = b - fac * cprev;
-        xp = F - fac * xp;
-      }
-      cprev = c;
-    }
-    double x = (1.0 / bp) * xp;
-    out[((long)m * nl + l) * (N + 1) + N] = x;
-    double nrb = N > 0 ? bk(f0, N - 1) : 0.0, ncp = N > 0 ? bk(f0 + 1, N - 1) : 0.0,
-           nxq = N > 0 ? bk(f0 + 2, N - 1) : 0.0;
-    for (int k = N - 1; k >= 0; k--) {
-      const double rb = nrb, c = ncp, xq = nxq;
-      if (k > 0) {
-        nrb = bk(f0, k - 1);
-        ncp = bk(f0 + 1, k - 1);
-        nxq = bk(f0 + 2, k - 1);
-      }
-      x = rb * (xq - c * x);
-      out[((long)m * nl + l) * (N + 1) + k] = x;
-    }
-    return;
-  }
-  double bp[NF][NF], xp[NF], cprev[NF][NF];
-  // each row's block data is loaded one row ahead, so that its latency overlaps the chain
-  double nL[NF][NF], ncf[NF][4];
-  auto load_row = [&](int k) {
-#pragma unroll
-    for (int i = 0; i < NF; i++)
-#pragma unroll
-      for (int j = 0; j < NF; j++) nL[i][j] = rf(i * NF + j, k);
-#pragma unroll
-    for (int v = 0; v < NF; v++)
-#pragma unroll
-      for (int w = 0; w < 4; w++) ncf[v][w] = coef(v, w, k);
-  };
-  load_row(0);
+  const int m = NF + chain - 1, f0 = 2 * NF * NF + NF + 3 * (chain - 1);
+  double bp = 0.0, xp = 0.0, cprev = 0.0;
+  // each row's coefficients are loaded one row ahead, so that their latency overlaps the chain
+  double na = coef(m, 0, 0), nb = coef(m, 1, 0), nc = coef(m, 2, 0), nF = coef(m, 3, 0);
   for (int k = 0; k <= N; k++) {
-    double L[NF][NF], cf[NF][4], A[NF][NF], Bm[NF][NF], Cm[NF][NF], F[NF];
-#pragma unroll
-    for (int i = 0; i < NF; i++) {
-#pragma unroll
-      for (int j = 0; j < NF; j++) L[i][j] = nL[i][j];
-#pragma unroll
-      for (int w = 0; w < 4; w++) cf[i][w] = ncf[i][w];
-    }
-    if (k < N) load_row(k + 1);
-#pragma unroll
-    for (int v = 0; v < NF; v++) {
-      const double a = cf[v][0], b = cf[v][1], c = cf[v][2];
-      F[v] = cf[v][3];
-#pragma unroll
-      for (int j = 0; j < NF; j++) {
-        A[v][j] = a * L[v][j];
-        Bm[v][j] = b * L[v][j];
-        Cm[v][j] = c * L[v][j];
-      }
+    const double a = na, b = nb, c = nc, F = nF;
+    if (k < N) {
+      na = coef(m, 0, k + 1);
+      nb = coef(m, 1, k + 1);
+      nc = coef(m, 2, k + 1);
+      nF = coef(m, 3, k + 1);
     }
     if (k == 0) {
-#pragma unroll
-      for (int i = 0; i < NF; i++) {
-        xp[i] = F[i];
-#pragma unroll
-        for (int j = 0; j < NF; j++) bp[i][j] = Bm[i][j];
-      }
+      bp = b;
+      xp = F;
     } else {
-      double binv[NF][NF], fac[NF][NF];
-      cw_inv<NF>(bp, binv);
-#pragma unroll
-      for (int i = 0; i < NF; i++)
-#pragma unroll
-        for (int j = 0; j < NF; j++) {
-          double acc = 0;
-#pragma unroll
-          for (int q = 0; q < NF; q++) acc += (A[i][q] * binv[q][j]);
-          fac[i][j] = acc;
-        }
-      // row k-1 for the back substitution
-#pragma unroll
-      for (int i = 0; i < NF; i++) {
-#pragma unroll
-        for (int j = 0; j < NF; j++) {
-          bk(i * NF + j, k - 1) = binv[i][j];
-          bk(NF * NF + i * NF + j, k - 1) = cprev[i][j];
-        }
-        bk(2 * NF * NF + i, k - 1) = xp[i];
-      }
-      double nx[NF];
-#pragma unroll
-      for (int i = 0; i < NF; i++) {
-#pragma unroll
-        for (int j = 0; j < NF; j++) {
-          double b = Bm[i][j];
-#pragma unroll
-          for (int q = 0; q < NF; q++) b -= (fac[i][q] * cprev[q][j]);
-          bp[i][j] = b;
-        }
-        double x = F[i];
-#pragma unroll
-        for (int q = 0; q < NF; q++) x -= (fac[i][q] * xp[q]);
-        nx[i] = x;
-      }
-#pragma unroll
-      for (int i = 0; i < NF; i++) xp[i] = nx[i];
+      const double rb = 1.0 / bp, fac = a * rb;
+      bk(f0, k - 1) = rb;
+      bk(f0 + 1, k - 1) = cprev;
+      bk(f0 + 2, k - 1) = xp;
+      bp = b - fac * cprev;
+      xp = F - fac * xp;
     }
-#pragma unroll
-    for (int i = 0; i < NF; i++)
-#pragma unroll
-      for (int j = 0; j < NF; j++) cprev[i][j] = Cm[i][j];
+    cprev = c;
   }
-  // last row, then back substitution x_k = binv_k (x'_k - C_k x_{k+1})
-  double binv[NF][NF], x[NF];
-  cw_inv<NF>(bp, binv);
-#pragma unroll
-  for (int i = 0; i < NF; i++) {
-    double acc = 0;
-#pragma unroll
-    for (int j = 0; j < NF; j++) acc += (binv[i][j] * xp[j]);
-    x[i] = acc;
-  }
-#pragma unroll
-  for (int i = 0; i < NF; i++) out[((long)i * nl + l) * (N + 1) + N] = x[i];
-  // the back-substitution data of row k-1 is loaded while row k is computed
-  double nbi[NF][NF], ncp[NF][NF], nxq[NF];
-  auto load_back = [&](int k) {
-#pragma unroll
-    for (int i = 0; i < NF; i++) {
-#pragma unroll
-      for (int j = 0; j < NF; j++) {
-        nbi[i][j] = bk(i * NF + j, k);
-        ncp[i][j] = bk(NF * NF + i * NF + j, k);
-      }
-      nxq[i] = bk(2 * NF * NF + i, k);
-    }
-  };
-  if (N > 0) load_back(N - 1);
+  double x = (1.0 / bp) * xp;
+  out[((long)m * nl + l) * (N + 1) + N] = x;
+  double nrb = N > 0 ? bk(f0, N - 1) : 0.0, ncp = N > 0 ? bk(f0 + 1, N - 1) : 0.0,
+         nxq = N > 0 ? bk(f0 + 2, N - 1) : 0.0;
   for (int k = N - 1; k >= 0; k--) {
-    double bi[NF][NF], cp[NF][NF], xq[NF], y[NF];
-#pragma unroll
-    for (int i = 0; i < NF; i++) {
-#pragma unroll
-      for (int j = 0; j < NF; j++) {
-        bi[i][j] = nbi[i][j];
-        cp[i][j] = ncp[i][j];
-      }
-      xq[i] = nxq[i];
+    const double rb = nrb, c = ncp, xq = nxq;
+    if (k > 0) {
+      nrb = bk(f0, k - 1);
+      ncp = bk(f0 + 1, k - 1);
+      nxq = bk(f0 + 2, k - 1);
     }
-    if (k > 0) load_back(k - 1);
-#pragma unroll
-    for (int i = 0; i < NF; i++) {
-      double r = xq[i];
-#pragma unroll
-      for (int j = 0; j < NF; j++) r -= (cp[i][j] * x[j]);
-      y[i] = r;
-    }
-#pragma unroll
-    for (int i = 0; i < NF; i++) {
-      double acc = 0;
-#pragma unroll
-      for (int j = 0; j < NF; j++) acc += (bi[i][j] * y[j]);
-      x[i] = acc;
-    }
-#pragma unroll
-    for (int i = 0; i < NF; i++) out[((long)i * nl + l) * (N + 1) + k] = x[i];
+    x = rb * (xq - c * x);
+    out[((long)m * nl + l) * (N + 1) + k] = x;
   }
 }
 
-// work items: (sweep, chain, system, line), the line fastest and the flow
-// chains first so that a wave does one kind of chain
-__host__ __device__ inline long cw_chains(const SweepGeom &S, int nv, int nf) { return 4L * S.nlines * (1 + nv - nf); }
+// The flow block chain of one system of one line (blocktridiagLU.c:159-301
+// with the blocks inverted by _MatrixInvert_, matops.h:96-130: Gauss-Jordan
+// without pivoting) on a group of 16 lanes: lane e < NF*NF holds entry
+// (e / NF, e % NF) of every NF x NF block, and each row of the vectors (x',
+// the solution) is held by the NF lanes of that row; lanes NF*NF .. 15 (1D)
+// only take part in the exchanges.  Every entry goes through the reference's
+// operations in its order, which the lanes can share: a step of the inverse
+// updates only rows below (forward) or the row above (backward) the rows it
+// reads.  All 16 lanes execute every exchange (none sits under a
+// lane-dependent branch).
+__device__ __forceinline__ double grp16(double v, int src) { return __shfl(v, src, 16); }
+
+template <int NF>
+__device__ __forceinline__ double cw_inv16(double a, int ri, int cj) {
+  double ac = a, b = (ri == cj) ? 1.0 : 0.0;
+#pragma unroll
+  for (int i = 0; i < NF - 1; i++) {
+    const double aii = grp16(ac, (NF + 1) * i), aji = grp16(ac, NF * ri + i);
+    const double aik = grp16(ac, NF * i + cj), bik = grp16(b, NF * i + cj);
+    if (ri > i) {
+      const double f = aji / aii;
+      if (cj > i) ac -= (f * aik);
+      if (cj < ri) b -= (f * bik);
+    }
+  }
+#pragma unroll
+  for (int i = NF - 1; i >= 0; i--) {
+    double sum = 0.0;
+#pragma unroll
+    for (int j = i + 1; j < NF; j++) sum += (grp16(ac, NF * i + j) * grp16(b, NF * j + cj));
+    const double aii = grp16(ac, (NF + 1) * i);
+    if (ri == i) b = (b - sum) / aii;
+  }
+  return b;
+}
+
+template <int NF>
+__device__ __forceinline__ void cwc_flow16(const CaseDev &C, const SweepGeom &S, int dir, int l, int sys, int e) {
+  const int nv = C.G.nv, nl = S.nlines, N = S.N;
+  const bool act = e < NF * NF;
+  const int ri = act ? e / NF : 0, cj = act ? e % NF : 0;
+  const long vs = (long)nl * (N + 1);
+  const long bs = 4L * nl * (N + 1), bb = (long)sys * (N + 1) * nl + l;
+  const double *rows = C.cwr[dir];
+  double *back = C.cwb[dir];
+  double *out = (dir ? C.iy : C.ix)[sys];
+  auto rf = [&](int f, int k) { return rows[(long)f * vs + (long)k * nl + l]; };
+  auto bk = [&](int f, int k) -> double & { return back[(long)f * bs + bb + (long)k * nl]; };
+  auto coef = [&](int which, int k) { return rf(NF * NF + (sys * nv + ri) * 4 + which, k); };
+  // this lane's entry of L and its row's coefficients, loaded one row ahead
+  double nL = rf(NF * ri + cj, 0), na = coef(0, 0), nb = coef(1, 0), nc = coef(2, 0), nF = coef(3, 0);
+  double bp = 0.0, xp = 0.0, cprev = 0.0;
+  for (int k = 0; k <= N; k++) {
+    const double Lv = nL, a = na, b = nb, c = nc, F = nF;
+    if (k < N) {
+      nL = rf(NF * ri + cj, k + 1);
+      na = coef(0, k + 1);
+      nb = coef(1, k + 1);
+      nc = coef(2, k + 1);
+      nF = coef(3, k + 1);
+    }
+    const double A = a * Lv, Bm = b * Lv, Cm = c * Lv;
+    if (k == 0) {
+      xp = F;
+      bp = Bm;
+    } else {
+      const double binv = cw_inv16<NF>(bp, ri, cj);
+      double fac = 0;
+#pragma unroll
+      for (int q = 0; q < NF; q++) fac += (grp16(A, NF * ri + q) * grp16(binv, NF * q + cj));
+      // row k-1 for the back substitution
+      if (act) {
+        bk(NF * ri + cj, k - 1) = binv;
+        bk(NF * NF + NF * ri + cj, k - 1) = cprev;
+        if (cj == 0) bk(2 * NF * NF + ri, k - 1) = xp;
+      }
+      double bn = Bm, xn = F;
+#pragma unroll
+      for (int q = 0; q < NF; q++) {
+        const double fq = grp16(fac, NF * ri + q);
+        bn -= (fq * grp16(cprev, NF * q + cj));
+        xn -= (fq * grp16(xp, NF * q));
+      }
+      bp = bn;
+      xp = xn;
+    }
+    cprev = Cm;
+  }
+  // last row, then back substitution x_k = binv_k (x'_k - C_k x_{k+1})
+  const double binv = cw_inv16<NF>(bp, ri, cj);
+  double x = 0;
+#pragma unroll
+  for (int j = 0; j < NF; j++) x += (grp16(binv, NF * ri + j) * grp16(xp, NF * j));
+  if (act && cj == 0) out[((long)ri * nl + l) * (N + 1) + N] = x;
+  double nbi = 0.0, ncp = 0.0, nxq = 0.0;
+  if (N > 0) {
+    nbi = bk(NF * ri + cj, N - 1);
+    ncp = bk(NF * NF + NF * ri + cj, N - 1);
+    nxq = bk(2 * NF * NF + ri, N - 1);
+  }
+  for (int k = N - 1; k >= 0; k--) {
+    const double bi = nbi, cp = ncp, xq = nxq;
+    if (k > 0) {
+      nbi = bk(NF * ri + cj, k - 1);
+      ncp = bk(NF * NF + NF * ri + cj, k - 1);
+      nxq = bk(2 * NF * NF + ri, k - 1);
+    }
+    double y = xq;
+#pragma unroll
+    for (int j = 0; j < NF; j++) y -= (grp16(cp, NF * ri + j) * grp16(x, NF * j));
+    double xn = 0;
+#pragma unroll
+    for (int j = 0; j < NF; j++) xn += (grp16(bi, NF * ri + j) * grp16(y, NF * j));
+    x = xn;
+    if (act && cj == 0) out[((long)ri * nl + l) * (N + 1) + k] = x;
+  }
+}
+
+// Work items of a sweep: the flow chains first (16 lanes per system; (system,
+// line) with the line fastest), then one item per species
+// chain ((chain, system, line), the line fastest), so that a wave does one
+// kind of chain.  The y sweep starts on a 64-item boundary, which keeps every
+// 16-lane group inside one wave's lane group.
+__host__ __device__ inline int cw_flow_lanes(int) { return 16; }
+__host__ __device__ inline long cw_items(const SweepGeom &S, int nv, int nf) {
+  return 4L * S.nlines * (cw_flow_lanes(nf) + nv - nf);
+}
+__host__ __device__ inline long cw_x_span(const CaseDev &c, int nf) {
+  const long nX = cw_items(c.Sx, c.G.nv, nf);
+  return c.G.ndims == 2 ? (nX + 63) & ~63L : nX;
+}
 
 __global__ void __launch_bounds__(256) k_cw_solve(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
   const int nv = C.G.nv, nf = C.model == 1 ? 3 : 4;
-  const long nX = cw_chains(C.Sx, nv, nf), nY = (C.G.ndims == 2) ? cw_chains(C.Sy, nv, nf) : 0;
-  if (t >= nX + nY) return;
-  const int dir = t < nX ? 0 : 1;
+  const long nX = cw_items(C.Sx, nv, nf), nXs = cw_x_span(C, nf);
+  const long nY = (C.G.ndims == 2) ? cw_items(C.Sy, nv, nf) : 0;
+  if (t >= nXs + nY || (t >= nX && t < nXs)) return;
+  const int dir = t < nXs ? 0 : 1;
   const SweepGeom &S = dir ? C.Sy : C.Sx;
-  const long u = dir ? t - nX : t;
-  const long per = 4L * S.nlines;
-  const int chain = (int)(u / per), sys = (int)((u % per) / S.nlines), l = (int)(u % S.nlines);
-  if (nf == 3) cwc_solve<3>(C, S, dir, l, sys, chain);
-  else cwc_solve<4>(C, S, dir, l, sys, chain);
+  const long u = dir ? t - nXs : t;
+  const long nl = S.nlines, nflow = 4L * nl * cw_flow_lanes(nf);
+  if (u < nflow) {
+    const long grp = u >> 4;
+    if (nf == 4) cwc_flow16<4>(C, S, dir, (int)(grp % nl), (int)(grp / nl), (int)(u & 15));
+    else cwc_flow16<3>(C, S, dir, (int)(grp % nl), (int)(grp / nl), (int)(u & 15));
+    return;
+  }
+  const long v = u - nflow, per = 4L * nl;
+  const int chain = 1 + (int)(v / per), sys = (int)((v % per) / nl), l = (int)(v % nl);
+  cwc_species(C, S, dir, l, sys, chain, nf);
 }
 
 // Blocks of a sweep for the team-based reconstruction (each block holds
@@ -2423,7 +2377,7 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
     const int nf = c.model == 1 ? 3 : 4;
     blocks[K_ENDS] = blocks[K_RECON] = blocks[K_RCHAR] = 0;
     blocks[K_CWROWS] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
-    blocks[K_CWSOLVE] = nb(cw_chains(c.Sx, G.nv, nf) + (two ? cw_chains(c.Sy, G.nv, nf) : 0));
+    blocks[K_CWSOLVE] = nb(cw_x_span(c, nf) + (two ? cw_items(c.Sy, G.nv, nf) : 0));
   } else if (c.R.characteristic) {
     blocks[K_ENDS] = 0;
     blocks[K_RECON] = 0;
