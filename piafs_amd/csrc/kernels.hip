@@ -1432,45 +1432,78 @@ __device__ __forceinline__ void cwc_species(const CaseDev &C, const SweepGeom &S
 
 // The flow block chain of one system of one line (blocktridiagLU.c:159-301
 // with the blocks inverted by _MatrixInvert_, matops.h:96-130: Gauss-Jordan
-// without pivoting) on a group of 16 lanes: lane e < NF*NF holds entry
-// (e / NF, e % NF) of every NF x NF block, and each row of the vectors (x',
-// the solution) is held by the NF lanes of that row; lanes NF*NF .. 15 (1D)
-// only take part in the exchanges.  Every entry goes through the reference's
-// operations in its order, which the lanes can share: a step of the inverse
-// updates only rows below (forward) or the row above (backward) the rows it
-// reads.  All 16 lanes execute every exchange (none sits under a
-// lane-dependent branch).
-__device__ __forceinline__ double grp16(double v, int src) { return __shfl(v, src, 16); }
+// without pivoting) on the four lanes of a quad: lane r < NF holds row r of
+// every NF x NF block and row r of the vectors (x', the solution); lane 3 in
+// 1D only takes part in the exchanges.  Rows move between the lanes by DPP
+// quad permutations (no LDS), and each lane keeps whole copies of the blocks
+// it multiplies by (the inverse, C_{k-1}).  Every entry goes through the
+// reference's operations in its order: a step of the inverse updates only
+// rows that the rows it reads do not depend on.  All four lanes execute every
+// exchange (none sits under a lane-dependent branch).
+template <int SRC>
+__device__ __forceinline__ double quad_get(double v) {
+  constexpr int ctrl = SRC | (SRC << 2) | (SRC << 4) | (SRC << 6);  // quad_perm [SRC, SRC, SRC, SRC]
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffLL), ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)((unsigned long long)b >> 32), ctrl, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double qget(double v, int src) {  // src is a constant after unrolling
+  switch (src) {
+    case 0: return quad_get<0>(v);
+    case 1: return quad_get<1>(v);
+    case 2: return quad_get<2>(v);
+    default: return quad_get<3>(v);
+  }
+}
 
+// inverse of the block whose row ri this lane holds in bp; every lane gets the whole inverse
 template <int NF>
-__device__ __forceinline__ double cw_inv16(double a, int ri, int cj) {
-  double ac = a, b = (ri == cj) ? 1.0 : 0.0;
+__device__ __forceinline__ void cw_inv_quad(const double (&bp)[NF], int ri, double (&Bi)[NF][NF]) {
+  double ac[NF], bb[NF];
+#pragma unroll
+  for (int j = 0; j < NF; j++) {
+    ac[j] = bp[j];
+    bb[j] = (ri == j) ? 1.0 : 0.0;
+  }
 #pragma unroll
   for (int i = 0; i < NF - 1; i++) {
-    const double aii = grp16(ac, (NF + 1) * i), aji = grp16(ac, NF * ri + i);
-    const double aik = grp16(ac, NF * i + cj), bik = grp16(b, NF * i + cj);
+    double pa[NF], pb[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      pa[k] = qget(ac[k], i);
+      pb[k] = qget(bb[k], i);
+    }
     if (ri > i) {
-      const double f = aji / aii;
-      if (cj > i) ac -= (f * aik);
-      if (cj < ri) b -= (f * bik);
+      const double f = ac[i] / pa[i];
+#pragma unroll
+      for (int k = i + 1; k < NF; k++) ac[k] -= (f * pa[k]);
+#pragma unroll
+      for (int k = 0; k < NF; k++)
+        if (k < ri) bb[k] -= (f * pb[k]);
     }
   }
 #pragma unroll
   for (int i = NF - 1; i >= 0; i--) {
-    double sum = 0.0;
+    if (ri == i) {
 #pragma unroll
-    for (int j = i + 1; j < NF; j++) sum += (grp16(ac, NF * i + j) * grp16(b, NF * j + cj));
-    const double aii = grp16(ac, (NF + 1) * i);
-    if (ri == i) b = (b - sum) / aii;
+      for (int k = 0; k < NF; k++) {
+        double sum = 0.0;
+#pragma unroll
+        for (int j = i + 1; j < NF; j++) sum += (ac[j] * Bi[j][k]);
+        bb[k] = (bb[k] - sum) / ac[i];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NF; k++) Bi[i][k] = qget(bb[k], i);
   }
-  return b;
 }
 
 template <int NF>
-__device__ __forceinline__ void cwc_flow16(const CaseDev &C, const SweepGeom &S, int dir, int l, int sys, int e) {
+__device__ __forceinline__ void cwc_flow_quad(const CaseDev &C, const SweepGeom &S, int dir, int l, int sys, int e) {
   const int nv = C.G.nv, nl = S.nlines, N = S.N;
-  const bool act = e < NF * NF;
-  const int ri = act ? e / NF : 0, cj = act ? e % NF : 0;
+  const bool act = e < NF;
+  const int ri = act ? e : NF - 1;  // the spare lane of a 1D quad reads row NF-1 and stores nothing
   const long vs = (long)nl * (N + 1);
   const long bs = 4L * nl * (N + 1), bb = (long)sys * (N + 1) * nl + l;
   const double *rows = C.cwr[dir];
@@ -1479,81 +1512,126 @@ __device__ __forceinline__ void cwc_flow16(const CaseDev &C, const SweepGeom &S,
   auto rf = [&](int f, int k) { return rows[(long)f * vs + (long)k * nl + l]; };
   auto bk = [&](int f, int k) -> double & { return back[(long)f * bs + bb + (long)k * nl]; };
   auto coef = [&](int which, int k) { return rf(NF * NF + (sys * nv + ri) * 4 + which, k); };
-  // this lane's entry of L and its row's coefficients, loaded one row ahead
-  double nL = rf(NF * ri + cj, 0), na = coef(0, 0), nb = coef(1, 0), nc = coef(2, 0), nF = coef(3, 0);
-  double bp = 0.0, xp = 0.0, cprev = 0.0;
+  // this lane's row of L and its coefficients, loaded one row ahead
+  double nL[NF], na, nb, nc, nF;
+  auto load_row = [&](int k) {
+#pragma unroll
+    for (int j = 0; j < NF; j++) nL[j] = rf(NF * ri + j, k);
+    na = coef(0, k);
+    nb = coef(1, k);
+    nc = coef(2, k);
+    nF = coef(3, k);
+  };
+  load_row(0);
+  double bp[NF], cprev[NF], xp = 0.0;
+#pragma unroll
+  for (int j = 0; j < NF; j++) bp[j] = cprev[j] = 0.0;
   for (int k = 0; k <= N; k++) {
-    const double Lv = nL, a = na, b = nb, c = nc, F = nF;
-    if (k < N) {
-      nL = rf(NF * ri + cj, k + 1);
-      na = coef(0, k + 1);
-      nb = coef(1, k + 1);
-      nc = coef(2, k + 1);
-      nF = coef(3, k + 1);
+    double A[NF], Bm[NF], Cm[NF];
+    const double a = na, b = nb, c = nc, F = nF;
+#pragma unroll
+    for (int j = 0; j < NF; j++) {
+      A[j] = a * nL[j];
+      Bm[j] = b * nL[j];
+      Cm[j] = c * nL[j];
     }
-    const double A = a * Lv, Bm = b * Lv, Cm = c * Lv;
+    if (k < N) load_row(k + 1);
     if (k == 0) {
       xp = F;
-      bp = Bm;
-    } else {
-      const double binv = cw_inv16<NF>(bp, ri, cj);
-      double fac = 0;
 #pragma unroll
-      for (int q = 0; q < NF; q++) fac += (grp16(A, NF * ri + q) * grp16(binv, NF * q + cj));
+      for (int j = 0; j < NF; j++) bp[j] = Bm[j];
+    } else {
+      double Bi[NF][NF];
+      cw_inv_quad<NF>(bp, ri, Bi);
+      double fac[NF];
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        double acc = 0;
+#pragma unroll
+        for (int q = 0; q < NF; q++) acc += (A[q] * Bi[q][j]);
+        fac[j] = acc;
+      }
       // row k-1 for the back substitution
       if (act) {
-        bk(NF * ri + cj, k - 1) = binv;
-        bk(NF * NF + NF * ri + cj, k - 1) = cprev;
-        if (cj == 0) bk(2 * NF * NF + ri, k - 1) = xp;
+#pragma unroll
+        for (int j = 0; j < NF; j++) {
+          bk(NF * ri + j, k - 1) = Bi[ri][j];
+          bk(NF * NF + NF * ri + j, k - 1) = cprev[j];
+        }
+        bk(2 * NF * NF + ri, k - 1) = xp;
       }
-      double bn = Bm, xn = F;
+      double cpf[NF][NF], xpf[NF];
 #pragma unroll
       for (int q = 0; q < NF; q++) {
-        const double fq = grp16(fac, NF * ri + q);
-        bn -= (fq * grp16(cprev, NF * q + cj));
-        xn -= (fq * grp16(xp, NF * q));
+        xpf[q] = qget(xp, q);
+#pragma unroll
+        for (int j = 0; j < NF; j++) cpf[q][j] = qget(cprev[j], q);
       }
-      bp = bn;
+      double xn = F;
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        double bn = Bm[j];
+#pragma unroll
+        for (int q = 0; q < NF; q++) bn -= (fac[q] * cpf[q][j]);
+        bp[j] = bn;
+      }
+#pragma unroll
+      for (int q = 0; q < NF; q++) xn -= (fac[q] * xpf[q]);
       xp = xn;
     }
-    cprev = Cm;
+#pragma unroll
+    for (int j = 0; j < NF; j++) cprev[j] = Cm[j];
   }
   // last row, then back substitution x_k = binv_k (x'_k - C_k x_{k+1})
-  const double binv = cw_inv16<NF>(bp, ri, cj);
+  double Bi[NF][NF], xpf[NF];
+  cw_inv_quad<NF>(bp, ri, Bi);
+#pragma unroll
+  for (int q = 0; q < NF; q++) xpf[q] = qget(xp, q);
   double x = 0;
 #pragma unroll
-  for (int j = 0; j < NF; j++) x += (grp16(binv, NF * ri + j) * grp16(xp, NF * j));
-  if (act && cj == 0) out[((long)ri * nl + l) * (N + 1) + N] = x;
-  double nbi = 0.0, ncp = 0.0, nxq = 0.0;
-  if (N > 0) {
-    nbi = bk(NF * ri + cj, N - 1);
-    ncp = bk(NF * NF + NF * ri + cj, N - 1);
-    nxq = bk(2 * NF * NF + ri, N - 1);
-  }
-  for (int k = N - 1; k >= 0; k--) {
-    const double bi = nbi, cp = ncp, xq = nxq;
-    if (k > 0) {
-      nbi = bk(NF * ri + cj, k - 1);
-      ncp = bk(NF * NF + NF * ri + cj, k - 1);
-      nxq = bk(2 * NF * NF + ri, k - 1);
+  for (int j = 0; j < NF; j++) x += (Bi[ri][j] * xpf[j]);
+  if (act) out[((long)ri * nl + l) * (N + 1) + N] = x;
+  double nbi[NF], ncp[NF], nxq = 0.0;
+  auto load_back = [&](int k) {
+#pragma unroll
+    for (int j = 0; j < NF; j++) {
+      nbi[j] = bk(NF * ri + j, k);
+      ncp[j] = bk(NF * NF + NF * ri + j, k);
     }
+    nxq = bk(2 * NF * NF + ri, k);
+  };
+#pragma unroll
+  for (int j = 0; j < NF; j++) nbi[j] = ncp[j] = 0.0;
+  if (N > 0) load_back(N - 1);
+  for (int k = N - 1; k >= 0; k--) {
+    double bi[NF], cp[NF], xf[NF], yf[NF];
+    const double xq = nxq;
+#pragma unroll
+    for (int j = 0; j < NF; j++) {
+      bi[j] = nbi[j];
+      cp[j] = ncp[j];
+      xf[j] = qget(x, j);
+    }
+    if (k > 0) load_back(k - 1);
     double y = xq;
 #pragma unroll
-    for (int j = 0; j < NF; j++) y -= (grp16(cp, NF * ri + j) * grp16(x, NF * j));
+    for (int j = 0; j < NF; j++) y -= (cp[j] * xf[j]);
+#pragma unroll
+    for (int j = 0; j < NF; j++) yf[j] = qget(y, j);
     double xn = 0;
 #pragma unroll
-    for (int j = 0; j < NF; j++) xn += (grp16(bi, NF * ri + j) * grp16(y, NF * j));
+    for (int j = 0; j < NF; j++) xn += (bi[j] * yf[j]);
     x = xn;
-    if (act && cj == 0) out[((long)ri * nl + l) * (N + 1) + k] = x;
+    if (act) out[((long)ri * nl + l) * (N + 1) + k] = x;
   }
 }
 
-// Work items of a sweep: the flow chains first (16 lanes per system; (system,
-// line) with the line fastest), then one item per species
+// Work items of a sweep: the flow chains first (a quad of lanes per system;
+// (system, line) with the line fastest), then one item per species
 // chain ((chain, system, line), the line fastest), so that a wave does one
 // kind of chain.  The y sweep starts on a 64-item boundary, which keeps every
-// 16-lane group inside one wave's lane group.
-__host__ __device__ inline int cw_flow_lanes(int) { return 16; }
+// quad of a flow chain on one quad of a wave.
+__host__ __device__ inline int cw_flow_lanes(int) { return 4; }
 __host__ __device__ inline long cw_items(const SweepGeom &S, int nv, int nf) {
   return 4L * S.nlines * (cw_flow_lanes(nf) + nv - nf);
 }
@@ -1574,9 +1652,9 @@ __global__ void __launch_bounds__(256) k_cw_solve(CASE_ARGS) {
   const long u = dir ? t - nXs : t;
   const long nl = S.nlines, nflow = 4L * nl * cw_flow_lanes(nf);
   if (u < nflow) {
-    const long grp = u >> 4;
-    if (nf == 4) cwc_flow16<4>(C, S, dir, (int)(grp % nl), (int)(grp / nl), (int)(u & 15));
-    else cwc_flow16<3>(C, S, dir, (int)(grp % nl), (int)(grp / nl), (int)(u & 15));
+    const long grp = u >> 2;
+    if (nf == 4) cwc_flow_quad<4>(C, S, dir, (int)(grp % nl), (int)(grp / nl), (int)(u & 3));
+    else cwc_flow_quad<3>(C, S, dir, (int)(grp % nl), (int)(grp / nl), (int)(u & 3));
     return;
   }
   const long v = u - nflow, per = 4L * nl;
