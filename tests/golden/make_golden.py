@@ -230,6 +230,11 @@ def example_logs():
     meta["InviscidVortexConvection_errors_dat"] = {"dt": float(vals[4]), "errors": [float(v) for v in vals[5:8]],
                                         "source": "Examples/2D_NavierStokes/InviscidVortexConvection/errors.dat",
                                         "note": "shipped run used 2x2 MPI ranks (Jacobi reduced solve)"}
+    # Riemann2DCase4 conservation.dat: NI, NJ, ranks along x and y, dt, then the error of each variable
+    vals = open(os.path.join(REF_TREE, "Examples", "2D_NavierStokes", "Riemann2DCase4", "conservation.dat")).read().split()
+    meta["Riemann2DCase4_conservation_dat"] = {"dt": float(vals[4]), "errors": [float(v) for v in vals[5:]],
+                                               "source": "Examples/2D_NavierStokes/Riemann2DCase4/conservation.dat",
+                                               "note": "shipped run used 2x2 MPI ranks"}
     with open(os.path.join(HERE, "examples.json"), "w") as f:
         json.dump(meta, f, indent=1)
 
@@ -443,6 +448,8 @@ if __name__ == "__main__":
         ref_runs(only_missing=True)
     elif len(sys.argv) > 1 and sys.argv[1] == "--examples-2d":
         examples_2d(sys.argv[2:] or None)
+    elif len(sys.argv) > 1 and sys.argv[1] == "--example-logs":
+        example_logs()
     else:
         ref_runs()
         example_logs()

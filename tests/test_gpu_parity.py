@@ -214,6 +214,9 @@ def test_shipped_examples_through_the_driver(examples, example):
         assert _close_in_last_digit(rows[it][1], nm, "%1.4E"), (it, rows[it][1], nm)
     if cons is not None:
         np.testing.assert_allclose(cons, ref["conservation"], rtol=0, atol=1e-11)
+    shipped = examples.get(example + "_conservation_dat")
+    if shipped is not None:  # the example's own conservation.dat (2x2 ranks): roundoff-level errors as well
+        np.testing.assert_allclose(cons, shipped["errors"], rtol=0, atol=1e-12)
     if "errors" in ref:  # L1, L2, Linf against the example's exact.inp (zeros without one)
         np.testing.assert_allclose(errs, ref["errors"], rtol=1e-9, atol=1e-15)
     norm = np.sqrt((U.reshape(-1, nv) ** 2).sum(0))

@@ -260,3 +260,12 @@ def test_chemistry_constants_from_formulas():
         np.testing.assert_allclose(c.param("Re"), 1000.0, rtol=1e-12)
         np.testing.assert_allclose(c.param("Pr"), Pr, rtol=1e-12)
         np.testing.assert_allclose(c.param("gamma"), 918.45 / 650.0, rtol=1e-15)
+
+
+def test_riemann2dcase4_conservation_fixture_against_shipped_file(examples):
+    """Examples/2D_NavierStokes/Riemann2DCase4/conservation.dat (written on 2x2
+    ranks) against the reference's serial run of the same example, the fixture
+    the driver test compares with: both are roundoff-level conservation errors
+    of each variable."""
+    np.testing.assert_allclose(examples["Riemann2DCase4"]["conservation"],
+                               examples["Riemann2DCase4_conservation_dat"]["errors"], rtol=0, atol=1e-12)
