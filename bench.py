@@ -70,6 +70,62 @@ def measured_traffic(nx):
     return (float(m.group(1)) + float(m.group(2))) * 1e6 if m else None
 
 
+# MI355X: 256 CUs x 4 SIMDs at 2.4 GHz; a wave64 FP64 VALU instruction issues
+# every 4 cycles on a 16-lane SIMD (the rate behind the 78.6 TFLOP/s FP64
+# vector peak: 256 * 4 * 16 lanes * 2 flops * 2.4e9)
+WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4
+FP64_PEAK_FLOPS = 78.6e12
+
+
+def valu_counters(nx, kernel):
+    """Per-launch VALU counters of `kernel` from the committed rocprofv3 --pmc
+    runs of this workload (the `counters-json` line of the profile summary),
+    or None."""
+    path = os.path.join(ROOT, "profiles", f"ns2d_crweno5_{nx}_step_kernels.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^counters-json: (\{.*\})\s*$", open(path).read(), re.M)
+    if not m:
+        return None
+    return json.loads(m.group(1)).get("k_" + kernel)
+
+
+def valu_fractions(c, ms):
+    """VALU issue fraction (every VALU wave-instruction costs one FP64 issue
+    slot), FP64 flop fraction of the vector peak (64 lanes per wave-instruction,
+    FMA = 2 flops) and the fraction of the attainable roofline
+    min(HBM BW x AI, FP64 peak), AI from the same flops over the kernel's
+    measured HBM bytes."""
+    t = ms * 1e-3
+    out = {"valu_wave_insts_per_launch": c["SQ_INSTS_VALU"],
+           "valu_issue_frac": c["SQ_INSTS_VALU"] / WAVE_INSTS_PER_S / t}
+    f64 = ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64")
+    if all(k in c for k in f64):
+        flops = 64.0 * (c[f64[0]] + c[f64[1]] + 2 * c[f64[2]] + c[f64[3]])
+        out["fp64_flops_per_launch"] = flops
+        out["fp64_frac_of_peak"] = flops / t / FP64_PEAK_FLOPS
+        out["fp64_insts_frac_of_valu"] = sum(c[k] for k in f64) / c["SQ_INSTS_VALU"]
+        hbm = c.get("hbm_bytes")
+        if hbm:
+            attainable = min(HBM_PEAK_GBS * 1e9 * flops / hbm, FP64_PEAK_FLOPS)
+            out["attainable_flops"] = attainable
+            out["attainable_frac"] = flops / t / attainable
+    return out
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and its logical core count."""
+    name = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return name, os.cpu_count()
+
+
 def cpu_baseline(tmp, nx_sample=256, steps=10):
     """Reference PIAFS (built from source, serial, 1 core) on a bounded sample of
     the same workload: the benchmark deck at nx_sample^2 with the same dx, dt and
@@ -81,8 +137,9 @@ def cpu_baseline(tmp, nx_sample=256, steps=10):
     dk.write(d)
     ref = os.path.join(ROOT, "oracle", "_ref", "PIAFS")
     cells = nx_sample * nx_sample * steps
+    model, ncpu = cpu_model()
     sample = (f"the 1024^2 chemistry deck restricted to {nx_sample}x{nx_sample} (same dx, dt, physics), "
-              f"{steps} RK4 steps")
+              f"{steps} RK4 steps, on one core of the host CPU ({model}, {ncpu} logical cores)")
     if os.path.exists(ref):
         r = subprocess.run([ref], cwd=d, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
         m = re.search(r"Solver runtime \(in seconds\):\s*(\S+)", r.stdout)
@@ -90,7 +147,7 @@ def cpu_baseline(tmp, nx_sample=256, steps=10):
             secs = float(m.group(1))
             return {"value": cells / secs, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
                     "sample": sample + "; reference binary built from source (oracle/_ref/PIAFS), serial",
-                    "seconds": secs}
+                    "seconds": secs, "cpu_model": model}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as orc
     c = orc.OCase(d)
@@ -98,7 +155,8 @@ def cpu_baseline(tmp, nx_sample=256, steps=10):
     c.step(steps)
     secs = time.perf_counter() - t0
     return {"value": cells / secs, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": sample + "; CPU restatement (oracle/liboracle.so), serial", "seconds": secs}
+            "sample": sample + "; CPU restatement (oracle/liboracle.so), serial", "seconds": secs,
+            "cpu_model": model}
 
 
 def _write_members(tmp, members, sub):
@@ -146,7 +204,7 @@ def bench_batch(tmp, n_cases=256, steps=20, serial_steps=10, oracle_check=True):
         err = np.sqrt(((got - ref) ** 2).sum(0)) / np.maximum(np.sqrt((ref ** 2).sum(0)), 1e-300)
         out["oracle_check"] = {"case": members[k].name, "steps": warm + steps, "max_rel_l2_per_var": float(err.max())}
     b.close()
-    # one case at a time (eager launches of the single-case path, as a caller of piafs_step)
+    # one case at a time through the single-case path (each step replayed from the case's own graph)
     t_s = 0.0
     for d in dirs:
         c = piafs_amd.Case(d)
@@ -287,7 +345,9 @@ def main():
     ap.add_argument("--grid", type=int, default=1024, choices=(1024, 4096),
                     help="1024: 1024^2, dt 0.01 (default); 4096: 4096^2, dt 0.0025, 50 steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-class timing pass")
+    ap.add_argument("--profile-steps", type=int, default=10,
+                    help="steps of the per-class timing pass that follows the timed region")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default line-solve segment length (for profiling runs)")
     ap.add_argument("--no-batch", action="store_true", help="skip the ensemble leg")
@@ -333,13 +393,22 @@ def main():
         tuned_rows = None if args.no_autotune else case.autotune()[0]
         case.step(args.warmup)
         case.sync()
-        case.profile(not args.no_profile)
+        # the timed region: steps replayed from the captured HIP graph, as a
+        # caller of piafs_step runs them
         t0 = time.perf_counter()
         case.step(args.steps)
         case.sync()
         secs = time.perf_counter() - t0
-        ktimes = {} if args.no_profile else case.kernel_times()
-        case.profile(False)
+        ktimes = {}
+        if not args.no_profile:
+            # per-class device times in a pass of their own right after it: the
+            # same launches issued eagerly, HIP events around each on the stream
+            # it runs on
+            case.profile(True)
+            case.step(args.profile_steps)
+            case.sync()
+            ktimes = case.kernel_times()
+            case.profile(False)
         cells = nx * nx
         value = cells * args.steps / secs
         line = {
@@ -359,13 +428,23 @@ def main():
             share = {k: v * per_step.get(k, 1) for k, v in ktimes.items()}
             dom = max(share.items(), key=lambda kv: kv[1])[0]
             paired = (not inf.characteristic) and dk.physics["upwinding"] in ("roe", "rusanov")
-            byts = kernel_algorithmic_bytes(dom, nx, nx, nv, g, inf.nstages, True, paired)
+            io_byts = kernel_algorithmic_bytes(dom, nx, nx, nv, g, inf.nstages, True, paired)
             ms = ktimes[dom]
-            ach = byts / (ms * 1e-3) / 1e9 if byts else None
+            # algorithmic bytes of one stage of a stage-fused RK step, 40 nv B per
+            # cell (read the stage state and u_n, read and write the accumulator,
+            # write the next stage state); the dominant kernel runs once per stage
+            byts = STEP_BYTES_PER_CELL(nv, 1) * cells
+            ach = byts / (ms * 1e-3) / 1e9
             line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                "frac": (ach / HBM_PEAK_GBS) if ach else None,
+                                "frac": ach / HBM_PEAK_GBS,
                                 "traffic": measured_traffic(nx) if dom == "recon" else None,
-                                "kernel": dom, "kernel_ms": ms, "algorithmic_bytes_per_launch": byts}
+                                "kernel": dom, "kernel_ms": ms, "algorithmic_bytes_per_launch": byts,
+                                "algorithmic_bytes_rule": "40 * nvars B per cell-update and stage",
+                                "kernel_io_bytes_per_launch": io_byts,
+                                "kernel_io_frac": (io_byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if io_byts else None}
+            valu = valu_counters(nx, dom)
+            if valu:
+                line["roofline"].update(valu_fractions(valu, ms))
             line["kernel_ms"] = ktimes
             line["kernel_ms_per_step"] = share
         step_gbs = STEP_BYTES_PER_CELL(nv, inf.nstages) * value / 1e9

@@ -128,6 +128,15 @@ int  piafs_write_binary(piafs_case *c, const char *path);
  * in dir.  Returns the wall time of the time loop in *seconds (may be NULL). */
 int  piafs_solve(piafs_case *c, const char *dir, int verbose, double *seconds);
 
+/* Conservation error of every variable after the steps taken so far (nvars
+ * values; -1 before the first step): |volume integral + time integral of the
+ * boundary flux - initial volume integral| / max(1, |initial volume
+ * integral|).  Replaces CalculateConservationError
+ * (src/HyParFunctions/CalculateConservationError.c:19-43) with the boundary
+ * integral of TimeRK.c:86-97 and BoundaryIntegral.c:37-58.  Needs
+ * conservation_check yes in solver.inp (returns 1 otherwise). */
+int  piafs_conservation_errors(piafs_case *c, double *errors);
+
 /* Device timing of the dominant kernel: average duration (ms) of each
  * kernel class over the last piafs_step call when profiling is enabled.
  * names/ms arrays of length *n (in: capacity, out: count). */

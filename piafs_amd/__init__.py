@@ -14,14 +14,29 @@ end with :meth:`Case.solve` (``Solve``), and written with
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpiafs_amd.so")
 _lib = None
+# every open handle, so that the ones a program leaves open are closed by an
+# exit hook while the interpreter and the HIP runtime are both fully alive,
+# not by a finalizer during interpreter teardown
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_handles():
+    for h in list(_live):
+        try:
+            h.close()
+        except Exception:
+            pass
 
 
 class PiafsError(RuntimeError):
@@ -47,7 +62,7 @@ EXPORTS = [
     "piafs_estimate_bytes", "piafs_autotune", "piafs_batch_autotune", "piafs_server_create", "piafs_server_destroy", "piafs_server_submit",
     "piafs_server_poll", "piafs_server_result", "piafs_server_release", "piafs_server_wait_all",
     "piafs_server_stats", "piafs_server_last_error", "piafs_boundary_fill", "piafs_flux",
-    "piafs_interface_flux", "piafs_rhs_terms",
+    "piafs_interface_flux", "piafs_rhs_terms", "piafs_conservation_errors",
 ]
 
 
@@ -77,6 +92,7 @@ def lib():
     L.piafs_interface_flux.argtypes = [vp, vp, ctypes.c_double, ip, vp]
     L.piafs_rhs_terms.argtypes = [vp, vp, ctypes.c_double, vp, vp, vp]
     L.piafs_write_binary.argtypes = [vp, cp]
+    L.piafs_conservation_errors.argtypes = [vp, vp]
     L.piafs_solve.argtypes = [vp, cp, ip, dp]
     L.piafs_kernel_times.argtypes = [vp, ip, ctypes.POINTER(cp), dp, ctypes.POINTER(ctypes.c_int)]
     L.piafs_tridiag_host.argtypes = [vp, vp, vp, vp, ip, ip]
@@ -132,6 +148,7 @@ class Case:
         self._h = h
         self._L = L
         self.directory = directory
+        _live.add(self)
 
     # -- queries --
     @property
@@ -231,6 +248,13 @@ class Case:
     def write_binary(self, path: str) -> None:
         self._check(self._L.piafs_write_binary(self._h, path.encode()))
 
+    def conservation_errors(self) -> np.ndarray:
+        """Per-variable conservation error after the steps taken so far
+        (piafs_conservation_errors; the deck must set conservation_check yes)."""
+        out = np.empty(self.info.nvars)
+        self._check(self._L.piafs_conservation_errors(self._h, _ptr(out)))
+        return out
+
     def solve(self, directory: str | None = None, verbose: bool = False) -> float:
         secs = ctypes.c_double(0)
         d = directory if directory is not None else self.directory
@@ -293,6 +317,7 @@ class Batch:
             self.close()
             raise PiafsError(f"piafs_batch_create failed ({rc}): {msg}")
         self.slots = list(range(len(directories)))
+        _live.add(self)
 
     def _check(self, rc, what):
         if rc:
@@ -381,6 +406,7 @@ class Server:
             msg = L.piafs_server_last_error(h).decode() if h else ""
             self.close()
             raise PiafsError(f"piafs_server_create failed ({rc}): {msg}")
+        _live.add(self)
 
     def _err(self):
         return self._L.piafs_server_last_error(self._h).decode()

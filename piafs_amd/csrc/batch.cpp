@@ -40,6 +40,7 @@ static Case *slot_case(piafs_batch *b, int k) {
 extern "C" {
 
 int piafs_batch_create(const char *const *dirs, int ncases, int device, piafs_batch **out) {
+  if (!out || (ncases > 0 && !dirs) || ncases < 0) return 1;
   *out = nullptr;
   piafs_batch *b = new piafs_batch();
   b->eng.owns_cases = false;

@@ -219,6 +219,7 @@ void piafs_server::run_lane(int L) {
 extern "C" {
 
 int piafs_server_create(int device, size_t hbm_budget, piafs_server **out) {
+  if (!out) return 1;
   *out = nullptr;
   piafs_server *s = new piafs_server();
   s->device = device;

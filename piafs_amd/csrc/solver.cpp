@@ -624,6 +624,10 @@ static int case_fail(piafs_case *pc, int rc) {
 extern "C" {
 
 int piafs_create(const char *dir, int device, piafs_case **out) {
+  if (!out) {
+    g_create_error = "output handle pointer is NULL";
+    return 1;
+  }
   *out = nullptr;
   piafs_case *pc = new piafs_case();
   pc->eng.use_graph = true;  // steps replay a captured graph (eager launches while profiling)
@@ -922,6 +926,21 @@ int piafs_write_binary(piafs_case *pc, const char *path) {
   return rc ? case_fail(pc, rc) : 0;
 }
 
+int piafs_conservation_errors(piafs_case *pc, double *errors) {
+  if (!pc || !pc->c) return 1;  // no case
+  if (!errors) return null_arg(pc, "output");
+  if (!pc->c->h.cons) {
+    pc->err = "the deck does not set conservation_check yes";
+    return 1;
+  }
+  std::vector<double> ce;
+  int rc = pc->eng.sync();
+  if (!rc) rc = pc->c->conservation_errors(ce, pc->eng.stream);
+  if (rc) return case_fail(pc, rc);
+  std::copy(ce.begin(), ce.end(), errors);
+  return 0;
+}
+
 int piafs_kernel_times(piafs_case *pc, int enable, const char **names, double *ms, int *n) {
   if (!pc || !pc->c) return 1;  // no case
   Engine &e = pc->eng;
@@ -961,6 +980,10 @@ int piafs_autotune(piafs_case *pc, int *rows_per_lane, double *recon_ms) {
 }
 
 int piafs_estimate_bytes(const char *dir, size_t *bytes) {
+  if (!bytes) {
+    g_create_error = "output pointer is NULL";
+    return 1;
+  }
   Config cfg;
   std::string err;
   if (cfg.parse(dir ? dir : "", err)) {

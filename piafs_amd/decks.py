@@ -883,12 +883,15 @@ class Request:
 
 
 def request_stream(n: int, rate: float, seed: int = 20261203, frac_1d: float = 0.3,
-                   n1d=(512, 8192), side2d=(128, 1024)) -> list:
+                   n1d=(512, 8192), side2d=(128, 1024), steps_1d: int | None = None,
+                   steps_2d: int = 200) -> list:
     """Poisson arrivals (rate per second) of solve requests to a fixed final
     time: a fraction frac_1d are Sod shock tubes (N log-uniform in n1d,
     t_final 0.2, the single-case 1D deck), the rest 2D NS + photochemistry
     (side log-uniform in side2d, the single-case 2D deck with dx = 10 pi /
-    1024, Re 1000, dt 0.01, t_final 2.0)."""
+    1024, Re 1000, dt 0.01, t_final 2.0).  steps_1d / steps_2d shorten the
+    requests to that many steps of the same dt (the same stream, stopped
+    early)."""
     dx = 10.0 * math.pi / 1024
     out = []
     t = 0.0
@@ -897,12 +900,14 @@ def request_stream(n: int, rate: float, seed: int = 20261203, frac_1d: float = 0
         t += -math.log(1.0 - r[0]) / rate
         if r[1] < frac_1d:
             N = int(round(math.exp(math.log(n1d[0]) + (math.log(n1d[1]) - math.log(n1d[0])) * r[2])))
-            dk = sod_1d(N=N)
-            m = Member(f"r{k:03d}_sod{N}", dk, N, int(dk.solver["n_iter"]))
+            full = int(math.ceil(0.88 * (N - 1)))
+            ns = full if steps_1d is None else min(steps_1d, full)
+            dk = sod_1d(N=N, n_iter=ns, t_final=0.2 if ns == full else 0.2 * ns / full)
+            m = Member(f"r{k:03d}_sod{N}", dk, N, ns)
         else:
             a = math.log(side2d[0])
             side = int(round(math.exp(a + (math.log(side2d[1]) - a) * r[2])))
-            dk = ns2d_photochem(nx=side, n_iter=200, dt=0.01, seed=seed + k, dx=dx, L=side * dx)
-            m = Member(f"r{k:03d}_ns{side}", dk, side * side, 200)
+            dk = ns2d_photochem(nx=side, n_iter=steps_2d, dt=0.01, seed=seed + k, dx=dx, L=side * dx)
+            m = Member(f"r{k:03d}_ns{side}", dk, side * side, steps_2d)
         out.append(Request(t, m))
     return out

@@ -144,6 +144,11 @@ CASES = {
     "ns2d_crweno_char_flow_16x4": ("ns2d_open_box", dict(nx=16, ny=4, n_iter=8, flow=(1.0, 0.3, 0.2, None),
                                                          scheme="crweno5", interp="characteristic",
                                                          boundary=PERIODIC_2D)),
+    # the remaining explicit RK tableaux (TimeExplicitRKInitialize.c:61-82): forward Euler, RK2, RK3
+    "sod_char_roe_rk1fe_N64": ("sod_1d", dict(N=64, n_iter=40, t_final=0.05, rk="1fe")),
+    "sine_crweno_comp_rk22_N64": ("density_sine_1d", dict(N=64, n_iter=30, rk="22")),
+    "ns2d_chem_crweno_rk33_16x12": ("ns2d_photochem", dict(nx=16, ny=12, n_iter=8, rk="33")),
+    "ns2d_chem_weno_rk22_12x16": ("ns2d_photochem", dict(nx=12, ny=16, n_iter=6, rk="22", scheme="weno5")),
     # upw5, the linear fifth-order upwind interpolation (Interp1PrimFifthOrderUpwind[Char].c)
     "sod_upw5_char_roe_N64": ("sod_1d", dict(N=64, n_iter=20, t_final=0.07, scheme="upw5")),
     "sod_upw5_comp_rf_N64": ("sod_1d", dict(N=64, n_iter=20, t_final=0.07, scheme="upw5", interp="components",
@@ -276,6 +281,34 @@ def ns2d_4096_nochem(run_dir):
         colabs=np.abs(U).sum(axis=0), rowabs=np.abs(U).sum(axis=1), sub=U[::128, ::128, :].copy(),
     )
     print("4096^2 chemistry-off fixture written", U.shape)
+
+
+def ns2d_4096_chem_oracle(steps=2):
+    """The 4096^2 photochemistry deck (nvars 11, CRWENO5 + viscous terms, RK4,
+    dt 0.0025) advanced `steps` steps by the CPU restatement (the reference
+    itself needs about 77 GB here; the restatement, bitwise equal to it on
+    every smaller reference run, about 40 GB), reduced to norms, sums and a
+    stride-128 subsample.  The photolysis pulse is on from t = 0, so the
+    chemistry source acts in every stage."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as orc
+    with tempfile.TemporaryDirectory() as d:
+        dk = decks.ns2d_photochem(nx=4096, n_iter=50, dt=0.0025)
+        dk.write(d)
+        del dk
+        o = orc.OCase(d)
+        o.step(steps)
+        u = o.solution()
+        t = o.time
+        del o
+    U = u.reshape(4096, 4096, -1)
+    np.savez_compressed(
+        os.path.join(HERE, f"ns2d_chem_crweno_4096_{steps}steps_oracle.npz"),
+        l2=np.sqrt((U * U).sum(axis=(0, 1))), colsum=U.sum(axis=0), rowsum=U.sum(axis=1),
+        colabs=np.abs(U).sum(axis=0), rowabs=np.abs(U).sum(axis=1), sub=U[::128, ::128, :].copy(),
+        steps=np.int64(steps), t=np.float64(t),
+    )
+    print(f"4096^2 chemistry fixture written ({steps} steps, t = {t})", U.shape)
 
 
 # Decks whose complete file output (solution, species, errors.dat and the
@@ -442,6 +475,8 @@ if __name__ == "__main__":
         ns2d_1024_full(sys.argv[2])
     elif len(sys.argv) > 2 and sys.argv[1] == "--ns2d-4096-nochem":
         ns2d_4096_nochem(sys.argv[2])
+    elif len(sys.argv) > 1 and sys.argv[1] == "--ns2d-4096-chem-oracle":
+        ns2d_4096_chem_oracle(int(sys.argv[2]) if len(sys.argv) > 2 else 2)
     elif len(sys.argv) > 1 and sys.argv[1] == "--outputs":
         outputs()
     elif len(sys.argv) > 1 and sys.argv[1] == "--add":

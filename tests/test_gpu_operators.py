@@ -85,3 +85,20 @@ def test_operator_pieces_match_oracle(ref_runs, name):
     assert np.all(_rel_max(rhs, rhs_ref) <= TOL_TERMS)
     # the case's own state is left as it was
     assert np.array_equal(after, dk.u)
+
+
+@pytest.mark.parametrize("which", ["2d", "1d"])
+def test_reflect_boundary_fill_matches_oracle(which):
+    """The reflect rule (BCReflect.c:17-50) through piafs_boundary_fill: the
+    ghost points equal the oracle's fill bit for bit (a negated copy), which
+    test_oracle_pinning pins against the closed form."""
+    from test_oracle_pinning import reflect_decks
+    dk = reflect_decks()[which]
+    rng = np.random.default_rng(17)
+    u = dk.u * (1.0 + 1e-2 * rng.uniform(-1, 1, dk.u.shape))
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        ref = orc.OCase(d).rhs_parts(u, 0.0)["u_ghosted"]
+        with piafs_amd.Case(d) as c:
+            got = c.boundary_fill(u, 0.0)
+    assert np.array_equal(got, ref)

@@ -96,6 +96,8 @@ FIXTURES = [
     "sod_upw5_char_roe_N64", "sod_upw5_comp_rf_N64", "ns2d_chem_upw5_comp_roe_20x16",
     "ns2d_upw5_char_llf_flow_20x16", "ns2d_upw5_cavity_re400_20x20",
     "ns2d_chem_crweno_16x4", "ns2d_chem_char_rf_24x4", "ns2d_crweno_char_flow_16x4",
+    "sod_char_roe_rk1fe_N64", "sine_crweno_comp_rk22_N64", "ns2d_chem_crweno_rk33_16x12",
+    "ns2d_chem_weno_rk22_12x16",
 ]
 
 

@@ -44,6 +44,8 @@ def _run_oracle(dk, n=None):
     "sod_upw5_char_roe_N64", "sod_upw5_comp_rf_N64", "ns2d_chem_upw5_comp_roe_20x16",
     "ns2d_upw5_char_llf_flow_20x16", "ns2d_upw5_cavity_re400_20x20",
     "ns2d_chem_crweno_16x4", "ns2d_chem_char_rf_24x4", "ns2d_crweno_char_flow_16x4",
+    "sod_char_roe_rk1fe_N64", "sine_crweno_comp_rk22_N64", "ns2d_chem_crweno_rk33_16x12",
+    "ns2d_chem_weno_rk22_12x16",
 ])
 def test_oracle_matches_reference_bitwise(ref_runs, name):
     data, meta = ref_runs
@@ -269,3 +271,50 @@ def test_riemann2dcase4_conservation_fixture_against_shipped_file(examples):
     of each variable."""
     np.testing.assert_allclose(examples["Riemann2DCase4"]["conservation"],
                                examples["Riemann2DCase4_conservation_dat"]["errors"], rtol=0, atol=1e-12)
+
+
+def reflect_decks():
+    """Decks whose every face is a reflect zone (BCReflect.c:17-50): a 2D box
+    with a moving gas (two zones on one face, split across it) and a 1D tube.
+    A step with them is meaningless (the ghost density is negative, which the
+    upwinders take the square root of); the boundary fill alone is pinned."""
+    two = decks.ns2d_open_box(nx=14, ny=11, n_iter=1, flow=(1.0, 0.3, 0.2, None), boundary=[
+        ("reflect", 0, 1, 0, 0, -1e99, 0.4), ("reflect", 0, 1, 0, 0, 0.4, 1e99), ("reflect", 0, -1, 0, 0, -1e99, 1e99),
+        ("reflect", 1, 1, -1e99, 1e99, 0, 0), ("reflect", 1, -1, -1e99, 1e99, 0, 0)])
+    one = decks.sod_1d(N=24, n_iter=1, boundary=[("reflect", 0, 1, 0, 0), ("reflect", 0, -1, 0, 0)])
+    return {"2d": two, "1d": one}
+
+
+@pytest.mark.parametrize("which", ["2d", "1d"])
+def test_reflect_ghosts_are_negated_mirror(which):
+    """The oracle's reflect fill equals the closed form of BCReflect.c:40-46:
+    ghost point g-1-b (low face) or N+b (high face) along the zone's dimension
+    holds -1 times interior point b (low) or N-1-b (high), every variable; the
+    corners, which no zone reaches, stay zero."""
+    dk = reflect_decks()[which]
+    rng = np.random.default_rng(17)
+    u = dk.u * (1.0 + 1e-2 * rng.uniform(-1, 1, dk.u.shape))
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        P = orc.OCase(d).rhs_parts(u, 0.0)
+    g, nv = 3, dk.u.shape[1]
+    if which == "1d":
+        N = dk.solver["size"]
+        ug = P["u_ghosted"].reshape(N + 2 * g, nv)
+        assert np.array_equal(ug[g:g + N], u)
+        for b in range(g):
+            assert np.array_equal(ug[g - 1 - b], -1.0 * u[b])
+            assert np.array_equal(ug[g + N + b], -1.0 * u[N - 1 - b])
+        return
+    nx, ny = dk.solver["size"]
+    ug = P["u_ghosted"].reshape(ny + 2 * g, nx + 2 * g, nv)
+    ui = u.reshape(ny, nx, nv)
+    assert np.array_equal(ug[g:g + ny, g:g + nx], ui)
+    for b in range(g):
+        assert np.array_equal(ug[g:g + ny, g - 1 - b], -1.0 * ui[:, b])
+        assert np.array_equal(ug[g:g + ny, g + nx + b], -1.0 * ui[:, nx - 1 - b])
+        assert np.array_equal(ug[g - 1 - b, g:g + nx], -1.0 * ui[b, :])
+        assert np.array_equal(ug[g + ny + b, g:g + nx], -1.0 * ui[ny - 1 - b, :])
+    for cj in (slice(0, g), slice(g + ny, None)):
+        for ci in (slice(0, g), slice(g + nx, None)):
+            assert not ug[cj, ci].any()
