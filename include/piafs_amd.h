@@ -128,6 +128,18 @@ int  piafs_write_binary(piafs_case *c, const char *path);
  * in dir.  Returns the wall time of the time loop in *seconds (may be NULL). */
 int  piafs_solve(piafs_case *c, const char *dir, int verbose, double *seconds);
 
+/* Derivative ghosts of the viscous term with the semantics of a run
+ * decomposed over more than one rank along each periodic dimension (on = 1)
+ * or of a serial run (on = 0, the default).  Replaces the call
+ * MPIExchangeBoundariesnD(QDerivX / QDerivY) of
+ * src/PhysicalModels/NavierStokes2D/NavierStokes2DParabolicFunction.c:91-94
+ * (src/MPIFunctions/MPIExchangeBoundariesnD.c:45-176), a no-op in a serial
+ * run: along a dimension with a periodic zone, the ghost points of the
+ * interior lines across it receive the derivatives of their periodic images;
+ * the viscous term is then invariant under a periodic translation of the
+ * state.  Affects navierstokes2d with Re > 0 only. */
+int  piafs_set_derivative_exchange(piafs_case *c, int on);
+
 /* Conservation error of every variable after the steps taken so far (nvars
  * values; -1 before the first step): |volume integral + time integral of the
  * boundary flux - initial volume integral| / max(1, |initial volume

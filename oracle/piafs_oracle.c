@@ -77,6 +77,10 @@ struct orc_case {
   /* optional capture of the interface fluxes of every line, per dimension
    * ([line][interface][var]), set by orc_rhs_parts */
   double *fi_cap[2];
+  /* derivative ghosts of the parabolic term exchanged along periodic
+   * dimensions, as a run with more than one rank along them does
+   * (orc_set_derivative_exchange) */
+  int    deriv_exchange;
 };
 
 /* ------------------------------------------------------------------ */
@@ -1554,6 +1558,44 @@ static void first_derivative(const orc_case *c, double *Df, const double *f, int
   }
 }
 
+/* MPIExchangeBoundariesnD (MPIExchangeBoundariesnD.c:45-176) on one rank per
+ * ... of a run decomposed along every periodic dimension, seen from the whole
+ * grid: along each dimension d with a periodic zone (InitializeBoundaries.c:
+ * 179-181 sets bcperiodic[d] when more than one rank lies along d), the g
+ * ghost layers of the interior lines across d receive the opposite g interior
+ * layers, all nvars components (:109-167); the corners are not exchanged.
+ * Every value read is an interior one, so the dimensions' order is immaterial. */
+static int periodic_dim(const orc_case *c, int d)
+{
+  int k;
+  for (k = 0; k < c->nbc; k++)
+    if (c->bc[k].dim == d && !strcmp(c->bc[k].type, "periodic")) return 1;
+  return 0;
+}
+
+static void exchange_periodic_ghosts(const orc_case *c, double *var)
+{
+  int g = c->ghosts, nv = c->nvars, d, l, k, v;
+  for (d = 0; d < c->ndims; d++) {
+    int other = (c->ndims == 2) ? 1 - d : -1, N = c->dim[d];
+    int nl = (other >= 0) ? c->dim[other] : 1;
+    if (!periodic_dim(c, d)) continue;
+    for (l = 0; l < nl; l++)
+      for (k = 0; k < g; k++) {
+        int glo[3] = {0, 0, 0}, ihi[3] = {0, 0, 0}, ghi[3] = {0, 0, 0}, ilo[3] = {0, 0, 0};
+        if (other >= 0) glo[other] = ihi[other] = ghi[other] = ilo[other] = l;
+        glo[d] = -g + k;  ihi[d] = N - g + k;   /* low ghosts <- the last g interior layers */
+        ghi[d] = N + k;   ilo[d] = k;           /* high ghosts <- the first g interior layers */
+        for (v = 0; v < nv; v++) {
+          var[(size_t)gidx(c, glo) * nv + v] = var[(size_t)gidx(c, ihi) * nv + v];
+          var[(size_t)gidx(c, ghi) * nv + v] = var[(size_t)gidx(c, ilo) * nv + v];
+        }
+      }
+  }
+}
+
+void orc_set_derivative_exchange(orc_case *c, int on) { c->deriv_exchange = on ? 1 : 0; }
+
 static void parabolic_ns2d(orc_case *c, const double *u, double *par)
 {
   int g = c->ghosts, nv = c->nvars, i, j, v, imax = c->dim[0], jmax = c->dim[1];
@@ -1575,6 +1617,11 @@ static void parabolic_ns2d(orc_case *c, const double *u, double *par)
     }
   first_derivative(c, QdX, Q, 0, 1);
   first_derivative(c, QdY, Q, 1, 1);
+  /* NavierStokes2DParabolicFunction.c:91-94: a no-op in a serial run */
+  if (c->deriv_exchange) {
+    exchange_periodic_ghosts(c, QdX);
+    exchange_periodic_ghosts(c, QdY);
+  }
   for (i = -g; i < imax + g; i++)
     for (j = -g; j < jmax + g; j++) {
       int idx[2] = {i, j};

@@ -69,6 +69,14 @@ int  orc_rhs(orc_case *c, const double *u_in, double t, double *rhs_out);
 int  orc_rhs_parts(orc_case *c, const double *u_in, double t, double *u_ghosted, double *flux0, double *flux1,
                    double *fi0, double *fi1, double *hyp, double *par, double *src);
 
+/* Derivative ghosts of the viscous term as a run decomposed over more than
+ * one rank along each periodic dimension computes them: the MPI exchange of
+ * QDerivX / QDerivY (NavierStokes2DParabolicFunction.c:91-94,
+ * MPIExchangeBoundariesnD.c:45-176), a no-op in a serial run, copies the
+ * periodic images into the ghost layers of the interior lines.  Off by
+ * default (serial semantics). */
+void orc_set_derivative_exchange(orc_case *c, int on);
+
 /* ---- function-level entry points used by unit tests ---- */
 
 /* tridiagLU serial path (tridiagLU.c:86-277): n rows, ns systems,

@@ -63,6 +63,7 @@ EXPORTS = [
     "piafs_server_poll", "piafs_server_result", "piafs_server_release", "piafs_server_wait_all",
     "piafs_server_stats", "piafs_server_last_error", "piafs_boundary_fill", "piafs_flux",
     "piafs_interface_flux", "piafs_rhs_terms", "piafs_conservation_errors",
+    "piafs_set_derivative_exchange",
 ]
 
 
@@ -93,6 +94,7 @@ def lib():
     L.piafs_rhs_terms.argtypes = [vp, vp, ctypes.c_double, vp, vp, vp]
     L.piafs_write_binary.argtypes = [vp, cp]
     L.piafs_conservation_errors.argtypes = [vp, vp]
+    L.piafs_set_derivative_exchange.argtypes = [vp, ip]
     L.piafs_solve.argtypes = [vp, cp, ip, dp]
     L.piafs_kernel_times.argtypes = [vp, ip, ctypes.POINTER(cp), dp, ctypes.POINTER(ctypes.c_int)]
     L.piafs_tridiag_host.argtypes = [vp, vp, vp, vp, ip, ip]
@@ -247,6 +249,12 @@ class Case:
 
     def write_binary(self, path: str) -> None:
         self._check(self._L.piafs_write_binary(self._h, path.encode()))
+
+    def set_derivative_exchange(self, on: bool) -> None:
+        """Derivative ghosts of the viscous term as a run decomposed along the
+        periodic dimensions computes them (on) or as a serial run (off, the
+        default); piafs_set_derivative_exchange."""
+        self._check(self._L.piafs_set_derivative_exchange(self._h, int(bool(on))))
 
     def conservation_errors(self) -> np.ndarray:
         """Per-variable conservation error after the steps taken so far

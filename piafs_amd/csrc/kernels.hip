@@ -2055,7 +2055,11 @@ __device__ __forceinline__ void deriv_prim(long plane, int g, double gam, const 
 }
 
 // Qd planes: 0 ux, 1 vx, 2 Tx (d/dx), 3 uy, 4 vy, 5 Ty (d/dy), each scaled by
-// dxinv (NavierStokes2DParabolicFunction.c:96-106).
+// dxinv (NavierStokes2DParabolicFunction.c:96-106).  With the derivative
+// exchange along a periodic dimension (xchg, MPIExchangeBoundariesnD.c:
+// 109-167 as called at NavierStokes2DParabolicFunction.c:91-94), a ghost point
+// of an interior line across that dimension takes the unscaled derivatives of
+// its periodic image, scaled by its own dxinv; corners are not exchanged.
 __global__ void __launch_bounds__(256) k_qderiv(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
@@ -2065,17 +2069,21 @@ __global__ void __launch_bounds__(256) k_qderiv(CASE_ARGS) {
   const long pl = (long)G.nxg * G.nyg;
   const double gam = C.P.gamma;
   double dx[3] = {0, 0, 0}, dy[3] = {0, 0, 0};
-  if (j >= 0 && j < G.ny) {
-    const long p0 = (long)(j + G.g) * G.nxg + G.g;  // (0, j)
-    if (C.P.par4) deriv_prim<1>(G.plane, G.g, gam, C.U, p0, 1, i, G.nx, dx);
-    else deriv_prim<0>(G.plane, G.g, gam, C.U, p0, 1, i, G.nx, dx);
+  const bool iin = i >= 0 && i < G.nx, jin = j >= 0 && j < G.ny;
+  int si = i, sj = j;  // the point whose derivatives this point holds
+  if (C.P.xchg[0] && jin && !iin) si = i < 0 ? i + G.nx : i - G.nx;
+  if (C.P.xchg[1] && iin && !jin) sj = j < 0 ? j + G.ny : j - G.ny;
+  if (sj >= 0 && sj < G.ny) {
+    const long p0 = (long)(sj + G.g) * G.nxg + G.g;  // (0, sj)
+    if (C.P.par4) deriv_prim<1>(G.plane, G.g, gam, C.U, p0, 1, si, G.nx, dx);
+    else deriv_prim<0>(G.plane, G.g, gam, C.U, p0, 1, si, G.nx, dx);
     const double s = C.dxinv[G.g + i];
     dx[0] *= s; dx[1] *= s; dx[2] *= s;
   }
-  if (i >= 0 && i < G.nx) {
-    const long p0 = (long)G.g * G.nxg + (i + G.g);  // (i, 0)
-    if (C.P.par4) deriv_prim<1>(G.plane, G.g, gam, C.U, p0, G.nxg, j, G.ny, dy);
-    else deriv_prim<0>(G.plane, G.g, gam, C.U, p0, G.nxg, j, G.ny, dy);
+  if (si >= 0 && si < G.nx) {
+    const long p0 = (long)G.g * G.nxg + (si + G.g);  // (si, 0)
+    if (C.P.par4) deriv_prim<1>(G.plane, G.g, gam, C.U, p0, G.nxg, sj, G.ny, dy);
+    else deriv_prim<0>(G.plane, G.g, gam, C.U, p0, G.nxg, sj, G.ny, dy);
     const double s = C.dxinv[(G.nx + 2 * G.g) + G.g + j];
     dy[0] *= s; dy[1] *= s; dy[2] *= s;
   }

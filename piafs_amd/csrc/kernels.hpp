@@ -74,6 +74,10 @@ struct PhysParams {
   double inv_Re, inv_Pr, inv_gm1, Tref, T0, TS, TA, TB, two_third;
   double kappa_num;   // T0 + TA*exp(-TB/T0), constant numerator factor
   int viscous, par4;
+  // derivative ghosts exchanged along dimension d (0: x, 1: y): the MPI
+  // exchange of QDerivX / QDerivY that a run with more than one rank along a
+  // periodic dimension performs (a no-op in a serial run)
+  int xchg[2];
   // chemistry
   int chem, nflow, nlayers, z_stride;
   double k0a, k0b, k1a, k1b, k2a, k2b, k3a, k3b, k4, k5, k6;

@@ -926,6 +926,20 @@ int piafs_write_binary(piafs_case *pc, const char *path) {
   return rc ? case_fail(pc, rc) : 0;
 }
 
+int piafs_set_derivative_exchange(piafs_case *pc, int on) {
+  if (!pc || !pc->c) return 1;  // no case
+  Case *c = pc->c;
+  for (int d = 0; d < 2; d++) {
+    bool periodic = false;
+    for (const Boundary &b : c->cfg.bcs)
+      periodic = periodic || (b.bctype == BC_PERIODIC && b.dim == d && d < c->cfg.ndims);
+    // only the viscous term of navierstokes2d exchanges derivative ghosts
+    c->h.P.xchg[d] = (on && periodic && c->cfg.model_id == MODEL_NS2D) ? 1 : 0;
+  }
+  int rc = pc->eng.build();  // the device descriptor table and a fresh step graph
+  return rc ? case_fail(pc, rc) : 0;
+}
+
 int piafs_conservation_errors(piafs_case *pc, double *errors) {
   if (!pc || !pc->c) return 1;  // no case
   if (!errors) return null_arg(pc, "output");

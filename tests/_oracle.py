@@ -35,6 +35,7 @@ def lib():
         L.orc_tridiag.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_int]
         L.orc_weno_weights.argtypes = [ctypes.c_int] + [ctypes.c_double] * 9 + [vp]
         L.orc_fd4_line.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
+        L.orc_set_derivative_exchange.argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -97,6 +98,9 @@ class OCase:
         self.L.orc_rhs_parts(self.h, _p(u), float(t), *[ptr(out[k]) for k in
                              ("u_ghosted", "flux0", "flux1", "fi0", "fi1", "hyp", "par", "src")])
         return out
+
+    def set_derivative_exchange(self, on):
+        self.L.orc_set_derivative_exchange(self.h, int(bool(on)))
 
     def param(self, name):
         return self.L.orc_param(self.h, name.encode())

@@ -386,3 +386,38 @@ def test_line_solve_relay_from_plain_start(ref_runs, name, monkeypatch):
             slow = c.solution()
     assert np.array_equal(fast, slow)
     assert np.all(_rel(slow, data[name]) <= TOL_STEPS)
+
+
+def test_derivative_exchange_translation_invariant_and_matches_oracle():
+    """The derivative-ghost exchange of a run decomposed along the periodic
+    dimensions (piafs_set_derivative_exchange): the right-hand side of a
+    shifted state is the shifted right-hand side bit for bit, it matches the
+    oracle's exchange to 1e-12, and steps match the oracle's to 1e-11; switched
+    off, the serial semantics come back."""
+    from test_oracle_pinning import _roll, translation_deck
+    dk = translation_deck()
+    rng = np.random.default_rng(23)
+    u = dk.u * (1.0 + 1e-2 * rng.uniform(-1, 1, dk.u.shape))
+    us = _roll(u, dk, 3, 5)
+    n = int(dk.solver["n_iter"])
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        o = orc.OCase(d)
+        serial_ref = o.rhs(u, 0.0)
+        o.set_derivative_exchange(True)
+        ref = o.rhs(u, 0.0)
+        o.step(n)
+        ref_steps = o.solution()
+        with piafs_amd.Case(d) as c:
+            c.set_derivative_exchange(True)
+            got, got_s = c.rhs(u, 0.0), c.rhs(us, 0.0)
+            c.step(n)
+            steps = c.solution()
+            c.set_derivative_exchange(False)
+            serial = c.rhs(u, 0.0)
+    assert np.array_equal(_roll(got, dk, 3, 5), got_s)
+    scale = np.abs(ref).max(axis=0) + 1e-300
+    assert np.all(np.abs(got - ref).max(axis=0) / scale <= TOL_RHS)
+    assert np.all(np.abs(serial - serial_ref).max(axis=0) / scale <= TOL_RHS)
+    assert not np.array_equal(serial, got)
+    assert np.all(_rel(steps, ref_steps) <= TOL_STEPS)

@@ -318,3 +318,52 @@ def test_reflect_ghosts_are_negated_mirror(which):
     for cj in (slice(0, g), slice(g + ny, None)):
         for ci in (slice(0, g), slice(g + nx, None)):
             assert not ug[cj, ci].any()
+
+
+PERIODIC_2D = [("periodic", 0, 1, 0, 0, -1e99, 1e99), ("periodic", 0, -1, 0, 0, -1e99, 1e99),
+               ("periodic", 1, 1, -1e99, 1e99, 0, 0), ("periodic", 1, -1, -1e99, 1e99, 0, 0)]
+
+
+def translation_deck():
+    """A viscous periodic 2D box on a grid of spacing 1/8 (every dxinv, ghosts
+    included, is exactly 8, so a translation of the state is a symmetry of the
+    discrete operator), WENO5 components (its interface values are pointwise
+    in the stencil values; the compact scheme's end rows are not), no source."""
+    dk = decks.ns2d_open_box(nx=20, ny=16, n_iter=6, dt=0.002, L=19.0 / 8.0, flow=(1.0, 0.3, 0.2, None),
+                             Re=200.0, scheme="weno5", boundary=PERIODIC_2D)
+    assert all(np.all(np.diff(xd) == 0.125) for xd in dk.x)
+    return dk
+
+
+def _roll(u, dk, sy, sx):
+    nx, ny = dk.solver["size"]
+    return np.roll(u.reshape(ny, nx, -1), (sy, sx), axis=(0, 1)).reshape(u.shape)
+
+
+def test_derivative_exchange_is_translation_invariant():
+    """With the derivative ghosts exchanged as a run decomposed along the
+    periodic dimensions does (MPIExchangeBoundariesnD.c:45-176 at
+    NavierStokes2DParabolicFunction.c:91-94), the right-hand side of a
+    periodically shifted state is the shifted right-hand side, bit for bit.
+    With the serial semantics (one-sided derivatives at the ghost points, the
+    other derivative zero there) it is not, near the edges only: the two
+    differ only within two points of the boundary, where the viscous stencil
+    reaches the ghost layers."""
+    dk = translation_deck()
+    rng = np.random.default_rng(23)
+    u = dk.u * (1.0 + 1e-2 * rng.uniform(-1, 1, dk.u.shape))
+    us = _roll(u, dk, 3, 5)
+    nx, ny = dk.solver["size"]
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        o = orc.OCase(d)
+        serial, serial_s = o.rhs(u, 0.0), o.rhs(us, 0.0)
+        o.set_derivative_exchange(True)
+        xchg, xchg_s = o.rhs(u, 0.0), o.rhs(us, 0.0)
+    assert np.array_equal(_roll(xchg, dk, 3, 5), xchg_s)
+    assert not np.array_equal(_roll(serial, dk, 3, 5), serial_s)
+    a, b = serial.reshape(ny, nx, -1), xchg.reshape(ny, nx, -1)
+    assert np.array_equal(a[2:-2, 2:-2], b[2:-2, 2:-2])
+    assert not np.array_equal(a, b)
+    # density has no viscous term: the exchange leaves it alone everywhere
+    assert np.array_equal(a[..., 0], b[..., 0])
