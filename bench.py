@@ -48,7 +48,7 @@ def kernel_algorithmic_bytes(name, nx, ny, nv, g, nstages, crweno, paired=True):
     pos = N + 2 * g
     sweeps = 2
     nio = 2 if paired else 4
-    if name == "recon":   # u and f at every position of every line; the interface arrays
+    if name in ("recon", "recon_full"):  # u and f at every position of every line; the interface arrays
         return sweeps * 8 * (2 * nv * pos * lines + nio * nv * (N + 1) * lines)
     if name == "upwind":  # the interface arrays + 4 flow vars at the two cells; fI
         return sweeps * 8 * (nio * nv * (N + 1) * lines + 2 * 4 * (N + 1) * lines + nv * (N + 1) * lines)
@@ -423,7 +423,7 @@ def main():
         if ktimes:
             # per-class average launch time (ms) x launches per step -> share of the step
             ns = inf.nstages
-            per_step = {"stage_bc": ns, "ghost": ns, "bflux": ns, "flux": ns, "recon_ends": ns, "recon": ns, "recon_char": ns, "upwind": ns, "qderiv": ns,
+            per_step = {"stage_bc": ns, "ghost": ns, "bflux": ns, "flux": ns, "recon_ends": ns, "recon": ns, "recon_full": ns, "recon_char": ns, "upwind": ns, "qderiv": ns,
                         "viscflux": ns, "assemble": ns, "update": 1}
             share = {k: v * per_step.get(k, 1) for k, v in ktimes.items()}
             dom = max(share.items(), key=lambda kv: kv[1])[0]
@@ -437,7 +437,7 @@ def main():
             ach = byts / (ms * 1e-3) / 1e9
             line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": ach / HBM_PEAK_GBS,
-                                "traffic": measured_traffic(nx) if dom == "recon" else None,
+                                "traffic": measured_traffic(nx) if dom in ("recon", "recon_full") else None,
                                 "kernel": dom, "kernel_ms": ms, "algorithmic_bytes_per_launch": byts,
                                 "algorithmic_bytes_rule": "40 * nvars B per cell-update and stage",
                                 "kernel_io_bytes_per_launch": io_byts,

@@ -113,6 +113,62 @@ int  piafs_interface_flux(piafs_case *c, const double *u_host, double t, int dir
  * src/HyParFunctions/SourceFunction.c:21-60).  rhs = -hyp + par + src. */
 int  piafs_rhs_terms(piafs_case *c, const double *u_host, double t, double *hyp, double *par, double *src);
 
+/* ---------------------------------------------------------------------------
+ * Device-buffer per-operator entry points: the same pieces on arrays the
+ * caller keeps on the device in the reference's own layouts, enqueued on
+ * `stream` (the caller's HIP stream handle; NULL = the case's own stream) without host
+ * synchronisation.  Ghosted arrays as above; the interface array of
+ * dimension dir holds the points of the grid with dim[dir]+1 interfaces along
+ * dir and no ghosts, x fastest, nvars contiguous (the fI, uL, uR, fL, fR of
+ * src/HyParFunctions/HyperbolicFunction.c:61-70).  These calls use the case's
+ * own device buffers as scratch: calls on one case must not overlap each
+ * other or its steps (the reference's function table is not reentrant
+ * either).  The case's solution and time are left as they are.  A zero pivot
+ * in a line solve shows as error -3 of the next piafs_sync.
+ *
+ * ApplyBoundaryConditions (src/HyParFunctions/ApplyBoundaryConditions.c:
+ * 31-54): fills the ghost points of u_ghosted in place from its interior
+ * (corners are not written). */
+int  piafs_dev_boundary_fill(piafs_case *c, double *u_ghosted, double t, void *stream);
+/* FFunction (NavierStokes2DFlux.c:24-52, Euler1DFlux.c:19): the flux along
+ * dir at every point of the ghosted grid. */
+int  piafs_dev_flux(piafs_case *c, const double *u_ghosted, int dir, double *f_ghosted, void *stream);
+/* InterpolateInterfacesHyp (HyperbolicFunction.c:204-207 calls it four
+ * times per dimension; Interp1PrimFifthOrderWENO.c:77-171,
+ * Interp1PrimFifthOrderCRWENO.c:81-240 and their characteristic forms): the
+ * interface values fI of fC along dir, left-biased for upw = 1, right-biased
+ * for upw = -1, on every interior line, with the deck's scheme and weights.
+ * As ReconstructHyperbolic computes the weights (SetInterpLimiterVar on the
+ * flux and the solution), they are those of fC itself: uflag = 1 with fC =
+ * u_ghosted interpolates the solution, uflag = 0 with fC the flux of
+ * u_ghosted along dir the flux; characteristic interpolation takes its
+ * eigenvectors from u_ghosted. */
+int  piafs_dev_interp_interfaces(piafs_case *c, double *fI, const double *fC, const double *u_ghosted, int upw,
+                                 int dir, int uflag, void *stream);
+/* Upwind (HyPar::Upwind, e.g. NavierStokes2DUpwindRoe
+ * NavierStokes2DUpwind.c:26-92, Euler1DUpwindRoe Euler1DUpwind.c:27-93, and the
+ * rf-char / llf-char / rusanov forms): the upwind interface flux fI along dir
+ * from the four interface arrays and the cell-centred ghosted state. */
+int  piafs_dev_upwind(piafs_case *c, double *fI, const double *fL, const double *fR, const double *uL,
+                      const double *uR, const double *u_ghosted, int dir, double t, void *stream);
+/* FirstDerivativePar (FirstDerivativeFourthOrder.c:39-132 for
+ * par_space_scheme 4; FirstDerivativeFirstOrder.c:20-95 for "2", as
+ * InitializeSolvers.c:69-72 selects it): Df = d f / d x_dir (unscaled) at every
+ * position, ghosts included, of every interior line across dir; other points
+ * of Df are not written. */
+int  piafs_dev_first_derivative_par(piafs_case *c, double *Df, const double *f_ghosted, int dir, int bias,
+                                    void *stream);
+/* SetInterpLimiterVar = WENOFifthOrderCalculateWeights (src/
+ * InterpolationFunctions/WENOFifthOrderCalculateWeights.c:41-84 with the JS,
+ * mapped, Z and YC forms and their characteristic variants): the nonlinear
+ * weights of every interface of dir on every interior line, four sets
+ * (left-biased flux, left-biased solution, right-biased flux, right-biased
+ * solution) at w + k*size (k = 0..3), each set an interface array of dir.
+ * Pass the reference's weno->w1 + weno->offset[dir] (and w2, w3) with
+ * size = weno->size to fill its arrays.  Not for upw5 (no weights). */
+int  piafs_dev_weno_weights(piafs_case *c, const double *fC, const double *u_ghosted, int dir, double *w1,
+                            double *w2, double *w3, long size, void *stream);
+
 /* Write the current solution in the reference's binary layout.  Replaces
  * WriteBinary (src/IOFunctions/WriteBinary.c:37-93) as called from
  * OutputSolution (src/Simulation/OutputSolution.cpp:25). */

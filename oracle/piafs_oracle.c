@@ -77,6 +77,15 @@ struct orc_case {
   /* optional capture of the interface fluxes of every line, per dimension
    * ([line][interface][var]), set by orc_rhs_parts */
   double *fi_cap[2];
+  /* optional capture, for dimension cap_dir (-1: none), of the interface
+   * values uL, uR, fL, fR of every line ([line][interface][var]) and of the
+   * nonlinear weights they were made with (four sets LF, LU, RF, RU at
+   * w + k*w_size, interface point p, variable v at p*nvars + v), set by
+   * orc_rhs_interfaces; wcap_p0 + k*wcap_pk is the interface point of
+   * interface k of the line being reconstructed */
+  double *if_cap[4], *w_cap[3];
+  long   w_size, wcap_p0, wcap_pk;
+  int    cap_dir, wcap_on;
   /* derivative ghosts of the parabolic term exchanged along periodic
    * dimensions, as a run with more than one rank along them does
    * (orc_set_derivative_exchange) */
@@ -447,6 +456,7 @@ int orc_load(const char *dir, orc_case **out)
   int d, i, found, s;
   char include_chem[MAXS] = "no";
   *out = NULL;
+  c->cap_dir = -1;
   if (read_solver(c, dir)) { orc_free(c); return 1; }
   if (c->ndims < 1 || c->ndims > 2 || c->ghosts < 3) { orc_free(c); return 2; }
   /* sizes (Initialize.c:121-175) */
@@ -982,6 +992,20 @@ static void optimal_weights(const orc_case *c, int k, int N, double *cw)
   else { cw[0] = 0.1; cw[1] = 0.6; cw[2] = 0.3; }
 }
 
+/* record the weights of interface k, variable v of reconstruction `which`
+ * (0 uL, 1 uR, 2 fL, 3 fR) while a capture is on: the set order of
+ * WENOFifthOrderCalculateWeights.c:144-155 is LF, LU, RF, RU */
+static void cap_weights(const orc_case *c, int which, int k, int v, const double *w)
+{
+  static const int set_of[4] = {1, 3, 0, 2};
+  size_t o;
+  if (!c->w_cap[0] || !c->wcap_on) return;
+  o = (size_t)set_of[which] * c->w_size + (size_t)(c->wcap_p0 + k * c->wcap_pk) * c->nvars + v;
+  c->w_cap[0][o] = w[0];
+  c->w_cap[1][o] = w[1];
+  c->w_cap[2][o] = w[2];
+}
+
 /* ------------------------------------------------------------------ */
 /* tridiagonal solver: tridiagLU.c:86-277 (serial stages 1 and 4)      */
 /* ------------------------------------------------------------------ */
@@ -1054,6 +1078,7 @@ static void recon_line_components(const orc_case *c, const double *fC, const dou
         optimal_weights(c, k, N, cw);
         if (c->no_limiting) { w[0] = cw[0]; w[1] = cw[1]; w[2] = cw[2]; }
         else orc_weno_weights(kind, cw[0], cw[1], cw[2], m3, m2, m1, p1, p2, c->eps, w);
+        cap_weights(c, which, k, v, w);
         if (!c->is_crweno) {
           f1 = (2 * s6) * m3 + (-7 * s6) * m2 + (11 * s6) * m1;
           f2 = (-s6) * m2 + (5 * s6) * m1 + (2 * s6) * p1;
@@ -1132,6 +1157,7 @@ static void recon_line_char(const orc_case *c, const double *fC, const double *u
         optimal_weights(c, k, N, cw);
         if (c->no_limiting) { w[0] = cw[0]; w[1] = cw[1]; w[2] = cw[2]; }
         else orc_weno_weights(kind, cw[0], cw[1], cw[2], s[0], s[1], s[2], s[3], s[4], c->eps, w);
+        cap_weights(c, which, k, v, w);
         f1 = (2 * s6) * s[0] - (7.0 * s6) * s[1] + (11.0 * s6) * s[2];
         f2 = (-s6) * s[1] + (5.0 * s6) * s[2] + (2 * s6) * s[3];
         f3 = (2 * s6) * s[2] + (5 * s6) * s[3] - (s6) * s[4];
@@ -1273,6 +1299,7 @@ static void recon_line_crweno_char(const orc_case *c, const double *fC, const do
         optimal_weights(c, k, N, cw);
         if (c->no_limiting) { w[0] = cw[0]; w[1] = cw[1]; w[2] = cw[2]; }
         else orc_weno_weights(kind, cw[0], cw[1], cw[2], s[0], s[1], s[2], s[3], s[4], c->eps, w);
+        cap_weights(c, which, k, v, w);
         if (bnd) {
           f1 = (2 * s6) * s[0] - (7.0 * s6) * s[1] + (11.0 * s6) * s[2];
           f2 = (-s6) * s[1] + (5.0 * s6) * s[2] + (2 * s6) * s[3];
@@ -1487,10 +1514,21 @@ static void hyperbolic(orc_case *c, const double *u, double *hyp)
       if (other >= 0) idx[other] = l;
       idx[d] = 0;
       base = gidx(c, idx);
+      c->wcap_on = (d == c->cap_dir);
+      if (d == c->cap_dir) {
+        c->wcap_p0 = (c->ndims == 1) ? 0 : (d == 0 ? (long)l * (N + 1) : l);
+        c->wcap_pk = (c->ndims == 1 || d == 0) ? 1 : c->dim[0];
+      }
       if (c->is_char && c->is_crweno)
         recon_line_crweno_char(c, c->fluxC, u, base, st, N, d, uLl, uRl, fLl, fRl, (size_t)nv);
       else if (c->is_char) recon_line_char(c, c->fluxC, u, base, st, N, d, uLl, uRl, fLl, fRl, (size_t)nv);
       else recon_line_components(c, c->fluxC, u, base, st, N, uLl, uRl, fLl, fRl, (size_t)nv);
+      if (d == c->cap_dir) {
+        double *src4[4] = {uLl, uRl, fLl, fRl};
+        int r;
+        for (r = 0; r < 4; r++)
+          if (c->if_cap[r]) memcpy(c->if_cap[r] + (size_t)l * (N + 1) * nv, src4[r], (size_t)(N + 1) * nv * sizeof(double));
+      }
       upwind_line(c, u, base, st, N, d, uLl, uRl, fLl, fRl, fIl, (size_t)nv);
       if (c->fi_cap[d]) memcpy(c->fi_cap[d] + (size_t)l * (N + 1) * nv, fIl, (size_t)(N + 1) * nv * sizeof(double));
       /* hyp += dxinv * (fI[k+1] - fI[k]) at the N interior points of the line */
@@ -1502,6 +1540,7 @@ static void hyperbolic(orc_case *c, const double *u, double *hyp)
     }
     xoff += N + 2 * g;
   }
+  c->wcap_on = 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1867,6 +1906,43 @@ int orc_rhs_parts(orc_case *c, const double *u_in, double t, double *u_ghosted, 
   if (src) interior_copy(c, src, c->src_a, 1);
   free(w);
   return 0;
+}
+
+int orc_rhs_interfaces(orc_case *c, const double *u_in, double t, int dir, double *uL, double *uR, double *fL,
+                       double *fR, double *w1, double *w2, double *w3)
+{
+  size_t n = (size_t)c->npts * c->nvars;
+  int N, nl, r;
+  double *w;
+  if (dir < 0 || dir >= c->ndims) return 1;
+  N = c->dim[dir];
+  nl = (c->ndims == 2) ? c->dim[1 - dir] : 1;
+  w = (double *)calloc(n, sizeof(double));
+  interior_copy(c, w, u_in, 0);
+  apply_bcs(c, w);
+  c->if_cap[0] = uL; c->if_cap[1] = uR; c->if_cap[2] = fL; c->if_cap[3] = fR;
+  c->w_cap[0] = w1; c->w_cap[1] = w2; c->w_cap[2] = w3;
+  c->w_size = (long)(N + 1) * nl * c->nvars;
+  c->cap_dir = dir;
+  if (c->is_upw5) c->w_cap[0] = c->w_cap[1] = c->w_cap[2] = NULL;  /* upw5 has no weights */
+  hyperbolic(c, w, c->hyp_a);
+  c->cap_dir = -1;
+  for (r = 0; r < 4; r++) c->if_cap[r] = NULL;
+  c->w_cap[0] = c->w_cap[1] = c->w_cap[2] = NULL;
+  free(w);
+  return 0;
+}
+
+void orc_first_derivative(const orc_case *c, const double *f, int dir, int bias, double *Df)
+{
+  int g = c->ghosts, nv = c->nvars, l, nl = (c->ndims == 2) ? c->dim[1 - dir] : 1;
+  for (l = 0; l < nl; l++) {
+    int idx[3] = {0, 0, 0}, base;
+    if (c->ndims == 2) idx[1 - dir] = l;
+    base = gidx(c, idx);
+    if (c->par4) fd4_line(f, Df, base, c->stride[dir], c->dim[dir], g, nv);
+    else fd1_line(f, Df, base, c->stride[dir], c->dim[dir], g, nv, bias);
+  }
 }
 
 int orc_rhs(orc_case *c, const double *u_in, double t, double *rhs_out)

@@ -69,6 +69,25 @@ int  orc_rhs(orc_case *c, const double *u_in, double t, double *rhs_out);
 int  orc_rhs_parts(orc_case *c, const double *u_in, double t, double *u_ghosted, double *flux0, double *flux1,
                    double *fi0, double *fi1, double *hyp, double *par, double *src);
 
+/* The interface values of dimension dir in the same evaluation, as
+ * InterpolateInterfacesHyp writes them for ReconstructHyperbolic
+ * (HyperbolicFunction.c:204-207): uL, uR, fL, fR of every interior line,
+ * [line][interface][var]; and the nonlinear weights they were made with
+ * (SetInterpLimiterVar, WENOFifthOrderCalculateWeights.c:112-237 and the
+ * M / Z / YC / characteristic forms), four sets LF, LU, RF, RU at w + k*size
+ * with size = (dim[dir]+1)*lines*nvars, each in the reference's interface
+ * order (interface points x fastest, nvars contiguous).  Any output may be
+ * NULL; upw5 writes no weights. */
+int  orc_rhs_interfaces(orc_case *c, const double *u_in, double t, int dir, double *uL, double *uR, double *fL,
+                        double *fR, double *w1, double *w2, double *w3);
+
+/* FirstDerivativePar on a ghosted array: FirstDerivativeFourthOrderCentral
+ * (FirstDerivativeFourthOrder.c:39-132) for par_space_scheme 4, else
+ * FirstDerivativeFirstOrder (FirstDerivativeFirstOrder.c:20-95) with `bias`,
+ * at every position (ghosts included) of the interior lines across dir;
+ * other points of Df are left as they are. */
+void orc_first_derivative(const orc_case *c, const double *f, int dir, int bias, double *Df);
+
 /* Derivative ghosts of the viscous term as a run decomposed over more than
  * one rank along each periodic dimension computes them: the MPI exchange of
  * QDerivX / QDerivY (NavierStokes2DParabolicFunction.c:91-94,

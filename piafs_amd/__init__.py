@@ -63,7 +63,8 @@ EXPORTS = [
     "piafs_server_poll", "piafs_server_result", "piafs_server_release", "piafs_server_wait_all",
     "piafs_server_stats", "piafs_server_last_error", "piafs_boundary_fill", "piafs_flux",
     "piafs_interface_flux", "piafs_rhs_terms", "piafs_conservation_errors",
-    "piafs_set_derivative_exchange",
+    "piafs_set_derivative_exchange", "piafs_dev_boundary_fill", "piafs_dev_flux", "piafs_dev_interp_interfaces",
+    "piafs_dev_upwind", "piafs_dev_first_derivative_par", "piafs_dev_weno_weights",
 ]
 
 
@@ -95,6 +96,12 @@ def lib():
     L.piafs_write_binary.argtypes = [vp, cp]
     L.piafs_conservation_errors.argtypes = [vp, vp]
     L.piafs_set_derivative_exchange.argtypes = [vp, ip]
+    L.piafs_dev_boundary_fill.argtypes = [vp, vp, ctypes.c_double, vp]
+    L.piafs_dev_flux.argtypes = [vp, vp, ip, vp, vp]
+    L.piafs_dev_interp_interfaces.argtypes = [vp, vp, vp, vp, ip, ip, ip, vp]
+    L.piafs_dev_upwind.argtypes = [vp, vp, vp, vp, vp, vp, vp, ip, ctypes.c_double, vp]
+    L.piafs_dev_first_derivative_par.argtypes = [vp, vp, vp, ip, ip, vp]
+    L.piafs_dev_weno_weights.argtypes = [vp, vp, vp, ip, vp, vp, vp, ctypes.c_long, vp]
     L.piafs_solve.argtypes = [vp, cp, ip, dp]
     L.piafs_kernel_times.argtypes = [vp, ip, ctypes.POINTER(cp), dp, ctypes.POINTER(ctypes.c_int)]
     L.piafs_tridiag_host.argtypes = [vp, vp, vp, vp, ip, ip]
@@ -255,6 +262,33 @@ class Case:
         periodic dimensions computes them (on) or as a serial run (off, the
         default); piafs_set_derivative_exchange."""
         self._check(self._L.piafs_set_derivative_exchange(self._h, int(bool(on))))
+
+    # -- device-buffer operators: arguments are device addresses (ints) of
+    # arrays in the reference's layouts; stream is a hipStream_t (0: the
+    # case's own stream); the work is enqueued, not waited for --
+    def dev_boundary_fill(self, u_ghosted: int, t: float = 0.0, stream: int = 0) -> None:
+        self._check(self._L.piafs_dev_boundary_fill(self._h, u_ghosted, float(t), stream or None))
+
+    def dev_flux(self, u_ghosted: int, dir: int, f_ghosted: int, stream: int = 0) -> None:
+        self._check(self._L.piafs_dev_flux(self._h, u_ghosted, int(dir), f_ghosted, stream or None))
+
+    def dev_interp_interfaces(self, fI: int, fC: int, u_ghosted: int, upw: int, dir: int, uflag: int,
+                              stream: int = 0) -> None:
+        self._check(self._L.piafs_dev_interp_interfaces(self._h, fI, fC, u_ghosted, int(upw), int(dir), int(uflag),
+                                                        stream or None))
+
+    def dev_upwind(self, fI: int, fL: int, fR: int, uL: int, uR: int, u_ghosted: int, dir: int, t: float = 0.0,
+                   stream: int = 0) -> None:
+        self._check(self._L.piafs_dev_upwind(self._h, fI, fL, fR, uL, uR, u_ghosted, int(dir), float(t),
+                                             stream or None))
+
+    def dev_first_derivative_par(self, Df: int, f: int, dir: int, bias: int = 1, stream: int = 0) -> None:
+        self._check(self._L.piafs_dev_first_derivative_par(self._h, Df, f, int(dir), int(bias), stream or None))
+
+    def dev_weno_weights(self, fC: int, u_ghosted: int, dir: int, w1: int, w2: int, w3: int, size: int,
+                         stream: int = 0) -> None:
+        self._check(self._L.piafs_dev_weno_weights(self._h, fC, u_ghosted, int(dir), w1, w2, w3, int(size),
+                                                   stream or None))
 
     def conservation_errors(self) -> np.ndarray:
         """Per-variable conservation error after the steps taken so far

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -81,12 +82,13 @@ struct Engine {
   int init(int dev);
   int add(Case *c);
   bool built = false;          // plan and descriptor table valid for the member set
+  const CaseDev *override_desc = nullptr;  // one-member engine: descriptor to use instead of the case's
   int build();                 // descriptor table and plans for the current member set
   int prepare();               // capture the step graph now (when graphs are used)
   void enqueue_step();         // eager launches of one full step
   int step(int n);             // n steps (graph replay when enabled)
   int stage_only(int s, int last = K_NCLASS);  // one stage, or its classes up to `last`
-  double time_class(int cls, int reps);
+  double time_class(int cls, int reps, int cls2 = -1);
   int autotune(const int *cands, int ncand, int reps, int *best_rows, double *best_ms);
   int sync();
   void mark(int cls, bool begin, hipStream_t st);
@@ -94,7 +96,25 @@ struct Engine {
   void release_plan();
 };
 
+// The device-buffer operators of one case (devops.cpp): a one-member engine
+// whose descriptor is the case's in split mode (uL, uR, fL, fR kept apart),
+// with a state array of its own for the staged interior state.
+struct UnitOps {
+  Engine eng;
+  CaseDev desc{};
+  double *u = nullptr;                         // staged state, planar and ghosted
+  double *io[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // fL, fR of each sweep when the case is paired
+};
+
 }  // namespace piafs
+
+// C ABI handle of a single case: an engine with one member.
+struct piafs_case {
+  piafs::Engine eng;
+  piafs::Case *c = nullptr;
+  std::string err;
+  std::unique_ptr<piafs::UnitOps> unit;  // built at the first device-buffer operator call
+};
 
 namespace piafs {
 void fill_info(const Case *c, piafs_info *info);

@@ -8,6 +8,7 @@
 #include "kernels.hpp"
 
 #include <cstdio>
+#include <type_traits>
 
 namespace piafs {
 
@@ -526,6 +527,39 @@ __global__ void __launch_bounds__(256) k_ghost(CASE_ARGS) {
 // NavierStokes2DFlux.c:24-52 (navierstokes2d.h:70-110), Euler1DFlux.c:19
 // (euler1d.h:61-85).
 // ---------------------------------------------------------------------------
+// The flux along dir at one point (NavierStokes2DFlux.c:24-52 with
+// navierstokes2d.h:70-110; Euler1DFlux.c:19 with euler1d.h:61-85): q(m) reads
+// variable m of the point, put(m, value) writes component m of the flux.
+template <class Q, class Put>
+__device__ __forceinline__ void point_flux(int model, int dir, double gamma, int nv, Q q, Put put) {
+  if (model == 1) {
+    const double rho = q(0), v = q(1) / rho, e = q(2);
+    const double P_ = (e - 0.5 * rho * v * v) * (gamma - 1.0);
+    put(0, rho * v);
+    put(1, rho * v * v + P_);
+    put(2, (e + P_) * v);
+    for (int m = 3; m < nv; m++) put(m, v * q(m));
+  } else {
+    const double rho = q(0);
+    const double vx = (rho == 0) ? 0 : q(1) / rho;
+    const double vy = (rho == 0) ? 0 : q(2) / rho;
+    const double e = q(3);
+    const double vsq = (vx * vx) + (vy * vy);
+    const double P_ = (e - 0.5 * rho * vsq) * (gamma - 1.0);
+    const double vn = dir == 0 ? vx : vy;
+    put(0, rho * vn);
+    if (dir == 0) {
+      put(1, rho * vx * vx + P_);
+      put(2, rho * vx * vy);
+    } else {
+      put(1, rho * vy * vx);
+      put(2, rho * vy * vy + P_);
+    }
+    put(3, (e + P_) * vn);
+    for (int m = 4; m < nv; m++) put(m, vn * q(m));
+  }
+}
+
 __global__ void __launch_bounds__(256) k_flux(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
@@ -540,34 +574,9 @@ __global__ void __launch_bounds__(256) k_flux(CASE_ARGS) {
   const int P = S.N + 2 * S.g;
   const int pos = (int)(tt % P) - S.g, l = (int)(tt / P);
   const long b = S.base + (long)l * S.line_stride + pos;
-  const double gamma = C.P.gamma;
-  const int nv = C.G.nv;
-  if (C.model == 1) {
-    const double rho = q[b], v = q[S.plane + b] / rho, e = q[2 * S.plane + b];
-    const double P_ = (e - 0.5 * rho * v * v) * (gamma - 1.0);
-    f[b] = rho * v;
-    f[S.plane + b] = rho * v * v + P_;
-    f[2 * S.plane + b] = (e + P_) * v;
-    for (int m = 3; m < nv; m++) f[m * S.plane + b] = v * q[m * S.plane + b];
-  } else {
-    const double rho = q[b];
-    const double vx = (rho == 0) ? 0 : q[S.plane + b] / rho;
-    const double vy = (rho == 0) ? 0 : q[2 * S.plane + b] / rho;
-    const double e = q[3 * S.plane + b];
-    const double vsq = (vx * vx) + (vy * vy);
-    const double P_ = (e - 0.5 * rho * vsq) * (gamma - 1.0);
-    const double vn = dir == 0 ? vx : vy;
-    f[b] = rho * vn;
-    if (dir == 0) {
-      f[S.plane + b] = rho * vx * vx + P_;
-      f[2 * S.plane + b] = rho * vx * vy;
-    } else {
-      f[S.plane + b] = rho * vy * vx;
-      f[2 * S.plane + b] = rho * vy * vy + P_;
-    }
-    f[3 * S.plane + b] = (e + P_) * vn;
-    for (int m = 4; m < nv; m++) f[m * S.plane + b] = vn * q[m * S.plane + b];
-  }
+  const long pl = S.plane;
+  point_flux(C.model, dir, C.P.gamma, C.G.nv, [&](int m) { return q[m * pl + b]; },
+             [&](int m, double x) { f[m * pl + b] = x; });
 }
 
 // ---------------------------------------------------------------------------
@@ -771,23 +780,24 @@ __device__ __forceinline__ double rcp_est(double b) {
 // (fac * c and fac * x' exactly as the reference computes them).  The back
 // substitution iterates the same way from the top.  If `plain_start` is set
 // the scanned estimates are replaced by the trivial ones (b' = 1, x' = 0), so
-// that the relay runs through many passes (testing).
-template <int MR>
+// that the relay runs through many passes (testing).  FULL: every lane holds
+// MR rows (n is not read), rows past the system's end being identity rows.
+template <int MR, bool FULL = false>
 __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], double (&Cc)[MR], double (&X)[MR],
                                                 int n, int lane, int T, bool is_last, double xfirst, double xlast,
                                                 double *red, int *status, bool plain_start) {
-  const bool has = n > 0;
+  const bool has = FULL || n > 0;
   double cin = 0.0, unused = 0.0;
 #pragma unroll
   for (int i = 0; i < MR; i++)
-    if (i == n - 1) cin = Cc[i];
+    if (FULL ? i == MR - 1 : i == n - 1) cin = Cc[i];
   team_shift2(cin, unused, lane, T, +1, red);
   if (lane == 0) cin = 0.0;
   // pivot entering the segment: scan of M_k = [[b_k, -a_k c_{k-1}], [1, 0]] on (b'_{k-1}, 1)
   Mob P = {1.0, 0.0, 0.0, 1.0};
 #pragma unroll
   for (int i = 0; i < MR; i++) {
-    if (i < n) {
+    if (FULL || i < n) {
       const double ac = A[i] * (i == 0 ? cin : Cc[i > 0 ? i - 1 : 0]);
       const Mob R = {B[i] * P.a - ac * P.c, B[i] * P.b - ac * P.d, P.a, P.b};
       P = R;
@@ -804,7 +814,7 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
     double c = cin;
 #pragma unroll
     for (int i = 0; i < MR; i++) {
-      if (i < n) {
+      if (FULL || i < n) {
         const double fac = A[i] / bend;
         bend = B[i] - fac * c;
         c = Cc[i];
@@ -822,7 +832,7 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
     double xend = xin;
 #pragma unroll
     for (int i = 0; i < MR; i++)
-      if (i < n) xend = X[i] - Fk[i] * xend;
+      if (FULL || i < n) xend = X[i] - Fk[i] * xend;
     double nb = bend;
     team_shift2(nb, xend, lane, T, +1, red);
     const bool bchg = lane != 0 && !(nb == bin);
@@ -838,7 +848,7 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
       bool live = true;
 #pragma unroll
       for (int i = 0; i < MR; i++) {
-        if (i < n && live) {
+        if ((FULL || i < n) && live) {
           const double fac = A[i] / bb;
           if (__double_as_longlong(fac) == __double_as_longlong(Fk[i])) {
             live = false;  // same multiplier, so the same pivot here and in every later row
@@ -858,7 +868,7 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
     double b = bin, x = xin, c = cin;
 #pragma unroll
     for (int i = 0; i < MR; i++) {
-      if (i < n) {
+      if (FULL || i < n) {
         b = B[i] - Fk[i] * c;
         x = X[i] - Fk[i] * x;
         c = Cc[i];
@@ -872,7 +882,7 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
   Aff Gm = {is_last ? 0.0 : 1.0, is_last ? xlast : 0.0};
 #pragma unroll
   for (int i = MR - 1; i >= 0; i--) {
-    if (i < n) {
+    if (FULL || i < n) {
       const double rb = rcp_est(B[i]);
       Gm = Aff{-(Cc[i] * rb) * Gm.a, (X[i] - Cc[i] * Gm.b) * rb};
     }
@@ -884,7 +894,7 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
     bool live = true;
 #pragma unroll
     for (int i = MR - 1; i >= 0; i--) {
-      if (i < n && live) {
+      if ((FULL || i < n) && live) {
         const double nx = (X[i] - Cc[i] * xv) / B[i];
         // from the second pass on: a row that repeats its value leaves every row above unchanged
         if (it > 0 && __double_as_longlong(nx) == __double_as_longlong(Fk[i])) live = false;
@@ -892,7 +902,7 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
         xv = nx;
       }
     }
-    if (n > 0) xv = Fk[0];
+    if (FULL || n > 0) xv = Fk[0];
     team_shift2(xv, dummy, lane, T, -1, red);
     const bool changed = has && !is_last && !(xv == yin);
     if (!is_last) yin = xv;
@@ -900,7 +910,7 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
   }
 #pragma unroll
   for (int i = 0; i < MR; i++)
-    if (i < n) X[i] = Fk[i];
+    if (FULL || i < n) X[i] = Fk[i];
   if (zp) atomicOr(status, 1);
 }
 
@@ -997,6 +1007,132 @@ __device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, 
         out[lo + i] = q ? 0.5 * (xl + X[i]) : 0.5 * (X[i] - xl);
       }
     }
+  }
+}
+
+// Paired mode with the team's line staged in LDS.  The two systems of a pair
+// read the same line (the state for pair 0, the flux for pair 1), so the team
+// loads it once, with consecutive lanes reading consecutive positions, into a
+// region of its own; each lane then reads its stencil window from there for
+// both systems.  The combinations are written back through the same region so
+// that the global stores are consecutive too.  The bias of each system is a
+// compile-time constant.  Arithmetic and results are those of recon_team.
+// Region layout: line position p (-2 .. T*m+2) at lds_slot(p + 2), one unused
+// slot after every 16 so that lanes reading 16 apart hit different banks.
+__device__ __forceinline__ int lds_slot(int i) { return i + (i >> 4); }
+__host__ __device__ inline int line_region(int T, int m) {
+  const int n = T * m + 5;
+  return n + (n >> 4) + 1;
+}
+// LDS doubles of the staged lines of one 256-thread block: the largest
+// (256 / T) * line_region(T, m) over teams T = 1 .. 256 and m <= 16 (T = 1),
+// plus the overrun of the last lane's window when its segment is short
+constexpr int kLineLds = 256 * (16 + 5 + 2) + 32;
+
+// all lanes of the team have finished their LDS accesses before any lane goes on
+__device__ __forceinline__ void team_lds_sync(int T) {
+  if (T > 64) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+}
+
+template <int CR, int KIND, bool FULL>
+__device__ __forceinline__ void recon_pair_lds(const CaseDev &C, int dir, long sys, bool active, int lane,
+                                               double *red, double *stash, double *lines) {
+  const SweepGeom &S = dir ? C.Sy : C.Sx;
+  const int nv = C.G.nv, N = S.N, nl = S.nlines, m = S.rows, g = S.g, T = S.team;
+  const long sy = active ? sys : 0;
+  const int q = (int)(sy % 2), v = (int)((sy / 2) % nv), l = (int)((sy / 2) / nv);
+  const double *src = (q == 0 ? (dir ? C.UT : C.U) : (dir ? C.flux_y : C.flux_x)) + (long)v * S.plane + S.base +
+                      (long)l * S.line_stride;
+  const long line = ((long)v * nl + l) * (N + 1);
+  const int lo = 1 + lane * m;
+  int n = N - lo;
+  n = n < 0 ? 0 : (n > m ? m : n);
+  if (!active) n = 0;
+  const int lastlane = (N - 2) / m;
+  const int hi_pos = N + g - 1;
+  const double *ends = C.ends + 4 * ((dir ? recon_systems(C.Sx, nv, 1) : 0) + sy);
+  double *L = lines + (threadIdx.x / T) * line_region(T, m);
+  // the line, positions -2 .. T*m+2 (zero past its last ghost)
+  const int tot = T * m + 5;
+  for (int b = 0; b < tot; b += T) {
+    const int idx = b + lane;
+    if (idx < tot) {
+      const int pos = idx - 2;
+      L[lds_slot(idx)] = (active && pos <= hi_pos) ? src[pos] : 0.0;
+    }
+  }
+  team_lds_sync(T);
+  double X[kMaxRows];
+  // FULL (every lane holds kMaxRows rows): the rows past row N-1 are identity
+  // rows, the first of them -- interface N -- carrying xN, so that no row
+  // needs a bound check; the identity row at interface N lies in the team
+  // unless T*kMaxRows == N-1, when it follows the team as in the general case
+  const int nlane = (N - 1) / kMaxRows;  // the lane holding row N in FULL mode
+  auto solve = [&](auto left_c, int side) {
+    constexpr bool LEFT = decltype(left_c)::value;
+    const int xn_lane = FULL ? (nlane < T ? nlane : T - 1) : lastlane;
+    // rows assembled: all kMaxRows in FULL mode (m == kMaxRows there; the
+    // row bound stays a run-time value, which keeps the row assembly in
+    // order and the register count at two waves per SIMD)
+    const int nrow = FULL ? m : n;
+    double x0 = 0.0, xN = 0.0;
+    if (CR && active && lane == 0) x0 = ends[2 * side];
+    if (CR && active && lane == xn_lane) xN = ends[2 * side + 1];
+    // stencil window: positions lo-3 .. lo+n+1
+    double win[kMaxRows + 5];
+#pragma unroll
+    for (int j = 0; j < kMaxRows + 5; j++) win[j] = (j < nrow + 5) ? L[lds_slot(lo - 1 + j)] : 0.0;
+    double A[kMaxRows], B[kMaxRows], Cc[kMaxRows];
+#pragma unroll
+    for (int i = 0; i < kMaxRows; i++) {
+      if (i < nrow) {
+        row_coeffs<CR, KIND>(lo + i, N, LEFT, C.R, win[i], win[i + 1], win[i + 2], win[i + 3], win[i + 4],
+                             win[i + 5], A[i], B[i], Cc[i], X[i]);
+      } else {
+        A[i] = 0.0; B[i] = 1.0; Cc[i] = 0.0; X[i] = 0.0;
+      }
+    }
+    if (FULL && lo + kMaxRows > N) {
+#pragma unroll
+      for (int i = 0; i < kMaxRows; i++) {
+        if (lo + i >= N) {
+          A[i] = 0.0; B[i] = 1.0; Cc[i] = 0.0;
+          X[i] = (lo + i == N) ? xN : 0.0;
+        }
+      }
+    }
+    if (CR) {
+      if (FULL)
+        team_solve<kMaxRows, true>(A, B, Cc, X, kMaxRows, lane, T, lane == T - 1, x0, nlane < T ? 0.0 : xN, red,
+                                   C.status, C.R.plain_start != 0);
+      else
+        team_solve<kMaxRows>(A, B, Cc, X, n, lane, T, lane == lastlane, x0, xN, red, C.status,
+                             C.R.plain_start != 0);
+    }
+  };
+  solve(std::integral_constant<bool, true>(), 0);
+#pragma unroll
+  for (int i = 0; i < kMaxRows; i++) stash[i * 256 + threadIdx.x] = X[i];  // this lane's own rows
+  solve(std::integral_constant<bool, false>(), 1);
+  team_lds_sync(T);  // every window of the team is read
+#pragma unroll
+  for (int i = 0; i < kMaxRows; i++) {
+    if (i < n) {
+      const double xl = stash[i * 256 + threadIdx.x];
+      L[lds_slot(lo + i)] = q ? 0.5 * (xl + X[i]) : 0.5 * (X[i] - xl);
+    }
+  }
+  team_lds_sync(T);
+  double *out = (dir ? C.iy : C.ix)[q] + line;  // [0]: 0.5*(uR - uL), [1]: 0.5*(fL + fR)
+  for (int b = 1; b < N; b += T) {
+    const int k = b + lane;
+    if (active && k < N) out[k] = L[lds_slot(k)];
   }
 }
 
@@ -1690,7 +1826,10 @@ __global__ void __launch_bounds__(256) k_recon_ends(CASE_ARGS) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_recon(CASE_ARGS) {
+// FULL: the kernel of class K_RECONF, for paired cases whose lanes all hold
+// kMaxRows rows (recon_pair_lds<..., true>); the other cases run in K_RECON.
+template <bool FULL>
+__device__ __forceinline__ void recon_body(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
   __shared__ double red[16];
@@ -1705,6 +1844,20 @@ __global__ void __launch_bounds__(256) k_recon(CASE_ARGS) {
   const long sys = (long)lb * (256 / T) + threadIdx.x / T;
   const bool active = sys < recon_systems(S, C.G.nv, C.paired);
   const int cr = C.R.crweno;
+#if PIAFS_RECON_LDS
+  if (FULL || C.paired) {
+    __shared__ double lines[kLineLds];
+    switch (C.R.kind) {
+      case 1: cr ? recon_pair_lds<1, 1, FULL>(C, dir, sys, active, lane, red, stash, lines) : recon_pair_lds<0, 1, FULL>(C, dir, sys, active, lane, red, stash, lines); break;
+      case 2: cr ? recon_pair_lds<1, 2, FULL>(C, dir, sys, active, lane, red, stash, lines) : recon_pair_lds<0, 2, FULL>(C, dir, sys, active, lane, red, stash, lines); break;
+      case 3: cr ? recon_pair_lds<1, 3, FULL>(C, dir, sys, active, lane, red, stash, lines) : recon_pair_lds<0, 3, FULL>(C, dir, sys, active, lane, red, stash, lines); break;
+      case 4: recon_pair_lds<0, 4, FULL>(C, dir, sys, active, lane, red, stash, lines); break;
+      default: cr ? recon_pair_lds<1, 0, FULL>(C, dir, sys, active, lane, red, stash, lines) : recon_pair_lds<0, 0, FULL>(C, dir, sys, active, lane, red, stash, lines); break;
+    }
+    return;
+  }
+#endif
+  if (FULL) return;  // not reached: K_RECONF holds paired cases only
   switch (C.R.kind) {
     case 1: cr ? recon_team<1, 1>(C, dir, sys, active, lane, red, stash) : recon_team<0, 1>(C, dir, sys, active, lane, red, stash); break;
     case 2: cr ? recon_team<1, 2>(C, dir, sys, active, lane, red, stash) : recon_team<0, 2>(C, dir, sys, active, lane, red, stash); break;
@@ -1713,6 +1866,8 @@ __global__ void __launch_bounds__(256) k_recon(CASE_ARGS) {
     default: cr ? recon_team<1, 0>(C, dir, sys, active, lane, red, stash) : recon_team<0, 0>(C, dir, sys, active, lane, red, stash); break;
   }
 }
+__global__ void __launch_bounds__(256) k_recon(CASE_ARGS) { recon_body<false>(cases, blockcase, firstblock, stage); }
+__global__ void __launch_bounds__(256) k_recon_full(CASE_ARGS) { recon_body<true>(cases, blockcase, firstblock, stage); }
 
 // ---------------------------------------------------------------------------
 // Upwinding.
@@ -2427,6 +2582,188 @@ __global__ void __launch_bounds__(TB) k_weno_weights(int kind, double c1, double
   w[3 * (long)t + 2] = w3;
 }
 
+// ---------------------------------------------------------------------------
+// Device-buffer operators on the reference's own array layouts, for callers
+// that keep the reference's data structures on the device.  Ghosted arrays:
+// every grid point with its ghost layers, x fastest, the nvars components of
+// a point contiguous (include/hypar.h:106-111).  Interface arrays of
+// dimension dir: the points of the grid with dim[dir]+1 interfaces along dir
+// and no ghosts, same ordering (HyperbolicFunction.c:61-70).
+// ---------------------------------------------------------------------------
+// point offset of (i, j) (interior indices; ghosts negative) in a ghosted array
+__device__ __forceinline__ long aos_point(const AosGrid &A, int i, int j) {
+  return A.ndims == 2 ? (long)(j + A.g) * (A.nx + 2 * A.g) + (i + A.g) : (long)(i + A.g);
+}
+__device__ __forceinline__ long aos_points(const AosGrid &A) {
+  return (long)(A.nx + 2 * A.g) * (A.ndims == 2 ? A.ny + 2 * A.g : 1);
+}
+
+// FluxFunction along dir at every point of the ghosted grid
+__global__ void __launch_bounds__(TB) k_dev_flux(AosGrid A, int model, double gamma, int dir,
+                                                 const double *__restrict__ u, double *__restrict__ f) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= aos_points(A)) return;
+  const double *q = u + p * A.nv;
+  double *o = f + p * A.nv;
+  point_flux(model, dir, gamma, A.nv, [&](int m) { return q[m]; }, [&](int m, double x) { o[m] = x; });
+}
+
+// FirstDerivativePar along dir at every position (ghosts included) of every
+// interior line across dir, one thread per (position, line, variable):
+// FirstDerivativeFourthOrderCentral (FirstDerivativeFourthOrder.c:39-132) with
+// par4, else FirstDerivativeFirstOrder (FirstDerivativeFirstOrder.c:20-95) with
+// the given bias.  Points off the interior lines are left as they are.
+__global__ void __launch_bounds__(TB) k_dev_fd(AosGrid A, int dir, int par4, int bias,
+                                               const double *__restrict__ f, double *__restrict__ df) {
+  const int N = dir ? A.ny : A.nx, g = A.g, nv = A.nv, P = N + 2 * g;
+  const int nl = A.ndims == 2 ? (dir ? A.nx : A.ny) : 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)P * nl * nv) return;
+  const int v = (int)(t % nv);
+  const long r = t / nv;
+  const int i = (int)(r % P) - g, l = (int)(r / P);
+  auto F = [&](int k) { return f[aos_point(A, dir ? l : k, dir ? k : l) * nv + v]; };
+  double d;
+  if (par4) {
+    const double t12 = 1.0 / 12.0;
+    if (i == -g) d = (-25 * F(i) + 48 * F(i + 1) - 36 * F(i + 2) + 16 * F(i + 3) - 3 * F(i + 4)) * t12;
+    else if (i == -g + 1) d = (-3 * F(i - 1) - 10 * F(i) + 18 * F(i + 1) - 6 * F(i + 2) + F(i + 3)) * t12;
+    else if (i == N + g - 2) d = (-F(i - 3) + 6 * F(i - 2) - 18 * F(i - 1) + 10 * F(i) + 3 * F(i + 1)) * t12;
+    else if (i == N + g - 1) d = (3 * F(i - 4) - 16 * F(i - 3) + 36 * F(i - 2) - 48 * F(i - 1) + 25 * F(i)) * t12;
+    else d = (F(i - 2) - 8 * F(i - 1) + 8 * F(i + 1) - F(i + 2)) * t12;
+  } else {
+    const double mx = bias > 0 ? bias : 0, mn = bias < 0 ? bias : 0, bb = bias;
+    if (i == -g) d = F(i + 1) - F(i);
+    else if (i == N + g - 1) d = F(i) - F(i - 1);
+    else d = mx * F(i + 1) - bb * F(i) + mn * F(i - 1);
+  }
+  df[aos_point(A, dir ? l : i, dir ? i : l) * nv + v] = d;
+}
+
+// SetInterpLimiterVar = WENOFifthOrderCalculateWeights{,Char}
+// (WENOFifthOrderCalculateWeights.c:41-84, :112-237 and the M / Z / YC and
+// characteristic variants): the nonlinear weights of every interface of dir
+// on every interior line, for the left-biased flux, left-biased solution,
+// right-biased flux and right-biased solution, in that order, set k at
+// w + k*size.  Within a set, interface point p, variable v at p*nvars + v.
+// Characteristic: the stencils are projected with the left eigenvectors of
+// the Roe average of the two cells beside the interface (the flow block;
+// species rows are the identity).  One thread per (interface, line).
+template <int KIND>
+__device__ __forceinline__ void dev_weights_at(const AosGrid &A, int model, double gam, int dir, int crweno,
+                                               int characteristic, int no_limiting, double eps, int k, int l,
+                                               const double *__restrict__ fC, const double *__restrict__ u,
+                                               double *w1, double *w2, double *w3, long size) {
+  const int nv = A.nv, N = dir ? A.ny : A.nx;
+  const long p = dir ? (long)k * A.nx + l : (A.ndims == 2 ? (long)l * (A.nx + 1) + k : (long)k);
+  auto at = [&](const double *a, int pos, int v) { return a[aos_point(A, dir ? l : pos, dir ? pos : l) * nv + v]; };
+  double c1, c2, c3;
+  if (crweno && k > 0 && k < N) { c1 = 0.2; c2 = 0.5; c3 = 0.3; }
+  else { c1 = 0.1; c2 = 0.6; c3 = 0.3; }
+  const int nf = characteristic ? (model == 1 ? 3 : 4) : 0;
+  double L[4][4] = {{0}};
+  if (nf == 3) {
+    double a0, a1, a2;
+    euler1d_roeavg(gam, at(u, k - 1, 0), at(u, k - 1, 1), at(u, k - 1, 2), at(u, k, 0), at(u, k, 1), at(u, k, 2),
+                   a0, a1, a2);
+    Eig3 E;
+    euler1d_eig(gam, a0, a1, a2, E);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) L[a][b] = E.L[a][b];
+  } else if (nf == 4) {
+    const double uLc[4] = {at(u, k - 1, 0), at(u, k - 1, 1), at(u, k - 1, 2), at(u, k - 1, 3)};
+    const double uRc[4] = {at(u, k, 0), at(u, k, 1), at(u, k, 2), at(u, k, 3)};
+    double ua[4], R[4][4];
+    ns2d_roeavg(gam, uLc, uRc, ua);
+    if (dir) ns2d_eigvecs<1>(gam, ua, L, R);
+    else ns2d_eigvecs<0>(gam, ua, L, R);
+  }
+  // stencil value of variable v at position pos (projected for the flow rows)
+  auto sv = [&](const double *a, int pos, int v) {
+    if (v >= nf) return at(a, pos, v);
+    double y = L[v][0] * at(a, pos, 0);
+    for (int b = 1; b < nf; b++) y = y + L[v][b] * at(a, pos, b);
+    return y;
+  };
+  for (int set = 0; set < 4; set++) {  // LF, LU, RF, RU
+    const double *a = (set & 1) ? u : fC;
+    const bool left = set < 2;
+    for (int v = 0; v < nv; v++) {
+      double x1, x2, x3;
+      if (no_limiting) {
+        x1 = c1; x2 = c2; x3 = c3;
+      } else if (left) {
+        weno_w<KIND>(c1, c2, c3, sv(a, k - 3, v), sv(a, k - 2, v), sv(a, k - 1, v), sv(a, k, v), sv(a, k + 1, v),
+                     eps, x1, x2, x3);
+      } else {
+        weno_w<KIND>(c1, c2, c3, sv(a, k + 2, v), sv(a, k + 1, v), sv(a, k, v), sv(a, k - 1, v), sv(a, k - 2, v),
+                     eps, x1, x2, x3);
+      }
+      const long o = set * size + p * nv + v;
+      w1[o] = x1;
+      w2[o] = x2;
+      w3[o] = x3;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(TB) k_dev_weights(AosGrid A, int model, double gam, int dir, int kind,
+                                                    int crweno, int characteristic, int no_limiting, double eps,
+                                                    const double *__restrict__ fC, const double *__restrict__ u,
+                                                    double *w1, double *w2, double *w3, long size) {
+  const int N = dir ? A.ny : A.nx, nl = A.ndims == 2 ? (dir ? A.nx : A.ny) : 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)(N + 1) * nl) return;
+  const int k = (int)(t % (N + 1)), l = (int)(t / (N + 1));
+  switch (kind) {
+    case 1: dev_weights_at<1>(A, model, gam, dir, crweno, characteristic, no_limiting, eps, k, l, fC, u, w1, w2, w3, size); break;
+    case 2: dev_weights_at<2>(A, model, gam, dir, crweno, characteristic, no_limiting, eps, k, l, fC, u, w1, w2, w3, size); break;
+    case 3: dev_weights_at<3>(A, model, gam, dir, crweno, characteristic, no_limiting, eps, k, l, fC, u, w1, w2, w3, size); break;
+    default: dev_weights_at<0>(A, model, gam, dir, crweno, characteristic, no_limiting, eps, k, l, fC, u, w1, w2, w3, size); break;
+  }
+}
+
+// Ghosted reference-layout array <-> the planar layouts of a case: the natural
+// copy (x fastest) and, in 2D, the transposed copy (y fastest), either may be
+// null.  One thread per (point, variable).  to_planar = 0 copies the other
+// way, from the natural copy, at the ghost points only when ghosts_only.
+__global__ void __launch_bounds__(TB) k_dev_planar(Grid G, const double *__restrict__ a, double *nat, double *tr,
+                                                   int to_planar, int ghosts_only, double *out) {
+  const long np = (long)G.nxg * G.nyg;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np * G.nv) return;
+  const int v = (int)(t % G.nv);
+  const long p = t / G.nv;
+  const int i = (int)(p % G.nxg) - G.g, j = (int)(p / G.nxg) - G.gy;
+  const long pn = (long)v * G.plane + p;
+  if (to_planar) {
+    const double x = a[p * G.nv + v];
+    if (nat) nat[pn] = x;
+    if (tr) tr[(long)v * G.plane + (long)(i + G.g) * G.nyg + (j + G.gy)] = x;
+    return;
+  }
+  const bool ghost = i < 0 || i >= G.nx || j < 0 || j >= G.ny;
+  const bool corner = (i < 0 || i >= G.nx) && (j < 0 || j >= G.ny) && G.ndims == 2;
+  if (ghosts_only && (!ghost || corner)) return;  // the reference never writes the corners
+  out[p * G.nv + v] = nat[pn];
+}
+
+// Interface arrays of one sweep: reference layout <-> line-major
+// a[(v*nlines + l)*(N+1) + k].  One thread per (interface, line, variable).
+__global__ void __launch_bounds__(TB) k_dev_iface(int ndims, int dir, int nx, int N, int nl, int nv,
+                                                  const double *__restrict__ src, double *__restrict__ dst,
+                                                  int to_lines) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)(N + 1) * nl * nv) return;
+  const int v = (int)(t % nv);
+  const long r = t / nv;
+  const int k = (int)(r % (N + 1)), l = (int)(r / (N + 1));
+  const long p = dir ? (long)k * nx + l : (ndims == 2 ? (long)l * (N + 1) + k : (long)k);
+  const long q = ((long)v * nl + l) * (N + 1) + k;
+  if (to_lines) dst[q] = src[p * nv + v];
+  else dst[p * nv + v] = src[q];
+}
+
 
 }  // namespace
 
@@ -2434,7 +2771,7 @@ __global__ void __launch_bounds__(TB) k_weno_weights(int kind, double c1, double
 // host side: work decomposition and launchers
 // ===========================================================================
 
-static const char *kKNames[K_NCLASS] = {"stage_bc", "ghost",  "flux",    "recon_ends", "recon",
+static const char *kKNames[K_NCLASS] = {"stage_bc", "ghost",  "flux",    "recon_ends", "recon", "recon_full",
                                         "recon_char", "cw_rows", "cw_solve", "upwind", "bflux",
                                         "qderiv",   "viscflux", "assemble", "update"};
 const char *kclass_name(int k) { return kKNames[k]; }
@@ -2461,17 +2798,22 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
   blocks[K_CWROWS] = blocks[K_CWSOLVE] = 0;
   if (c.R.characteristic && c.R.crweno) {
     const int nf = c.model == 1 ? 3 : 4;
-    blocks[K_ENDS] = blocks[K_RECON] = blocks[K_RCHAR] = 0;
+    blocks[K_ENDS] = blocks[K_RECON] = blocks[K_RECONF] = blocks[K_RCHAR] = 0;
     blocks[K_CWROWS] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
     blocks[K_CWSOLVE] = nb(cw_x_span(c, nf) + (two ? cw_items(c.Sy, G.nv, nf) : 0));
   } else if (c.R.characteristic) {
     blocks[K_ENDS] = 0;
-    blocks[K_RECON] = 0;
+    blocks[K_RECON] = blocks[K_RECONF] = 0;
     blocks[K_RCHAR] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
   } else {
     blocks[K_RCHAR] = 0;
     blocks[K_ENDS] = nb(2 * (recon_systems(c.Sx, G.nv, c.paired) + (two ? recon_systems(c.Sy, G.nv, c.paired) : 0)));
-    blocks[K_RECON] = recon_sweep_blocks(c.Sx, G.nv, c.paired) + (two ? recon_sweep_blocks(c.Sy, G.nv, c.paired) : 0);
+    const int nrec = recon_sweep_blocks(c.Sx, G.nv, c.paired) + (two ? recon_sweep_blocks(c.Sy, G.nv, c.paired) : 0);
+    // full segments: both sweeps' lanes hold kMaxRows rows each
+    const bool full = PIAFS_RECON_LDS && PIAFS_RECON_FULL && c.paired && c.Sx.rows == kMaxRows &&
+                      (!two || c.Sy.rows == kMaxRows);
+    blocks[K_RECON] = full ? 0 : nrec;
+    blocks[K_RECONF] = full ? nrec : 0;
   }
   blocks[K_UPWIND] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
   blocks[K_BFLUX] = c.cons ? 1 : 0;
@@ -2514,6 +2856,7 @@ void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, MarkF
     hipEventRecord(fk->join, fk->side);
   }
   R(k_recon, K_RECON, st);
+  R(k_recon_full, K_RECONF, st);
   R(k_recon_char, K_RCHAR, st);
   R(k_cw_rows, K_CWROWS, st);
   R(k_cw_solve, K_CWSOLVE, st);
@@ -2549,6 +2892,44 @@ bool launch_tridiag(const double *a, const double *b, const double *c, double *x
   const long nblocks = (ns + teams_per_block - 1) / teams_per_block;
   hipLaunchKernelGGL(k_tridiag, dim3((unsigned)nblocks), dim3(TB), 0, s, a, b, c, x, n, ns, T, m, flag);
   return true;
+}
+
+void launch_dev_flux(const AosGrid &A, int model, double gamma, int dir, const double *u, double *f, hipStream_t s) {
+  const long n = (long)(A.nx + 2 * A.g) * (A.ndims == 2 ? A.ny + 2 * A.g : 1);
+  hipLaunchKernelGGL(k_dev_flux, dim3(nblk(n)), dim3(TB), 0, s, A, model, gamma, dir, u, f);
+}
+
+void launch_dev_fd(const AosGrid &A, int dir, int par4, int bias, const double *f, double *df, hipStream_t s) {
+  const int N = dir ? A.ny : A.nx, nl = A.ndims == 2 ? (dir ? A.nx : A.ny) : 1;
+  const long n = (long)(N + 2 * A.g) * nl * A.nv;
+  hipLaunchKernelGGL(k_dev_fd, dim3(nblk(n)), dim3(TB), 0, s, A, dir, par4, bias, f, df);
+}
+
+void launch_dev_weights(const AosGrid &A, int model, double gamma, int dir, int kind, int crweno,
+                        int characteristic, int no_limiting, double eps, const double *fC, const double *u,
+                        double *w1, double *w2, double *w3, long size, hipStream_t s) {
+  const int N = dir ? A.ny : A.nx, nl = A.ndims == 2 ? (dir ? A.nx : A.ny) : 1;
+  hipLaunchKernelGGL(k_dev_weights, dim3(nblk((long)(N + 1) * nl)), dim3(TB), 0, s, A, model, gamma, dir, kind,
+                     crweno, characteristic, no_limiting, eps, fC, u, w1, w2, w3, size);
+}
+
+void launch_dev_planar(const Grid &G, const double *a, double *nat, double *tr, int to_planar, int ghosts_only,
+                       double *out, hipStream_t s) {
+  const long n = (long)G.nxg * G.nyg * G.nv;
+  hipLaunchKernelGGL(k_dev_planar, dim3(nblk(n)), dim3(TB), 0, s, G, a, nat, tr, to_planar, ghosts_only, out);
+}
+
+void launch_dev_iface(int ndims, int dir, int nx, int N, int nl, int nv, const double *src, double *dst,
+                      int to_lines, hipStream_t s) {
+  const long n = (long)(N + 1) * nl * nv;
+  hipLaunchKernelGGL(k_dev_iface, dim3(nblk(n)), dim3(TB), 0, s, ndims, dir, nx, N, nl, nv, src, dst, to_lines);
+}
+
+void launch_class(const CaseDev *d, const Plan &pl, int cls, int s, hipStream_t st) {
+  static const KernFn table[K_NCLASS] = {k_stage, k_ghost, k_flux, k_recon_ends, k_recon, k_recon_full, k_recon_char,
+                                         k_cw_rows, k_cw_solve, k_upwind, k_bflux, k_qderiv, k_viscflux,
+                                         k_assemble, k_update};
+  run(table[cls], cls, d, pl, s, st, nullptr, nullptr);
 }
 
 void launch_weno_weights(int kind, const double *copt, const double *st, int count, double eps, double *w,

@@ -46,6 +46,22 @@ struct SweepGeom {
 #define PIAFS_MAX_ROWS 16
 #endif
 constexpr int kMaxRows = PIAFS_MAX_ROWS;
+// Doubles in the allocation-free entry point's device scratch array (32 MiB):
+// piafs_tridiag_host_noalloc takes n*ns <= (capacity - 1) / 4.
+#ifndef PIAFS_NOALLOC_SCRATCH_DOUBLES
+#define PIAFS_NOALLOC_SCRATCH_DOUBLES ((size_t)1 << 22)
+#endif
+// Paired reconstruction with each team's line staged in LDS (1) or read by
+// every lane from global memory (0).
+#ifndef PIAFS_RECON_LDS
+#define PIAFS_RECON_LDS 1
+#endif
+// Paired reconstruction of cases whose lanes all hold kMaxRows rows in a
+// kernel of its own (class K_RECONF), whose line solves need no per-row
+// bounds (rows past the system's end made identity rows).
+#ifndef PIAFS_RECON_FULL
+#define PIAFS_RECON_FULL 1
+#endif
 // Team shape for lines of N points: the fewest lanes (a power of two) whose
 // segments of at most max_rows rows cover interior rows 1..N-1.  Returns
 // false when a line is too long for one block.
@@ -125,7 +141,8 @@ struct CaseDev {
 
 // Kernel classes launched per RK stage (plus the final update).
 enum KClass {
-  K_STAGE = 0, K_GHOST, K_FLUX, K_ENDS, K_RECON, K_RCHAR, K_CWROWS, K_CWSOLVE, K_UPWIND, K_BFLUX, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE,
+  K_STAGE = 0, K_GHOST, K_FLUX, K_ENDS, K_RECON, K_RECONF, K_RCHAR, K_CWROWS, K_CWSOLVE, K_UPWIND, K_BFLUX, K_QDERIV,
+  K_VISCFLUX, K_ASSEMBLE, K_UPDATE,
   K_NCLASS
 };
 const char *kclass_name(int k);
@@ -175,6 +192,26 @@ void launch_update(const CaseDev *d_cases, const Plan &plan, hipStream_t st);
 void launch_cfl_norm(const Grid &G, int model, double gamma, double dt, const double *u, const double *u_prev,
                      const double *dxinv, double *partials, int nblocks, hipStream_t s);
 int cfl_norm_blocks(const Grid &G);
+
+// Sizes of an array in the reference's layout: every point with its ghosts,
+// x fastest, the nv components of a point contiguous (ny = 1 in 1D).
+struct AosGrid {
+  int ndims, nv, g, nx, ny;
+};
+// device-buffer operators on reference-layout arrays (see kernels.hip)
+void launch_dev_flux(const AosGrid &A, int model, double gamma, int dir, const double *u, double *f, hipStream_t s);
+void launch_dev_fd(const AosGrid &A, int dir, int par4, int bias, const double *f, double *df, hipStream_t s);
+void launch_dev_weights(const AosGrid &A, int model, double gamma, int dir, int kind, int crweno,
+                        int characteristic, int no_limiting, double eps, const double *fC, const double *u,
+                        double *w1, double *w2, double *w3, long size, hipStream_t s);
+// ghosted reference-layout array <-> planar natural / transposed copies
+void launch_dev_planar(const Grid &G, const double *a, double *nat, double *tr, int to_planar, int ghosts_only,
+                       double *out, hipStream_t s);
+// one sweep's interface array: reference layout <-> line-major
+void launch_dev_iface(int ndims, int dir, int nx, int N, int nl, int nv, const double *src, double *dst,
+                      int to_lines, hipStream_t s);
+// one kernel class of stage s over the plan's cases
+void launch_class(const CaseDev *d_cases, const Plan &plan, int cls, int s, hipStream_t st);
 
 // per-operator kernels used by the unit-level entry points
 // false when n is too long for one team

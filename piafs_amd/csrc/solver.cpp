@@ -398,8 +398,12 @@ int Engine::build() {
   HIPCHK(hipMemsetAsync(d_status, 0, nc * sizeof(int), stream));
   std::vector<CaseDev> hd(nc);
   for (int k = 0; k < nc; k++) {
-    hd[k] = cases[k]->h;
-    hd[k].status = d_status + k;
+    if (override_desc && nc == 1) {
+      hd[k] = *override_desc;  // the descriptor brings its own status word
+    } else {
+      hd[k] = cases[k]->h;
+      hd[k].status = d_status + k;
+    }
   }
   HIPCHK(hipMalloc(&p, nc * sizeof(CaseDev)));
   plan_allocs.push_back(p);
@@ -500,28 +504,30 @@ int Engine::step(int n) {
   return 0;
 }
 
-// Average device time (ms) of kernel class `cls` over `reps` evaluations of
-// stage 0 of every member.  A stage leaves the solution and the time alone
+// Device time (ms) per stage of the kernel classes `cls` and `cls2` (-1:
+// none) -- the sum of their average launch times -- over `reps` evaluations
+// of stage 0 of every member.  A stage leaves the solution and the time alone
 // (it writes the stage state, the interface arrays and Udot[0] only).
-double Engine::time_class(int cls, int reps) {
+double Engine::time_class(int cls, int reps, int cls2) {
   if (!built || cases.empty()) return 0.0;
   const bool was = profiling;
   profiling = true;
   ev_used = 0;
   for (int r = 0; r < reps; r++) launch_stage(d_cases, plan, 0, stream, mark_cb, this);
   hipStreamSynchronize(stream);
-  double tot = 0.0;
-  int cnt = 0;
+  double tot[2] = {0.0, 0.0};
+  int cnt[2] = {0, 0};
   for (size_t k = 0; k < ev_used; k++) {
-    if (ev_cls[k] != cls) continue;
+    const int which = ev_cls[k] == cls ? 0 : (ev_cls[k] == cls2 ? 1 : -1);
+    if (which < 0) continue;
     float m = 0.f;
     hipEventElapsedTime(&m, ev[k].first, ev[k].second);
-    tot += m;
-    cnt++;
+    tot[which] += m;
+    cnt[which]++;
   }
   ev_used = 0;
   profiling = was;
-  return cnt ? tot / cnt : 0.0;
+  return (cnt[0] ? tot[0] / cnt[0] : 0.0) + (cnt[1] ? tot[1] / cnt[1] : 0.0);
 }
 
 // Segment length of the line solves (rows per lane, which sets each case's
@@ -538,8 +544,8 @@ int Engine::autotune(const int *cands, int ncand, int reps, int *best_rows, doub
     if (!ok) continue;
     int rc = build();
     if (rc) return rc;
-    time_class(K_RECON, 1);  // warm
-    const double ms = time_class(K_RECON, reps);
+    time_class(K_RECON, 1, K_RECONF);  // warm
+    const double ms = time_class(K_RECON, reps, K_RECONF);
     if (best < 0 || ms < bms) {
       best = cands[k];
       bms = ms;
@@ -602,12 +608,6 @@ Engine::~Engine() {
 // C ABI: single case = an engine with one member
 // ===========================================================================
 using namespace piafs;
-
-struct piafs_case {
-  Engine eng;
-  Case *c = nullptr;
-  std::string err;
-};
 
 static thread_local std::string g_create_error;  // the calling thread's last failed create
 

@@ -36,6 +36,8 @@ def lib():
         L.orc_weno_weights.argtypes = [ctypes.c_int] + [ctypes.c_double] * 9 + [vp]
         L.orc_fd4_line.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int]
         L.orc_set_derivative_exchange.argtypes = [vp, ctypes.c_int]
+        L.orc_rhs_interfaces.argtypes = [vp, vp, ctypes.c_double, ctypes.c_int] + [vp] * 7
+        L.orc_first_derivative.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp]
         _lib = L
     return _lib
 
@@ -98,6 +100,35 @@ class OCase:
         self.L.orc_rhs_parts(self.h, _p(u), float(t), *[ptr(out[k]) for k in
                              ("u_ghosted", "flux0", "flux1", "fi0", "fi1", "hyp", "par", "src")])
         return out
+
+    def _dims(self):
+        nd = self.L.orc_ndims(self.h)
+        return nd, [self.L.orc_dim(self.h, d) for d in range(nd)], self.L.orc_ghosts(self.h)
+
+    def rhs_interfaces(self, u, t, dir):
+        """uL, uR, fL, fR of every interior line along dir ([line][k][var]) and
+        the weights w1, w2, w3 (each [4 sets][interface point][var], sets LF,
+        LU, RF, RU, interface points in the reference's order) of one
+        right-hand-side evaluation (orc_rhs_interfaces)."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.shape)
+        nd, dims, g = self._dims()
+        nv = self.shape[1]
+        N = dims[dir]
+        nl = dims[1 - dir] if nd == 2 else 1
+        iv = {k: np.zeros((nl, N + 1, nv)) for k in ("uL", "uR", "fL", "fR")}
+        w = [np.zeros((4, (N + 1) * nl, nv)) for _ in range(3)]
+        rc = self.L.orc_rhs_interfaces(self.h, _p(u), float(t), int(dir), *[_p(iv[k]) for k in ("uL", "uR", "fL", "fR")],
+                                       *[_p(x) for x in w])
+        assert rc == 0
+        return iv, w
+
+    def first_derivative(self, f, dir, bias=1):
+        """FirstDerivativePar on a ghosted array (orc_first_derivative); the
+        points it does not compute are zero."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        df = np.zeros_like(f)
+        self.L.orc_first_derivative(self.h, _p(f), int(dir), int(bias), _p(df))
+        return df
 
     def set_derivative_exchange(self, on):
         self.L.orc_set_derivative_exchange(self.h, int(bool(on)))

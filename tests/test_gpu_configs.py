@@ -14,6 +14,7 @@
 Tolerances: FP64 relative L2 per variable <= 1e-11 against the oracle;
 batched and served results equal the same deck run alone bit for bit (the same
 device code runs on the same data)."""
+import math
 import os
 import tempfile
 import threading
@@ -149,8 +150,12 @@ def test_4096_chem_matches_oracle_then_conserves():
     the oracle's run of the same deck (norms to 1e-12; stride-128 subsample,
     row and column sums to 1e-11); after all 50 steps the state is finite and
     the variables without a source or viscous term -- density, O2 and CO2 --
-    are conserved to roundoff (periodic box: the boundary fluxes of each line
-    cancel)."""
+    are conserved (periodic box: the boundary fluxes of each line cancel).
+    Conservation is measured on exact sums of the states (math.fsum), 1e-13
+    relative.  The reference's own measure (CalculateConservationError.c:19,
+    piafs_conservation_errors) sums 1.7e7 values in point order, and that
+    rounding alone is about 1e-10 for O2, which starts at exactly 1 at every
+    point; it is held to 1e-8."""
     z = np.load(C5_FIXTURE)
     steps = int(z["steps"])
     dk = decks.ns2d_photochem(nx=4096, n_iter=50, dt=0.0025)
@@ -158,8 +163,12 @@ def test_4096_chem_matches_oracle_then_conserves():
     with tempfile.TemporaryDirectory() as d:
         dk.write(d)
         del dk
+        cons = (0, 4, 10)
         with piafs_amd.Case(d) as c:
             assert (c.info.nvars, c.info.dim[0], c.info.dim[1]) == (11, 4096, 4096)
+            u0 = c.solution()
+            s0 = [math.fsum(u0[:, v]) for v in cons]
+            del u0
             c.step(steps)
             U = c.solution().reshape(4096, 4096, -1)
             assert abs(c.time - float(z["t"])) < 1e-15
@@ -173,5 +182,6 @@ def test_4096_chem_matches_oracle_then_conserves():
             ce = c.conservation_errors()
             u = c.solution()
     assert np.isfinite(u).all()
-    for v in (0, 4, 10):
-        assert 0.0 <= ce[v] <= 1e-12, (v, ce)
+    for v, a0 in zip(cons, s0):
+        assert abs(math.fsum(u[:, v]) - a0) <= 1e-13 * max(1.0, abs(a0)), (v, math.fsum(u[:, v]), a0)
+        assert 0.0 <= ce[v] <= 1e-8, (v, ce)

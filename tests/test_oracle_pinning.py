@@ -367,3 +367,146 @@ def test_derivative_exchange_is_translation_invariant():
     assert not np.array_equal(a, b)
     # density has no viscous term: the exchange leaves it alone everywhere
     assert np.array_equal(a[..., 0], b[..., 0])
+
+
+def _ghosted(dk, u):
+    """The boundary-filled ghosted state of u (the oracle's rhs_parts)."""
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        return orc.OCase(d).rhs_parts(u, 0.0)["u_ghosted"]
+
+
+def _interfaces(dk, u, dir, t=0.0):
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        c = orc.OCase(d)
+        P = c.rhs_parts(u, t)
+        iv, w = c.rhs_interfaces(u, t, dir)
+    return P, iv, w
+
+
+@pytest.mark.parametrize("name,dir", [("sine_weno_comp_N100", 0), ("ns2d_chem_weno_z_16x24", 0),
+                                      ("ns2d_chem_weno_z_16x24", 1)])
+def test_captured_weights_make_the_interface_values(ref_runs, name, dir):
+    """orc_rhs_interfaces: WENO5 (Interp1PrimFifthOrderWENO.c:124-166)
+    assembled from the captured weights and the stencils of the boundary-filled
+    state and flux reproduces the captured uL, uR, fL, fR bit for bit.  The
+    interface values come out of the reconstruction the reference runs pin;
+    this pins the weights' set order (LF, LU, RF, RU), their interface-point
+    order and the [line][interface][var] layout of both."""
+    _, meta = ref_runs
+    dk = build_deck(meta[name])
+    rng = np.random.default_rng(3)
+    u = dk.u * (1.0 + 1e-2 * rng.uniform(-1, 1, dk.u.shape))
+    P, iv, w = _interfaces(dk, u, dir)
+    nd = dk.solver["ndims"]
+    g, nv = 3, dk.u.shape[1]
+    dims = list(dk.solver["size"]) if nd == 2 else [dk.solver["size"]]
+    N = dims[dir]
+    shape = [n + 2 * g for n in dims][::-1] + [nv]
+    uG = P["u_ghosted"].reshape(shape)
+    fG = P["flux%d" % dir].reshape(shape)
+    s6 = 1.0 / 6.0
+    for src, (lname, rname), (sl, sr) in ((uG, ("uL", "uR"), (1, 3)), (fG, ("fL", "fR"), (0, 2))):
+        if nd == 2:   # lines along dir: [line][position][var]
+            lines = src[g:-g, :, :] if dir == 0 else np.transpose(src[:, g:-g, :], (1, 0, 2))
+        else:
+            lines = src[None, :, :]
+        k = np.arange(N + 1)
+        pos = lambda o: lines[:, g + k + o, :]                     # noqa: E731
+        for side, name_, st in ((0, lname, sl), (1, rname, sr)):
+            if side == 0:
+                m3, m2, m1, p1, p2 = pos(-3), pos(-2), pos(-1), pos(0), pos(1)
+            else:
+                m3, m2, m1, p1, p2 = pos(2), pos(1), pos(0), pos(-1), pos(-2)
+            f1 = (2 * s6) * m3 + (-7 * s6) * m2 + (11 * s6) * m1
+            f2 = (-s6) * m2 + (5 * s6) * m1 + (2 * s6) * p1
+            f3 = (2 * s6) * m1 + (5 * s6) * p1 + (-s6) * p2
+            nl = lines.shape[0]
+            ws = [x[st].reshape((nl, N + 1, nv) if (nd == 1 or dir == 0) else (N + 1, nl, nv)) for x in w]
+            if nd == 2 and dir == 1:
+                ws = [np.transpose(x, (1, 0, 2)) for x in ws]
+            got = ws[0] * f1 + ws[1] * f2 + ws[2] * f3
+            assert np.array_equal(got, iv[name_]), (name_, np.abs(got - iv[name_]).max())
+
+
+@pytest.mark.parametrize("name", ["sod_char_roe_N128", "sod_crweno_char_roe_N64", "ns2d_chem_char_rf_24x20",
+                                  "ns2d_crweno_char_flow_20x16", "ns2d_chem_crweno_js_24x20"])
+def test_captured_weights_are_normalised(ref_runs, name):
+    """Every captured weight set (component-wise and characteristic, WENO5
+    and CRWENO5) lies in [0, 1] and sums to one to roundoff."""
+    _, meta = ref_runs
+    dk = build_deck(meta[name])
+    for dir in range(dk.solver["ndims"]):
+        _, _, w = _interfaces(dk, dk.u, dir)
+        assert min(x.min() for x in w) >= 0.0 and max(x.max() for x in w) <= 1.0
+        assert np.allclose(w[0] + w[1] + w[2], 1.0, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("interp", ["characteristic", "components"])
+def test_interface_values_and_weights_are_biased_the_right_way(interp):
+    """On the Sod step (density 1 | 0.125 between cells k0-1 and k0): the
+    left-biased value at interface k0 is the left state and the right-biased
+    one the right state (ENO stencils); at interface k0+1 the left-biased
+    solution weights put everything on the third stencil (cells k0..k0+2, the
+    only one without the jump), while the right-biased ones give the third
+    stencil (cells k0+1..k0-1, the only one with it) nothing and the other two
+    their optimal ratio 0.1 : 0.6."""
+    dk = decks.sod_1d(N=64, n_iter=10, interp=interp)
+    u = dk.u
+    k0 = int(np.argmax(u[:, 0] < 0.5))
+    _, iv, w = _interfaces(dk, u, 0)
+    assert abs(iv["uL"][0, k0, 0] - 1.0) < 1e-6 and abs(iv["uR"][0, k0, 0] - 0.125) < 1e-6
+    LU, RU = 1, 3
+    nv = u.shape[1]
+    # the variables that jump (the momentum is zero on both sides, so its
+    # component-wise weights stay optimal)
+    jv = np.abs(u[k0] - u[k0 - 1]) > 0 if interp == "components" else np.ones(nv, bool)
+    w3_lu = w[2][LU].reshape(-1, nv)[k0 + 1][jv]
+    w_ru = [x[RU].reshape(-1, nv)[k0 + 1][jv] for x in w]
+    assert np.all(w3_lu > 0.999), w3_lu
+    assert np.all(w_ru[2] < 1e-6) and np.allclose(w_ru[0], 1.0 / 7.0, atol=1e-6), w_ru
+
+
+def test_interface_layout_across_y():
+    """2D, dimension 1: on the Riemann quadrants (a density jump along y in
+    every column), the captured [x line][y interface] values of a column sit
+    at that column's jump: uL is the state below it and uR the state above."""
+    dk = decks.riemann_2d(N=33, n_iter=4, dt=0.004)
+    n = 33
+    u = dk.u
+    _, iv, _ = _interfaces(dk, u, 1)
+    U = u.reshape(n, n, -1)  # [y][x][var]
+    for i in (3, n - 4):
+        col = U[:, i, 0]
+        k0 = int(np.argmax(np.abs(np.diff(col)) > 1e-3)) + 1   # first cell above the jump
+        # within 1e-4 of the states either side of a jump of about 0.6 (mapped
+        # weights leave the stencils across the jump a little weight)
+        assert abs(iv["uL"][i, k0, 0] - col[k0 - 1]) < 1e-4 and abs(iv["uR"][i, k0, 0] - col[k0]) < 1e-4
+
+
+@pytest.mark.parametrize("par", ["4", "2"])
+def test_first_derivative_exact_on_polynomials(par):
+    """orc_first_derivative: with par_space_scheme 4 the central and one-sided
+    fourth-order stencils (FirstDerivativeFourthOrder.c:39-132) are exact for
+    quartics at every position of every interior line, ghosts included; with
+    "2" (FirstDerivativeFirstOrder.c:20-95, bias +1 and -1) the differences are
+    exact for linear data.  Both dimensions; the points off the interior lines
+    are not written."""
+    dk = decks.ns2d_photochem(nx=12, ny=9, n_iter=2, par=par)
+    g, nv, nx, ny = 3, dk.u.shape[1], 12, 9
+    X, Y = np.meshgrid(np.arange(-g, nx + g, dtype=float), np.arange(-g, ny + g, dtype=float))
+    deg = 4 if par == "4" else 1
+    coef = np.arange(1, nv + 1, dtype=float)
+    f = np.stack([c * (0.5 + 0.1 * X) ** deg + (1.5 - 0.2 * Y) ** deg for c in coef], axis=-1)
+    dfx = np.stack([c * deg * 0.1 * (0.5 + 0.1 * X) ** (deg - 1) for c in coef], axis=-1)
+    dfy = np.stack([deg * (-0.2) * (1.5 - 0.2 * Y) ** (deg - 1) + 0 * c for c in coef], axis=-1)
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        c = orc.OCase(d)
+        for bias in ((1,) if par == "4" else (1, -1)):
+            gx = c.first_derivative(f.reshape(-1, nv), 0, bias).reshape(f.shape)
+            gy = c.first_derivative(f.reshape(-1, nv), 1, bias).reshape(f.shape)
+            assert np.allclose(gx[g:-g, :], dfx[g:-g, :], rtol=0, atol=1e-11)
+            assert np.allclose(gy[:, g:-g], dfy[:, g:-g], rtol=0, atol=1e-11)
+            assert not gx[:g].any() and not gx[-g:].any() and not gy[:, :g].any() and not gy[:, -g:].any()
