@@ -59,14 +59,15 @@ def kernel_algorithmic_bytes(name, nx, ny, nv, g, nstages, crweno, paired=True):
     return None
 
 
-def measured_traffic(nx):
-    """HBM bytes per launch of the reconstruction kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by a copy
-    kernel of known size), or None when no profile of this grid is committed."""
+def measured_traffic(nx, kernel):
+    """HBM bytes per launch of a kernel class from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by a copy kernel of
+    known size), or None when no profile of this grid and kernel is committed."""
     path = os.path.join(ROOT, "profiles", f"ns2d_crweno5_{nx}_step_kernels.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"`k_recon`: HBM traffic ([0-9.]+) MB read \+ ([0-9.]+) MB written", open(path).read())
+    m = re.search(r"`k_%s`: HBM traffic ([0-9.]+) MB read \+ ([0-9.]+) MB written" % re.escape(kernel),
+                  open(path).read())
     return (float(m.group(1)) + float(m.group(2))) * 1e6 if m else None
 
 
@@ -437,7 +438,7 @@ def main():
             ach = byts / (ms * 1e-3) / 1e9
             line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": ach / HBM_PEAK_GBS,
-                                "traffic": measured_traffic(nx) if dom in ("recon", "recon_full") else None,
+                                "traffic": measured_traffic(nx, dom),
                                 "kernel": dom, "kernel_ms": ms, "algorithmic_bytes_per_launch": byts,
                                 "algorithmic_bytes_rule": "40 * nvars B per cell-update and stage",
                                 "kernel_io_bytes_per_launch": io_byts,

@@ -222,7 +222,8 @@ int  piafs_autotune(piafs_case *c, int *rows_per_lane, double *recon_ms);
  * the device work runs on an internal stream and is synchronised).
  */
 
-/* Batched tridiagonal solve, one lane per system.  Replaces tridiagLU
+/* Batched tridiagonal solve: a team of lanes per system of up to 4097 rows,
+ * one thread per longer system.  Replaces tridiagLU
  * (src/TridiagLU/tridiagLU.c:86-277, serial stages 1 and 4): a, b, c, x each
  * [n][ns] (row-major over rows, systems contiguous, as tridiagLU.h:60-62);
  * x is overwritten with the solution.  Returns -1 on a zero pivot. */

@@ -935,7 +935,7 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
 // combinations the Roe upwinding uses, 0.5*(uR - uL) and 0.5*(fL + fR)
 // (NavierStokes2DUpwind.c:58-60, Euler1DUpwind.c:45-52).
 __host__ __device__ inline long recon_systems(const SweepGeom &S, int nv, int paired) {
-  return (long)S.nlines * nv * (paired ? 2 : 4);
+  return S.longl ? 0 : (long)S.nlines * nv * (paired ? 2 : 4);
 }
 
 template <int CR, int KIND>
@@ -1806,6 +1806,128 @@ __host__ __device__ inline int recon_sweep_blocks(const SweepGeom &S, int nv, in
   return (int)((nsys + teams_per_block - 1) / teams_per_block);
 }
 
+// ---------------------------------------------------------------------------
+// Component-wise reconstruction of lines longer than one team covers
+// (SweepGeom::longl): k_lrows assembles every row of every system, end rows
+// included, one thread per (interface, system), into the sweep's scratch;
+// k_lsolve then solves the two systems of a pair (uL and uR, or fL and fR of
+// one line and variable) one thread each pair, by the sequential Thomas sweep
+// of tridiagLU.c:133-150 and :243-255 over rows 0..N (the reference solves the
+// same N+1 rows, end rows being identity rows), and writes the interface
+// arrays as the team kernels do.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline long long_rows(const SweepGeom &S, int nv) {
+  return S.longl ? (long)(S.N + 1) * S.nlines * nv * 4 : 0;
+}
+
+template <int CR, int KIND>
+__device__ __forceinline__ void lrow_at(const CaseDev &C, int dir, long t) {
+  const SweepGeom &S = dir ? C.Sy : C.Sx;
+  const int nv = C.G.nv, N = S.N;
+  const long ns4 = (long)S.nlines * nv * 4;
+  const long sys = t % ns4;
+  const int k = (int)(t / ns4);
+  const int r = (int)(sys % 4), v = (int)((sys / 4) % nv), l = (int)(sys / 4 / nv);
+  const double *src = (r < 2 ? (dir ? C.UT : C.U) : (dir ? C.flux_y : C.flux_x)) + (long)v * S.plane + S.base +
+                      (long)l * S.line_stride;
+  double a, b, c, x;
+  row_coeffs<CR, KIND>(k, N, !(r & 1), C.R, src[k - 3], src[k - 2], src[k - 1], src[k], src[k + 1], src[k + 2], a,
+                       b, c, x);
+  double *L = C.lrow[dir];
+  const long f = (long)(N + 1) * ns4, o = (long)k * ns4 + sys;
+  L[o] = a;
+  L[f + o] = b;
+  L[2 * f + o] = c;
+  L[3 * f + o] = x;
+}
+
+__global__ void __launch_bounds__(256) k_lrows(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const long nX = long_rows(C.Sx, C.G.nv), nY = C.G.ndims == 2 ? long_rows(C.Sy, C.G.nv) : 0;
+  if (t >= nX + nY) return;
+  const int dir = t < nX ? 0 : 1;
+  const long tt = dir ? t - nX : t;
+  const int cr = C.R.crweno;
+  switch (C.R.kind) {
+    case 1: cr ? lrow_at<1, 1>(C, dir, tt) : lrow_at<0, 1>(C, dir, tt); break;
+    case 2: cr ? lrow_at<1, 2>(C, dir, tt) : lrow_at<0, 2>(C, dir, tt); break;
+    case 3: cr ? lrow_at<1, 3>(C, dir, tt) : lrow_at<0, 3>(C, dir, tt); break;
+    case 4: lrow_at<0, 4>(C, dir, tt); break;
+    default: cr ? lrow_at<1, 0>(C, dir, tt) : lrow_at<0, 0>(C, dir, tt); break;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_lsolve(CASE_ARGS) {
+  CASE_PREAMBLE
+  if (stage >= C.nstages) return;
+  const int nv = C.G.nv;
+  const long nX = C.Sx.longl ? (long)C.Sx.nlines * nv * 2 : 0;
+  const long nY = (C.G.ndims == 2 && C.Sy.longl) ? (long)C.Sy.nlines * nv * 2 : 0;
+  if (t >= nX + nY) return;
+  const int dir = t < nX ? 0 : 1;
+  const long p = dir ? t - nX : t;  // pair (line*nv + var)*2 + q
+  const SweepGeom &S = dir ? C.Sy : C.Sx;
+  const int N = S.N, q = (int)(p % 2);
+  const long lv = p / 2;  // line*nv + var
+  const int v = (int)(lv % nv), l = (int)(lv / nv);
+  const long ns4 = (long)S.nlines * nv * 4, f = (long)(N + 1) * ns4;
+  double *L = C.lrow[dir];
+  const long s0 = lv * 4 + 2 * q;  // the left-biased system; s0 + 1 the right-biased one
+  double *A = L, *B = L + f, *Cc = L + 2 * f, *X = L + 3 * f;
+  if (C.R.crweno) {
+    // forward: b'_k = b_k - (a_k / b'_{k-1}) c_{k-1}, x'_k = x_k - (a_k / b'_{k-1}) x'_{k-1}
+    double bp[2], xp[2], cp[2];
+    bool zp = false;
+    for (int j = 0; j < 2; j++) {
+      bp[j] = B[s0 + j];
+      xp[j] = X[s0 + j];
+      cp[j] = Cc[s0 + j];
+      zp = zp || bp[j] == 0.0;
+    }
+    for (int k = 1; k <= N; k++) {
+      const long o = (long)k * ns4 + s0;
+      for (int j = 0; j < 2; j++) {
+        const double fac = A[o + j] / bp[j];
+        bp[j] = B[o + j] - fac * cp[j];
+        xp[j] = X[o + j] - fac * xp[j];
+        cp[j] = Cc[o + j];
+        zp = zp || bp[j] == 0.0;
+        B[o + j] = bp[j];
+        X[o + j] = xp[j];
+      }
+    }
+    if (zp) atomicOr(C.status, 1);
+    // back substitution: x_N = x'_N / b'_N, x_k = (x'_k - c_k x_{k+1}) / b'_k
+    double xn[2];
+    for (int j = 0; j < 2; j++) {
+      const long o = (long)N * ns4 + s0 + j;
+      xn[j] = X[o] / B[o];
+      X[o] = xn[j];
+    }
+    for (int k = N - 1; k >= 0; k--) {
+      const long o = (long)k * ns4 + s0;
+      for (int j = 0; j < 2; j++) {
+        xn[j] = (X[o + j] - Cc[o + j] * xn[j]) / B[o + j];
+        X[o + j] = xn[j];
+      }
+    }
+  }
+  // interface arrays: split mode uL, uR (q 0) or fL, fR (q 1); paired mode
+  // 0.5*(uR - uL) or 0.5*(fL + fR)
+  double *const *io = dir ? C.iy : C.ix;
+  const long line = ((long)v * S.nlines + l) * (N + 1);
+  for (int k = 0; k <= N; k++) {
+    const long o = (long)k * ns4 + s0;
+    const double xl = X[o], xr = X[o + 1];
+    if (C.paired) io[q][line + k] = q ? 0.5 * (xl + xr) : 0.5 * (xr - xl);
+    else {
+      io[2 * q][line + k] = xl;
+      io[2 * q + 1][line + k] = xr;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_recon_ends(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
@@ -2569,6 +2691,36 @@ __global__ void __launch_bounds__(TB) k_tridiag(const double *__restrict__ a, co
     if (i < nr) x[(long)(lo + i) * ns + d] = X[i];
 }
 
+// The same for systems longer than one team covers: one thread per system,
+// the sequential sweep itself (tridiagLU.c:133-150 and :243-255, a_0 = c_{n-1}
+// = 0).  b receives the pivots b', x receives x' and then the solution.
+__global__ void __launch_bounds__(TB) k_tridiag_seq(const double *__restrict__ a, double *b,
+                                                    const double *__restrict__ c, double *__restrict__ x, int n,
+                                                    int ns, int *__restrict__ flag) {
+  const long d = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ns) return;
+  double bp = b[d], xp = x[d], cp = (n > 1) ? c[d] : 0.0;
+  bool zp = bp == 0.0;
+  for (int i = 1; i < n; i++) {
+    const long q = (long)i * ns + d;
+    const double fac = a[q] / bp;
+    bp = b[q] - fac * cp;
+    xp = x[q] - fac * xp;
+    cp = (i == n - 1) ? 0.0 : c[q];
+    zp = zp || bp == 0.0;
+    b[q] = bp;
+    x[q] = xp;
+  }
+  if (zp) atomicOr(flag, 1);
+  double xn = x[(long)(n - 1) * ns + d] / b[(long)(n - 1) * ns + d];
+  x[(long)(n - 1) * ns + d] = xn;
+  for (int i = n - 2; i >= 0; i--) {
+    const long q = (long)i * ns + d;
+    xn = (x[q] - c[q] * xn) / b[q];
+    x[q] = xn;
+  }
+}
+
 __global__ void __launch_bounds__(TB) k_weno_weights(int kind, double c1, double c2, double c3,
                                                      const double *__restrict__ st, int count, double eps,
                                                      double *__restrict__ w) {
@@ -2772,6 +2924,7 @@ __global__ void __launch_bounds__(TB) k_dev_iface(int ndims, int dir, int nx, in
 // ===========================================================================
 
 static const char *kKNames[K_NCLASS] = {"stage_bc", "ghost",  "flux",    "recon_ends", "recon", "recon_full",
+                                        "long_rows", "long_solve",
                                         "recon_char", "cw_rows", "cw_solve", "upwind", "bflux",
                                         "qderiv",   "viscflux", "assemble", "update"};
 const char *kclass_name(int k) { return kKNames[k]; }
@@ -2795,7 +2948,7 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
   blocks[K_STAGE] = two ? ((G.nx + 31) / 32) * ((G.ny + 31) / 32) * G.nv : nb((long)G.nx * G.nv);
   blocks[K_GHOST] = nb(ghost_points(G) * G.nv);
   blocks[K_FLUX] = nb((long)(c.Sx.N + 2 * c.Sx.g) * c.Sx.nlines + (two ? (long)(c.Sy.N + 2 * c.Sy.g) * c.Sy.nlines : 0));
-  blocks[K_CWROWS] = blocks[K_CWSOLVE] = 0;
+  blocks[K_CWROWS] = blocks[K_CWSOLVE] = blocks[K_LROWS] = blocks[K_LSOLVE] = 0;
   if (c.R.characteristic && c.R.crweno) {
     const int nf = c.model == 1 ? 3 : 4;
     blocks[K_ENDS] = blocks[K_RECON] = blocks[K_RECONF] = blocks[K_RCHAR] = 0;
@@ -2814,6 +2967,9 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
                       (!two || c.Sy.rows == kMaxRows);
     blocks[K_RECON] = full ? 0 : nrec;
     blocks[K_RECONF] = full ? nrec : 0;
+    blocks[K_LROWS] = nb(long_rows(c.Sx, G.nv) + (two ? long_rows(c.Sy, G.nv) : 0));
+    blocks[K_LSOLVE] = nb((c.Sx.longl ? (long)c.Sx.nlines * G.nv * 2 : 0) +
+                          (two && c.Sy.longl ? (long)c.Sy.nlines * G.nv * 2 : 0));
   }
   blocks[K_UPWIND] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
   blocks[K_BFLUX] = c.cons ? 1 : 0;
@@ -2857,6 +3013,8 @@ void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, MarkF
   }
   R(k_recon, K_RECON, st);
   R(k_recon_full, K_RECONF, st);
+  R(k_lrows, K_LROWS, st);
+  R(k_lsolve, K_LSOLVE, st);
   R(k_recon_char, K_RCHAR, st);
   R(k_cw_rows, K_CWROWS, st);
   R(k_cw_solve, K_CWSOLVE, st);
@@ -2884,10 +3042,13 @@ void launch_cfl_norm(const Grid &G, int model, double gamma, double dt, const do
   hipLaunchKernelGGL(k_cfl_norm, dim3(nblocks), dim3(TB), 0, s, G, model, gamma, dt, u, u_prev, dxinv, partials);
 }
 
-bool launch_tridiag(const double *a, const double *b, const double *c, double *x, int n, int ns, int *flag,
+bool launch_tridiag(const double *a, double *b, const double *c, double *x, int n, int ns, int *flag,
                     hipStream_t s) {
   int T, m;
-  if (!team_shape(n + 1, kMaxRows, T, m)) return false;
+  if (!team_shape(n + 1, kMaxRows, T, m)) {  // longer than one team: one thread per system
+    hipLaunchKernelGGL(k_tridiag_seq, dim3(nblk(ns)), dim3(TB), 0, s, a, b, c, x, n, ns, flag);
+    return true;
+  }
   const long teams_per_block = TB / T;
   const long nblocks = (ns + teams_per_block - 1) / teams_per_block;
   hipLaunchKernelGGL(k_tridiag, dim3((unsigned)nblocks), dim3(TB), 0, s, a, b, c, x, n, ns, T, m, flag);
@@ -2926,7 +3087,8 @@ void launch_dev_iface(int ndims, int dir, int nx, int N, int nl, int nv, const d
 }
 
 void launch_class(const CaseDev *d, const Plan &pl, int cls, int s, hipStream_t st) {
-  static const KernFn table[K_NCLASS] = {k_stage, k_ghost, k_flux, k_recon_ends, k_recon, k_recon_full, k_recon_char,
+  static const KernFn table[K_NCLASS] = {k_stage, k_ghost, k_flux, k_recon_ends, k_recon, k_recon_full, k_lrows,
+                                         k_lsolve, k_recon_char,
                                          k_cw_rows, k_cw_solve, k_upwind, k_bflux, k_qderiv, k_viscflux,
                                          k_assemble, k_update};
   run(table[cls], cls, d, pl, s, st, nullptr, nullptr);

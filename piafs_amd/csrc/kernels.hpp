@@ -38,6 +38,8 @@ struct SweepGeom {
   int line_stride;   // elements between consecutive lines
   int team;          // lanes sharing one line system (power of two, <= 256)
   int rows;          // interior rows (1 .. N-1) held by each lane, <= kMaxRows
+  int longl;         // lines longer than one team covers (team_shape fails): their
+                     // component-wise systems go through k_lrows / k_lsolve instead
 };
 
 // Longest segment of a line system one lane keeps in registers (the
@@ -136,13 +138,17 @@ struct CaseDev {
   // characteristic CRWENO5: block rows of each sweep [field][line][k] and the
   // back-substitution data of its four systems [field][system][line][k]
   double *cwr[2], *cwb[2];
+  // sweeps of long lines: rows A, B, C and right-hand side of every system
+  // [field][interface k][system], system = (line*nvars + var)*4 + r
+  // (r: uL, uR, fL, fR), written by k_lrows and solved in place by k_lsolve
+  double *lrow[2];
   int *status;   // bit 0: a zero pivot met in a line solve (tridiagLU.c:139-142 returns -1)
 };
 
 // Kernel classes launched per RK stage (plus the final update).
 enum KClass {
-  K_STAGE = 0, K_GHOST, K_FLUX, K_ENDS, K_RECON, K_RECONF, K_RCHAR, K_CWROWS, K_CWSOLVE, K_UPWIND, K_BFLUX, K_QDERIV,
-  K_VISCFLUX, K_ASSEMBLE, K_UPDATE,
+  K_STAGE = 0, K_GHOST, K_FLUX, K_ENDS, K_RECON, K_RECONF, K_LROWS, K_LSOLVE, K_RCHAR, K_CWROWS, K_CWSOLVE, K_UPWIND,
+  K_BFLUX, K_QDERIV, K_VISCFLUX, K_ASSEMBLE, K_UPDATE,
   K_NCLASS
 };
 const char *kclass_name(int k);
@@ -213,9 +219,13 @@ void launch_dev_iface(int ndims, int dir, int nx, int N, int nl, int nv, const d
 // one kernel class of stage s over the plan's cases
 void launch_class(const CaseDev *d_cases, const Plan &plan, int cls, int s, hipStream_t st);
 
+// doubles of the long-line scratch of one sweep (SweepGeom::longl)
+inline size_t lrow_doubles(const SweepGeom &S, int nv) { return (size_t)4 * (S.N + 1) * S.nlines * nv * 4; }
+
 // per-operator kernels used by the unit-level entry points
-// false when n is too long for one team
-bool launch_tridiag(const double *a, const double *b, const double *c, double *x, int n, int ns, int *flag,
+// (systems longer than one team are solved one thread per system)
+// b may be overwritten (with the pivots, for systems longer than one team)
+bool launch_tridiag(const double *a, double *b, const double *c, double *x, int n, int ns, int *flag,
                     hipStream_t s);
 void launch_weno_weights(int kind, const double *copt, const double *st, int count, double eps, double *w,
                          hipStream_t s);

@@ -36,7 +36,7 @@ extern "C" int piafs_tridiag_host(const double *a, const double *b, const double
       hipMemcpy(dx.p, x, m * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(flag, 0, sizeof(int)) != hipSuccess)
     return -1;
-  if (!piafs::launch_tridiag(da.p, db.p, dc.p, dx.p, n, ns, flag, 0)) return 1;  // longer than one team covers
+  if (!piafs::launch_tridiag(da.p, db.p, dc.p, dx.p, n, ns, flag, 0)) return 1;
   if (hipGetLastError() != hipSuccess) return -1;
   int hflag = 0;
   if (hipMemcpy(x, dx.p, m * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||

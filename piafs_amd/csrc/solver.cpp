@@ -81,6 +81,12 @@ size_t Case::plan_bytes(const Config &c) {
     n += (size_t)(cw_row_fields(nf, (int)nv) + 4 * cw_back_fields(nf, (int)nv)) * (ix + iy) / nv;
   }
   if (c.model_id == MODEL_NS2D) n += 12 * (size_t)g.nxg * g.nyg;
+  if (!c.characteristic) {  // long-line scratch of sweeps longer than one team
+    int T, m;
+    for (int d = 0; d < c.ndims; d++)
+      if (!team_shape(c.dim[d], kMaxRows, T, m))
+        n += (size_t)4 * (c.dim[d] + 1) * (c.ndims == 2 ? c.dim[1 - d] : 1) * nv * 4;
+  }
   n += plane + c.x.size() + 4096;
   return n * sizeof(double);
 }
@@ -196,6 +202,9 @@ int Case::setup(int dev, hipStream_t st) {
   const int narr = D.paired ? 2 : 4;
   for (int k = 0; k < narr; k++) D.ix[k] = dalloc(ix, st);
   D.fIx = dalloc(ix, st);
+  if (D.Sx.longl) D.lrow[0] = dalloc(lrow_doubles(D.Sx, nv), st);
+  if (two && D.Sy.longl) D.lrow[1] = dalloc(lrow_doubles(D.Sy, nv), st);
+  if ((D.Sx.longl && !D.lrow[0]) || (two && D.Sy.longl && !D.lrow[1])) return fail(-2, "device allocation failed");
   if (two) {
     const size_t iy = (size_t)nv * (D.Sy.N + 1) * D.Sy.nlines;
     D.flux_y = dalloc(st_sz, st);
@@ -279,10 +288,19 @@ int Case::set_team_rows(int max_rows) {
     D.Sx.rows = D.Sy.rows = 1;
     return 0;
   }
-  if (!team_shape(D.Sx.N, max_rows, D.Sx.team, D.Sx.rows) ||
-      (cfg.ndims == 2 && !team_shape(D.Sy.N, max_rows, D.Sy.team, D.Sy.rows)))
-    return fail(1, "grid lines longer than " + std::to_string(256 * kMaxRows + 1) +
-                       " points are not supported by the line solver");
+  for (int d = 0; d < cfg.ndims; d++) {
+    SweepGeom &S = d ? D.Sy : D.Sx;
+    int T, m;
+    if (!team_shape(S.N, kMaxRows, T, m)) {  // longer than a 256-lane team covers: the sequential solver
+      S.longl = 1;
+      S.team = S.rows = 1;
+      continue;
+    }
+    S.longl = 0;
+    if (!team_shape(S.N, max_rows, S.team, S.rows))
+      return fail(1, "lines of " + std::to_string(S.N) + " points need more than 256 lanes at " +
+                         std::to_string(max_rows) + " rows per lane");
+  }
   return 0;
 }
 

@@ -149,6 +149,13 @@ CASES = {
     "sine_crweno_comp_rk22_N64": ("density_sine_1d", dict(N=64, n_iter=30, rk="22")),
     "ns2d_chem_crweno_rk33_16x12": ("ns2d_photochem", dict(nx=16, ny=12, n_iter=8, rk="33")),
     "ns2d_chem_weno_rk22_12x16": ("ns2d_photochem", dict(nx=12, ny=16, n_iter=6, rk="22", scheme="weno5")),
+    # lines longer than a 256-lane team of 16-row segments covers (4097 points):
+    # the sequential line solver, paired (Roe) and split (rf-char), 1D and 2D
+    "sod_crweno_comp_N8193": ("sod_1d", dict(N=8193, n_iter=40, t_final=40 * 0.2 / 7209, scheme="crweno5",
+                                             interp="components")),
+    "sod_crweno_comp_rf_N5000": ("sod_1d", dict(N=5000, n_iter=20, t_final=20 * 0.2 / 4399, scheme="crweno5",
+                                                interp="components", upwinding="rf-char")),
+    "ns2d_chem_crweno_long_4200x8": ("ns2d_photochem", dict(nx=4200, ny=8, n_iter=4, dt=0.002)),
     # upw5, the linear fifth-order upwind interpolation (Interp1PrimFifthOrderUpwind[Char].c)
     "sod_upw5_char_roe_N64": ("sod_1d", dict(N=64, n_iter=20, t_final=0.07, scheme="upw5")),
     "sod_upw5_comp_rf_N64": ("sod_1d", dict(N=64, n_iter=20, t_final=0.07, scheme="upw5", interp="components",

@@ -5,6 +5,7 @@ host only; no device call is made here)."""
 import ctypes
 import os
 import re
+import subprocess
 import tempfile
 
 import pytest
@@ -138,3 +139,57 @@ def test_missing_initial_solution_is_an_error():
         os.remove(os.path.join(d, "initial.inp"))
         with pytest.raises(piafs_amd.PiafsError):
             piafs_amd.estimate_bytes(d)
+
+
+def _c_blocks(md):
+    return re.findall(r"```c\n(.*?)```", md, re.S)
+
+
+def test_integration_binding_compiles(tmp_path):
+    """INTEGRATION.md's reference-side bindings (the RHS binding with its
+    interior gather/scatter and the device-buffer function-table wrappers)
+    compile with a C compiler against include/piafs_amd.h and link against the
+    library; the gather/scatter helpers are run on a 2D and a 1D ghosted
+    array.  The four HyPar fields they read (include/hypar.h of the reference)
+    are declared in a stand-in struct."""
+    md = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = [b for b in _c_blocks(md) if "GpuRHS" in b or "GpuFFunction" in b]
+    assert len(blocks) == 2
+    src = tmp_path / "binding.c"
+    main = r'''
+#include <stdio.h>
+#include <stdlib.h>
+static int check(int ndims, int nx, int ny) {
+  int dims[2] = {nx, ny}, g = 3, nv = 5;
+  HyPar s = {ndims, nv, g, dims};
+  long ng = (long)(nx + 2 * g) * (ndims > 1 ? ny + 2 * g : 1), ni = interior_points(&s);
+  double *u = calloc(ng * nv, sizeof(double)), *w = calloc(ng * nv, sizeof(double)), *in = malloc(ni * nv * sizeof(double));
+  for (long k = 0; k < ng * nv; k++) u[k] = (double)k;
+  gather_interior(u, in, &s);
+  scatter_interior(in, w, &s);
+  int bad = 0;
+  for (long k = 0; k < ng * nv; k++) {
+    long p = k / nv; int i = (int)(p % (nx + 2 * g)) - g, j = ndims > 1 ? (int)(p / (nx + 2 * g)) - g : 0;
+    int inside = i >= 0 && i < nx && j >= 0 && j < (ndims > 1 ? ny : 1);
+    bad += inside ? (w[k] != u[k]) : (w[k] != 0.0);
+  }
+  bad += in[0] != u[(ndims > 1 ? (long)g * (nx + 2 * g) + g : g) * nv];
+  free(u); free(w); free(in);
+  return bad;
+}
+int main(void) {
+  int bad = check(2, 7, 5) + check(1, 9, 1);
+  printf("%s\n", bad ? "MISMATCH" : "OK");
+  return bad != 0;
+}
+'''
+    stand_in = "typedef struct { int ndims, nvars, ghosts; int *dim_local; } HyPar;\n"
+    src.write_text(stand_in + blocks[0] + blocks[1] + main)
+    exe = tmp_path / "binding"
+    lib_dir = os.path.join(ROOT, "piafs_amd")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-Wno-unused-function", "-Wno-unused-variable",
+                        "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe), "-L", lib_dir,
+                        "-lpiafs_amd", "-Wl,-rpath," + lib_dir], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip() == "OK", (r.stdout, r.stderr)

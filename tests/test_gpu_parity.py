@@ -28,14 +28,17 @@ def _rel(a, b):
     return np.where(den > 0, num / scale, num)
 
 
-def test_tridiag_matches_oracle():
+@pytest.mark.parametrize("n,ns", [(257, 300), (6000, 7)])
+def test_tridiag_matches_oracle(n, ns):
+    """Team solves (n <= 4097) and the sequential solver of longer systems
+    against the oracle's tridiagLU."""
     rng = np.random.default_rng(11)
-    n, ns = 257, 300
     a = rng.uniform(-1, 1, (n, ns)); c = rng.uniform(-1, 1, (n, ns))
     b = 2.5 + rng.uniform(0, 1, (n, ns)); x = rng.uniform(-1, 1, (n, ns))
     a[0] = 0; c[-1] = 0
     rc, ref = orc.tridiag(a, b, c, x)
     got = piafs_amd.tridiag(a, b, c, x)
+    assert rc == 0
     np.testing.assert_allclose(got, ref, rtol=1e-13, atol=1e-14)
 
 
@@ -98,6 +101,7 @@ FIXTURES = [
     "ns2d_chem_crweno_16x4", "ns2d_chem_char_rf_24x4", "ns2d_crweno_char_flow_16x4",
     "sod_char_roe_rk1fe_N64", "sine_crweno_comp_rk22_N64", "ns2d_chem_crweno_rk33_16x12",
     "ns2d_chem_weno_rk22_12x16",
+    "sod_crweno_comp_N8193", "sod_crweno_comp_rf_N5000", "ns2d_chem_crweno_long_4200x8",
 ]
 
 

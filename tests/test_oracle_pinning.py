@@ -46,6 +46,7 @@ def _run_oracle(dk, n=None):
     "ns2d_chem_crweno_16x4", "ns2d_chem_char_rf_24x4", "ns2d_crweno_char_flow_16x4",
     "sod_char_roe_rk1fe_N64", "sine_crweno_comp_rk22_N64", "ns2d_chem_crweno_rk33_16x12",
     "ns2d_chem_weno_rk22_12x16",
+    "sod_crweno_comp_N8193", "sod_crweno_comp_rf_N5000", "ns2d_chem_crweno_long_4200x8",
 ])
 def test_oracle_matches_reference_bitwise(ref_runs, name):
     data, meta = ref_runs
