@@ -229,6 +229,16 @@ int  piafs_autotune(piafs_case *c, int *rows_per_lane, double *recon_ms);
  * x is overwritten with the solution.  Returns -1 on a zero pivot. */
 int  piafs_tridiag_host(const double *a, const double *b, const double *c, double *x, int n, int ns);
 
+/* The same solve without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * device copies of a, b, c, x and the zero-pivot flag live in a fixed device
+ * array of 4,194,304 doubles (32 MiB) that each device holds until the
+ * process ends: n*ns must not exceed 1,048,575 (returns 2 otherwise, having
+ * done nothing).  Every call on a device shares that array, so calls that
+ * overlap on one device race.  Returns as piafs_tridiag_host. */
+int  piafs_tridiag_host_noalloc(const double *a, const double *b, const double *c, double *x, int n, int ns);
+
 /* Nonlinear WENO5 weights for `count` stencils (five values each, ordered
  * m3 m2 m1 p1 p2).  kind: 0 JS, 1 mapped, 2 borges (Z), 3 YC.  Replaces the
  * weight macros of include/interpolation.h:255-490 as used by

@@ -64,8 +64,10 @@ EXPORTS = [
     "piafs_server_stats", "piafs_server_last_error", "piafs_boundary_fill", "piafs_flux",
     "piafs_interface_flux", "piafs_rhs_terms", "piafs_conservation_errors",
     "piafs_set_derivative_exchange", "piafs_dev_boundary_fill", "piafs_dev_flux", "piafs_dev_interp_interfaces",
-    "piafs_dev_upwind", "piafs_dev_first_derivative_par", "piafs_dev_weno_weights",
+    "piafs_dev_upwind", "piafs_dev_first_derivative_par", "piafs_dev_weno_weights", "piafs_tridiag_host_noalloc",
 ]
+# doubles in the device array of piafs_tridiag_host_noalloc (include/piafs_amd.h)
+NOALLOC_SCRATCH_DOUBLES = 1 << 22
 
 
 def lib():
@@ -105,6 +107,7 @@ def lib():
     L.piafs_solve.argtypes = [vp, cp, ip, dp]
     L.piafs_kernel_times.argtypes = [vp, ip, ctypes.POINTER(cp), dp, ctypes.POINTER(ctypes.c_int)]
     L.piafs_tridiag_host.argtypes = [vp, vp, vp, vp, ip, ip]
+    L.piafs_tridiag_host_noalloc.argtypes = [vp, vp, vp, vp, ip, ip]
     L.piafs_weno_weights_host.argtypes = [ip, vp, vp, ip, ctypes.c_double, vp]
     L.piafs_batch_create.argtypes = [ctypes.POINTER(cp), ip, ip, ctypes.POINTER(vp)]
     L.piafs_batch_destroy.argtypes = [vp]
