@@ -632,11 +632,50 @@ __device__ __forceinline__ Mob mob_mul(const Mob &x, const Mob &y) {
 // outer o inner
 __device__ __forceinline__ Aff aff_cmp(const Aff &o, const Aff &i) { return Aff{o.a * i.a, o.a * i.b + o.b}; }
 
+// Lane moves of a double by DPP (a VALU operand modifier, no LDS round trip):
+// lanes without a source (or outside ROWMASK) keep their own value.  CTRL:
+// row_shr:d 0x110+d and row_shl:d 0x100+d move within rows of 16 lanes,
+// wave_shr:1 0x138 and wave_shl:1 0x130 across the wave, row_bcast:15 0x142
+// and row_bcast:31 0x143 broadcast a row's last lane to the following rows.
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int l0 = (int)(unsigned)b, h0 = (int)(unsigned)(b >> 32);
+  const int lo = __builtin_amdgcn_update_dpp(l0, l0, CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(h0, h0, CTRL, ROWMASK, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ Mob dpp_mob(const Mob &x) {
+  return Mob{dpp_move<CTRL, ROWMASK>(x.a), dpp_move<CTRL, ROWMASK>(x.b), dpp_move<CTRL, ROWMASK>(x.c),
+             dpp_move<CTRL, ROWMASK>(x.d)};
+}
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ Aff dpp_aff(const Aff &x) {
+  return Aff{dpp_move<CTRL, ROWMASK>(x.a), dpp_move<CTRL, ROWMASK>(x.b)};
+}
+// value of lane i-1 (UP) or i+1 of the wave
+template <bool UP>
+__device__ __forceinline__ double lane_step(double v) {
+  return UP ? dpp_move<0x138>(v) : dpp_move<0x130>(v);
+}
+
 // Exclusive scans over the lanes of a team.  `lane` is the lane within the
 // team; `red` is LDS for one Mob per wave of the block.
 __device__ Mob team_scan_mob(Mob x, int lane, int T, double *red) {
   const int W = T < 64 ? T : 64;
   const int wl = lane & (W - 1);
+#if PIAFS_SCAN_DPP
+  // within rows of 16 lanes (a team of fewer lanes lies inside one row), then
+  // across the rows of the wave from each row's last lane
+  const int rl = wl & 15;
+  if (W > 1) { const Mob o = dpp_mob<0x111>(x); if (rl >= 1) x = mob_mul(x, o); }
+  if (W > 2) { const Mob o = dpp_mob<0x112>(x); if (rl >= 2) x = mob_mul(x, o); }
+  if (W > 4) { const Mob o = dpp_mob<0x114>(x); if (rl >= 4) x = mob_mul(x, o); }
+  if (W > 8) { const Mob o = dpp_mob<0x118>(x); if (rl >= 8) x = mob_mul(x, o); }
+  if (W > 16) { const Mob o = dpp_mob<0x142, 0xA>(x); if (wl & 16) x = mob_mul(x, o); }
+  if (W > 32) { const Mob o = dpp_mob<0x143, 0xC>(x); if (wl & 32) x = mob_mul(x, o); }
+#else
   for (int d = 1; d < W; d <<= 1) {
     Mob o;
     o.a = __shfl_up(x.a, d, W);
@@ -645,6 +684,7 @@ __device__ Mob team_scan_mob(Mob x, int lane, int T, double *red) {
     o.d = __shfl_up(x.d, d, W);
     if (wl >= d) x = mob_mul(x, o);
   }
+#endif
   Mob pre = {1.0, 0.0, 0.0, 1.0};
   if (T > 64) {
     const int w = lane >> 6, wave = threadIdx.x >> 6;
@@ -660,10 +700,14 @@ __device__ Mob team_scan_mob(Mob x, int lane, int T, double *red) {
     __syncthreads();
   }
   Mob e;
+#if PIAFS_SCAN_DPP
+  e = Mob{lane_step<true>(x.a), lane_step<true>(x.b), lane_step<true>(x.c), lane_step<true>(x.d)};
+#else
   e.a = __shfl_up(x.a, 1, W);
   e.b = __shfl_up(x.b, 1, W);
   e.c = __shfl_up(x.c, 1, W);
   e.d = __shfl_up(x.d, 1, W);
+#endif
   if (wl == 0) e = pre;
   else if (T > 64) e = mob_mul(e, pre);
   return e;
@@ -672,10 +716,20 @@ __device__ Mob team_scan_mob(Mob x, int lane, int T, double *red) {
 __device__ Aff team_scan_aff_fwd(Aff x, int lane, int T, double *red) {
   const int W = T < 64 ? T : 64;
   const int wl = lane & (W - 1);
+#if PIAFS_SCAN_DPP
+  const int rl = wl & 15;
+  if (W > 1) { const Aff o = dpp_aff<0x111>(x); if (rl >= 1) x = aff_cmp(x, o); }
+  if (W > 2) { const Aff o = dpp_aff<0x112>(x); if (rl >= 2) x = aff_cmp(x, o); }
+  if (W > 4) { const Aff o = dpp_aff<0x114>(x); if (rl >= 4) x = aff_cmp(x, o); }
+  if (W > 8) { const Aff o = dpp_aff<0x118>(x); if (rl >= 8) x = aff_cmp(x, o); }
+  if (W > 16) { const Aff o = dpp_aff<0x142, 0xA>(x); if (wl & 16) x = aff_cmp(x, o); }
+  if (W > 32) { const Aff o = dpp_aff<0x143, 0xC>(x); if (wl & 32) x = aff_cmp(x, o); }
+#else
   for (int d = 1; d < W; d <<= 1) {
     const Aff o = {__shfl_up(x.a, d, W), __shfl_up(x.b, d, W)};
     if (wl >= d) x = aff_cmp(x, o);
   }
+#endif
   Aff pre = {1.0, 0.0};
   if (T > 64) {
     const int w = lane >> 6, wave = threadIdx.x >> 6;
@@ -687,7 +741,11 @@ __device__ Aff team_scan_aff_fwd(Aff x, int lane, int T, double *red) {
     for (int k = 0; k < w; k++) pre = aff_cmp(Aff{red[2 * (wave - w + k)], red[2 * (wave - w + k) + 1]}, pre);
     __syncthreads();
   }
+#if PIAFS_SCAN_DPP
+  Aff e = {lane_step<true>(x.a), lane_step<true>(x.b)};
+#else
   Aff e = {__shfl_up(x.a, 1, W), __shfl_up(x.b, 1, W)};
+#endif
   if (wl == 0) e = pre;
   else if (T > 64) e = aff_cmp(e, pre);
   return e;
@@ -696,10 +754,29 @@ __device__ Aff team_scan_aff_fwd(Aff x, int lane, int T, double *red) {
 __device__ Aff team_scan_aff_bwd(Aff x, int lane, int T, double *red) {
   const int W = T < 64 ? T : 64;
   const int wl = lane & (W - 1);
+#if PIAFS_SCAN_DPP
+  // within rows of 16 lanes, then from the first lane of the following rows
+  // (no DPP broadcast runs backwards: two LDS permutes)
+  const int rl = wl & 15, rw = W < 16 ? W : 16;
+  if (W > 1) { const Aff o = dpp_aff<0x101>(x); if (rl + 1 < rw) x = aff_cmp(x, o); }
+  if (W > 2) { const Aff o = dpp_aff<0x102>(x); if (rl + 2 < rw) x = aff_cmp(x, o); }
+  if (W > 4) { const Aff o = dpp_aff<0x104>(x); if (rl + 4 < rw) x = aff_cmp(x, o); }
+  if (W > 8) { const Aff o = dpp_aff<0x108>(x); if (rl + 8 < rw) x = aff_cmp(x, o); }
+  if (W > 16) {
+    const int src = ((int)(threadIdx.x & 63) | 15) + 1;  // first lane of the next row
+    const Aff o = {__shfl(x.a, src & 63, 64), __shfl(x.b, src & 63, 64)};
+    if (!(wl & 16)) x = aff_cmp(x, o);
+  }
+  if (W > 32) {
+    const Aff o = {__shfl(x.a, 32, 64), __shfl(x.b, 32, 64)};
+    if (!(wl & 32)) x = aff_cmp(x, o);
+  }
+#else
   for (int d = 1; d < W; d <<= 1) {
     const Aff o = {__shfl_down(x.a, d, W), __shfl_down(x.b, d, W)};
     if (wl + d < W) x = aff_cmp(x, o);
   }
+#endif
   Aff suf = {1.0, 0.0};
   if (T > 64) {
     const int w = lane >> 6, nw = T >> 6, wave = threadIdx.x >> 6;
@@ -711,7 +788,11 @@ __device__ Aff team_scan_aff_bwd(Aff x, int lane, int T, double *red) {
     for (int k = nw - 1; k > w; k--) suf = aff_cmp(Aff{red[2 * (wave - w + k)], red[2 * (wave - w + k) + 1]}, suf);
     __syncthreads();
   }
+#if PIAFS_SCAN_DPP
+  Aff e = {lane_step<false>(x.a), lane_step<false>(x.b)};
+#else
   Aff e = {__shfl_down(x.a, 1, W), __shfl_down(x.b, 1, W)};
+#endif
   if (wl == W - 1) e = suf;
   else if (T > 64) e = aff_cmp(e, suf);
   return e;
@@ -723,6 +804,17 @@ __device__ __forceinline__ void team_shift2(double &a, double &b, int lane, int 
   const int W = T < 64 ? T : 64;
   const int wl = lane & (W - 1);
   double pa, pb;
+#if PIAFS_SCAN_DPP
+  // across team boundaries too: a team's first (last) lane receives another
+  // team's value, which it does not use
+  if (dir > 0) {
+    pa = lane_step<true>(a);
+    pb = lane_step<true>(b);
+  } else {
+    pa = lane_step<false>(a);
+    pb = lane_step<false>(b);
+  }
+#else
   if (dir > 0) {
     pa = __shfl_up(a, 1, W);
     pb = __shfl_up(b, 1, W);
@@ -730,6 +822,7 @@ __device__ __forceinline__ void team_shift2(double &a, double &b, int lane, int 
     pa = __shfl_down(a, 1, W);
     pb = __shfl_down(b, 1, W);
   }
+#endif
   if (T > 64) {
     const int wave = threadIdx.x >> 6;
     const bool edge_out = dir > 0 ? (wl == 63) : (wl == 0);
@@ -1875,56 +1968,117 @@ __global__ void __launch_bounds__(256) k_lsolve(CASE_ARGS) {
   double *L = C.lrow[dir];
   const long s0 = lv * 4 + 2 * q;  // the left-biased system; s0 + 1 the right-biased one
   double *A = L, *B = L + f, *Cc = L + 2 * f, *X = L + 3 * f;
-  if (C.R.crweno) {
-    // forward: b'_k = b_k - (a_k / b'_{k-1}) c_{k-1}, x'_k = x_k - (a_k / b'_{k-1}) x'_{k-1}
-    double bp[2], xp[2], cp[2];
-    bool zp = false;
-    for (int j = 0; j < 2; j++) {
-      bp[j] = B[s0 + j];
-      xp[j] = X[s0 + j];
-      cp[j] = Cc[s0 + j];
-      zp = zp || bp[j] == 0.0;
-    }
-    for (int k = 1; k <= N; k++) {
-      const long o = (long)k * ns4 + s0;
-      for (int j = 0; j < 2; j++) {
-        const double fac = A[o + j] / bp[j];
-        bp[j] = B[o + j] - fac * cp[j];
-        xp[j] = X[o + j] - fac * xp[j];
-        cp[j] = Cc[o + j];
-        zp = zp || bp[j] == 0.0;
-        B[o + j] = bp[j];
-        X[o + j] = xp[j];
-      }
-    }
-    if (zp) atomicOr(C.status, 1);
-    // back substitution: x_N = x'_N / b'_N, x_k = (x'_k - c_k x_{k+1}) / b'_k
-    double xn[2];
-    for (int j = 0; j < 2; j++) {
-      const long o = (long)N * ns4 + s0 + j;
-      xn[j] = X[o] / B[o];
-      X[o] = xn[j];
-    }
-    for (int k = N - 1; k >= 0; k--) {
-      const long o = (long)k * ns4 + s0;
-      for (int j = 0; j < 2; j++) {
-        xn[j] = (X[o + j] - Cc[o + j] * xn[j]) / B[o + j];
-        X[o + j] = xn[j];
-      }
-    }
-  }
-  // interface arrays: split mode uL, uR (q 0) or fL, fR (q 1); paired mode
-  // 0.5*(uR - uL) or 0.5*(fL + fR)
   double *const *io = dir ? C.iy : C.ix;
   const long line = ((long)v * S.nlines + l) * (N + 1);
-  for (int k = 0; k <= N; k++) {
-    const long o = (long)k * ns4 + s0;
-    const double xl = X[o], xr = X[o + 1];
+  // the interface values of row k: split mode uL, uR (q 0) or fL, fR (q 1);
+  // paired mode 0.5*(uR - uL) or 0.5*(fL + fR)
+  auto put = [&](int k, double xl, double xr) {
     if (C.paired) io[q][line + k] = q ? 0.5 * (xl + xr) : 0.5 * (xr - xl);
     else {
       io[2 * q][line + k] = xl;
       io[2 * q + 1][line + k] = xr;
     }
+  };
+  // Rows go through in chunks of kLChunk, the loads of the next chunk issued
+  // before the elimination chain of the current one, so that memory latency
+  // hides behind the chain.
+  constexpr int kLChunk = 8;
+  if (!C.R.crweno) {
+    for (int k0 = 0; k0 <= N; k0 += kLChunk) {
+      double x[kLChunk][2];
+#pragma unroll
+      for (int i = 0; i < kLChunk; i++)
+        if (k0 + i <= N)
+#pragma unroll
+          for (int j = 0; j < 2; j++) x[i][j] = X[(long)(k0 + i) * ns4 + s0 + j];
+#pragma unroll
+      for (int i = 0; i < kLChunk; i++)
+        if (k0 + i <= N) put(k0 + i, x[i][0], x[i][1]);
+    }
+    return;
+  }
+  // forward: b'_k = b_k - (a_k / b'_{k-1}) c_{k-1}, x'_k = x_k - (a_k / b'_{k-1}) x'_{k-1}
+  double bp[2], xp[2], cp[2];
+  bool zp = false;
+  for (int j = 0; j < 2; j++) {
+    bp[j] = B[s0 + j];
+    xp[j] = X[s0 + j];
+    cp[j] = Cc[s0 + j];
+    zp = zp || bp[j] == 0.0;
+  }
+  double ra[2][kLChunk][2], rb[2][kLChunk][2], rc[2][kLChunk][2], rx[2][kLChunk][2];
+  auto load_fwd = [&](int k0, int h) {
+#pragma unroll
+    for (int i = 0; i < kLChunk; i++)
+      if (k0 + i <= N)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          const long o = (long)(k0 + i) * ns4 + s0 + j;
+          ra[h][i][j] = A[o];
+          rb[h][i][j] = B[o];
+          rc[h][i][j] = Cc[o];
+          rx[h][i][j] = X[o];
+        }
+  };
+  auto run_fwd = [&](int k0, int h) {
+#pragma unroll
+    for (int i = 0; i < kLChunk; i++)
+      if (k0 + i <= N)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          const long o = (long)(k0 + i) * ns4 + s0 + j;
+          const double fac = ra[h][i][j] / bp[j];
+          bp[j] = rb[h][i][j] - fac * cp[j];
+          xp[j] = rx[h][i][j] - fac * xp[j];
+          cp[j] = rc[h][i][j];
+          zp = zp || bp[j] == 0.0;
+          B[o] = bp[j];
+          X[o] = xp[j];
+        }
+  };
+  // two register buffers, h = 0 and 1 in turn (constant indices after unrolling)
+  load_fwd(1, 0);
+  for (int k0 = 1; k0 <= N; k0 += 2 * kLChunk) {
+    if (k0 + kLChunk <= N) load_fwd(k0 + kLChunk, 1);
+    run_fwd(k0, 0);
+    if (k0 + kLChunk > N) break;
+    if (k0 + 2 * kLChunk <= N) load_fwd(k0 + 2 * kLChunk, 0);
+    run_fwd(k0 + kLChunk, 1);
+  }
+  if (zp) atomicOr(C.status, 1);
+  // back substitution: x_N = x'_N / b'_N, x_k = (x'_k - c_k x_{k+1}) / b'_k,
+  // each value written to the interface arrays as it is found
+  double xn[2];
+  for (int j = 0; j < 2; j++) xn[j] = xp[j] / bp[j];
+  put(N, xn[0], xn[1]);
+  auto load_bwd = [&](int k1, int h) {
+#pragma unroll
+    for (int i = 0; i < kLChunk; i++)
+      if (k1 - i >= 0)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+          const long o = (long)(k1 - i) * ns4 + s0 + j;
+          rb[h][i][j] = B[o];
+          rc[h][i][j] = Cc[o];
+          rx[h][i][j] = X[o];
+        }
+  };
+  auto run_bwd = [&](int k1, int h) {
+#pragma unroll
+    for (int i = 0; i < kLChunk; i++)
+      if (k1 - i >= 0) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) xn[j] = (rx[h][i][j] - rc[h][i][j] * xn[j]) / rb[h][i][j];
+        put(k1 - i, xn[0], xn[1]);
+      }
+  };
+  load_bwd(N - 1, 0);
+  for (int k1 = N - 1; k1 >= 0; k1 -= 2 * kLChunk) {
+    if (k1 - kLChunk >= 0) load_bwd(k1 - kLChunk, 1);
+    run_bwd(k1, 0);
+    if (k1 - kLChunk < 0) break;
+    if (k1 - 2 * kLChunk >= 0) load_bwd(k1 - 2 * kLChunk, 0);
+    run_bwd(k1 - kLChunk, 1);
   }
 }
 
@@ -1989,7 +2143,7 @@ __device__ __forceinline__ void recon_body(CASE_ARGS) {
   }
 }
 __global__ void __launch_bounds__(256) k_recon(CASE_ARGS) { recon_body<false>(cases, blockcase, firstblock, stage); }
-__global__ void __launch_bounds__(256) k_recon_full(CASE_ARGS) { recon_body<true>(cases, blockcase, firstblock, stage); }
+__global__ void __launch_bounds__(256, PIAFS_RECONF_MINB) k_recon_full(CASE_ARGS) { recon_body<true>(cases, blockcase, firstblock, stage); }
 
 // ---------------------------------------------------------------------------
 // Upwinding.
@@ -2460,12 +2614,10 @@ __device__ __forceinline__ double sponge(const CaseDev &C, int i, int j, int v, 
   return s;
 }
 
-__device__ __forceinline__ void chem_source(const CaseDev &C, int stage, long p, double *udot, bool sp, int i,
-                                            int j) {
-  const Grid &G = C.G;
+// photon density reaching layer 0 at the stage time: the Gaussian pulse
+// (ChemistrySolve.c:36-43) times the beam map
+__device__ __forceinline__ double chem_nh0(const CaseDev &C, int stage, long p) {
   const PhysParams &P = C.P;
-  const double *U = C.U;
-  // photon density of the pulse at the stage time (ChemistrySolve.c:36-43)
   const double time = *C.time + C.c[stage] * C.dt;
   double hs = 0.0;
   if (time > P.t_start_norm) {
@@ -2473,33 +2625,52 @@ __device__ __forceinline__ void chem_source(const CaseDev &C, int stage, long p,
     const double tp = time - (P.t_start_norm + P.t_pulse_norm);
     hs = P.I0 * exp(-(tp * tp) / (2 * sigma * sigma));
   }
+  return (hs * C.imap[p]) / P.hnu_den;
+}
+
+// sources of layer iz at point p (chemistry.h:30-127): its seven species s[]
+// and the energy term, for photon density nh reaching the layer
+__device__ __forceinline__ void chem_layer(const CaseDev &C, int iz, long p, double nh, double s[7], double &src_e) {
+  const PhysParams &P = C.P;
+  const double *U = C.U;
+  const long pl = C.G.plane;
+  const int nfv = P.nflow + P.z_stride * iz;
+  const double nO2 = U[(long)(nfv + 0) * pl + p], nO3 = U[(long)(nfv + 1) * pl + p];
+  const double n1D = U[(long)(nfv + 2) * pl + p], n1Dg = U[(long)(nfv + 3) * pl + p];
+  const double n3Su = U[(long)(nfv + 4) * pl + p], n1Sg = U[(long)(nfv + 5) * pl + p];
+  const double nCO2 = U[(long)(nfv + 6) * pl + p];
+  s[0] = 0.0;
+  s[1] = -(P.k0a + P.k0b) * nh * nO3 - (P.k2a + P.k2b) * nO3 * n1D - (P.k3a + P.k3b) * nO3 * n1Sg -
+         P.k5 * nO3 * n3Su - P.k6 * n1Dg * nO3;
+  s[2] = P.k0a * nh * nO3 - (P.k1a + P.k1b) * n1D * nO2 - (P.k2a + P.k2b) * n1D * nO3 - P.k4 * n1D * nCO2;
+  s[3] = P.k0a * nh * nO3 + P.k5 * nO3 * n3Su - P.k6 * n1Dg * nO3;
+  s[4] = P.k2a * nO3 * n1D - P.k5 * nO3 * n3Su;
+  s[5] = P.k1a * n1D * nO2 - (P.k3a + P.k3b) * nO3 * n1Sg;
+  s[6] = 0.0;
+  src_e = ((P.q0a * P.k0a + P.q0b * P.k0b) * nh * nO3 + (P.q1a * P.k1a + P.q1b * P.k1b) * nO2 * n1D +
+           (P.q2a * P.k2a + P.q2b * P.k2b) * nO3 * n1D + (P.q3a * P.k3a + P.q3b * P.k3b) * nO3 * n1Sg +
+           P.q4 * P.k4 * n1D * nCO2 + P.q5 * P.k5 * nO3 * n3Su + P.q6 * P.k6 * n1Dg * nO3) /
+          P.ch_gm1;
+}
+
+// rhs += chemistry source, layer by layer (ChemistrySolve.c:45-110); the
+// energy takes the last layer's term
+__device__ __forceinline__ void chem_source(const CaseDev &C, int stage, long p, double *udot, bool sp, int i,
+                                            int j) {
+  const Grid &G = C.G;
+  const PhysParams &P = C.P;
   const long pl = G.plane;
-  double nh = (hs * C.imap[p]) / P.hnu_den;
+  double nh = chem_nh0(C, stage, p);
   double src_e = 0.0;
   for (int iz = 0; iz < P.nlayers; iz++) {
     const int nfv = P.nflow + P.z_stride * iz;
-    if (iz > 0) nh = nh * (1.0 - P.damp_coef * U[(long)(P.nflow + P.z_stride * (iz - 1) + 1) * pl + p]);
-    const double nO2 = U[(long)(nfv + 0) * pl + p], nO3 = U[(long)(nfv + 1) * pl + p];
-    const double n1D = U[(long)(nfv + 2) * pl + p], n1Dg = U[(long)(nfv + 3) * pl + p];
-    const double n3Su = U[(long)(nfv + 4) * pl + p], n1Sg = U[(long)(nfv + 5) * pl + p];
-    const double nCO2 = U[(long)(nfv + 6) * pl + p];
+    if (iz > 0) nh = nh * (1.0 - P.damp_coef * C.U[(long)(P.nflow + P.z_stride * (iz - 1) + 1) * pl + p]);
     double s[7];
-    s[0] = 0.0;
-    s[1] = -(P.k0a + P.k0b) * nh * nO3 - (P.k2a + P.k2b) * nO3 * n1D - (P.k3a + P.k3b) * nO3 * n1Sg -
-           P.k5 * nO3 * n3Su - P.k6 * n1Dg * nO3;
-    s[2] = P.k0a * nh * nO3 - (P.k1a + P.k1b) * n1D * nO2 - (P.k2a + P.k2b) * n1D * nO3 - P.k4 * n1D * nCO2;
-    s[3] = P.k0a * nh * nO3 + P.k5 * nO3 * n3Su - P.k6 * n1Dg * nO3;
-    s[4] = P.k2a * nO3 * n1D - P.k5 * nO3 * n3Su;
-    s[5] = P.k1a * n1D * nO2 - (P.k3a + P.k3b) * nO3 * n1Sg;
-    s[6] = 0.0;
+    chem_layer(C, iz, p, nh, s, src_e);
     for (int m = 0; m < 7; m++) {
       double *o = udot + (long)(nfv + m) * pl + p;
       *o = *o + (sp ? sponge(C, i, j, nfv + m, p, s[m]) : s[m]);
     }
-    src_e = ((P.q0a * P.k0a + P.q0b * P.k0b) * nh * nO3 + (P.q1a * P.k1a + P.q1b * P.k1b) * nO2 * n1D +
-             (P.q2a * P.k2a + P.q2b * P.k2b) * nO3 * n1D + (P.q3a * P.k3a + P.q3b * P.k3b) * nO3 * n1Sg +
-             P.q4 * P.k4 * n1D * nCO2 + P.q5 * P.k5 * nO3 * n3Su + P.q6 * P.k6 * n1Dg * nO3) /
-            P.ch_gm1;
   }
   double *o = udot + (long)(P.nflow - 1) * pl + p;
   *o = *o + (sp ? sponge(C, i, j, P.nflow - 1, p, src_e) : src_e);

@@ -64,6 +64,16 @@ constexpr int kMaxRows = PIAFS_MAX_ROWS;
 #ifndef PIAFS_RECON_FULL
 #define PIAFS_RECON_FULL 1
 #endif
+// Lane exchanges of the line-solve scans and relays by DPP (1) or by LDS
+// permutes (0).
+#ifndef PIAFS_SCAN_DPP
+#define PIAFS_SCAN_DPP 1
+#endif
+// blocks per CU that the register allocation of k_recon_full aims at (2: two
+// waves per SIMD, which the DPP exchanges need a few registers' spill for)
+#ifndef PIAFS_RECONF_MINB
+#define PIAFS_RECONF_MINB 2
+#endif
 // Team shape for lines of N points: the fewest lanes (a power of two) whose
 // segments of at most max_rows rows cover interior rows 1..N-1.  Returns
 // false when a line is too long for one block.
