@@ -279,8 +279,29 @@ int  piafs_batch_download(piafs_batch *b, int slot, double *u_host);
 int  piafs_batch_info(piafs_batch *b, int slot, piafs_info *info);
 const char *piafs_batch_last_error(const piafs_batch *b);
 
+/* Ensemble directory: replaces EnsembleSimulation::define and the ensemble
+   branch of main (EnsembleSimulationsDefine.cpp:31-119, main.cpp:92-111).
+   `dir` holds simulation.inp ("begin nsims N end"), one solver.inp whose
+   `size` (and `size_exact`) line lists one tuple per member, and per-member
+   files initial_<i>.inp, boundary_<i>.inp (else the shared boundary.inp) and
+   exact_<i>.inp, <i> zero-padded to floor(log10 N) + 1 digits
+   (InitialSolution.c:32-41, InitializeBoundaries.c:51-75).  Creates one batch
+   whose slot i is member i.  Returns 0; 1 when simulation.inp is absent or
+   malformed or a member fails to load -- the batch handle is still returned
+   so that piafs_batch_last_error can say why; destroy it. */
+int  piafs_ensemble_create(const char *dir, int device, piafs_batch **out);
+/* The reference's run of the ensemble (Solve.cpp:28-110, TimePrintStep.c:
+   19-75, SimulationWriteErrors.c:21-100): n_iter steps of every member,
+   writing op_<i>[_NNNNN], op_species_z<k>_<i>_[_NNNNN], errors_<i>.dat,
+   conservation_<i>.dat and function_counts_<i>.dat into dir.  The batch must
+   hold the ensemble as created (no member added or removed).  Returns 0, or
+   nonzero with piafs_batch_last_error set. */
+int  piafs_ensemble_solve(piafs_batch *b, const char *dir, int verbose, double *seconds);
+
 /* Device memory a case built from `dir` would hold (admission control for
- * serving).  Returns 0 on success. */
+ * serving); for an ensemble directory (simulation.inp) the sum over its
+ * members.  Parses the decks on the host only.  Returns 0 on success, 1 with
+ * piafs_last_error(NULL) saying why. */
 int  piafs_estimate_bytes(const char *dir, size_t *bytes);
 
 /* ---------------------------------------------------------------------------

@@ -65,6 +65,7 @@ EXPORTS = [
     "piafs_interface_flux", "piafs_rhs_terms", "piafs_conservation_errors",
     "piafs_set_derivative_exchange", "piafs_dev_boundary_fill", "piafs_dev_flux", "piafs_dev_interp_interfaces",
     "piafs_dev_upwind", "piafs_dev_first_derivative_par", "piafs_dev_weno_weights", "piafs_tridiag_host_noalloc",
+    "piafs_ensemble_create", "piafs_ensemble_solve",
 ]
 # doubles in the device array of piafs_tridiag_host_noalloc (include/piafs_amd.h)
 NOALLOC_SCRATCH_DOUBLES = 1 << 22
@@ -121,6 +122,8 @@ def lib():
     L.piafs_batch_info.argtypes = [vp, ip, ctypes.POINTER(Info)]
     L.piafs_batch_last_error.argtypes = [vp]
     L.piafs_batch_last_error.restype = cp
+    L.piafs_ensemble_create.argtypes = [cp, ip, ctypes.POINTER(vp)]
+    L.piafs_ensemble_solve.argtypes = [vp, cp, ip, dp]
     L.piafs_estimate_bytes.argtypes = [cp, ctypes.POINTER(ctypes.c_size_t)]
     L.piafs_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
     L.piafs_batch_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
@@ -364,6 +367,40 @@ class Batch:
         self.slots = list(range(len(directories)))
         _live.add(self)
 
+    @classmethod
+    def ensemble(cls, directory: str, device: int = 0) -> "Batch":
+        """The members of an ensemble directory (simulation.inp with nsims;
+        piafs_ensemble_create), slot i holding member i."""
+        L = lib()
+        self = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        rc = L.piafs_ensemble_create(str(directory).encode(), int(device), ctypes.byref(h))
+        self._h, self._L = h, L
+        if rc:
+            msg = L.piafs_batch_last_error(h).decode() if h else ""
+            self.close()
+            raise PiafsError(f"piafs_ensemble_create failed ({rc}): {msg}")
+        self.slots = []
+        k = 0
+        while True:
+            inf = Info()
+            if L.piafs_batch_info(h, k, ctypes.byref(inf)):
+                break
+            self.slots.append(k)
+            k += 1
+        _live.add(self)
+        return self
+
+    def solve(self, directory: str = None, verbose: bool = False) -> float:
+        """piafs_ensemble_solve: the reference's run of an ensemble created by
+        :meth:`ensemble`, files written into ``directory``; returns the
+        solver's wall seconds."""
+        secs = ctypes.c_double(0)
+        d = None if directory is None else str(directory).encode()
+        self._check(self._L.piafs_ensemble_solve(self._h, d, int(bool(verbose)), ctypes.byref(secs)),
+                    "ensemble solve")
+        return secs.value
+
     def _check(self, rc, what):
         if rc:
             raise PiafsError(f"{what} failed ({rc}): {self._L.piafs_batch_last_error(self._h).decode()}")
@@ -539,7 +576,8 @@ def weno_weights(kind: int, copt, stencils, eps: float = 1e-6) -> np.ndarray:
 
 def estimate_bytes(directory: str) -> int:
     n = ctypes.c_size_t(0)
-    rc = lib().piafs_estimate_bytes(directory.encode(), ctypes.byref(n))
+    L = lib()
+    rc = L.piafs_estimate_bytes(directory.encode(), ctypes.byref(n))
     if rc:
-        raise PiafsError(f"estimate_bytes failed ({rc})")
+        raise PiafsError(f"estimate_bytes failed ({rc}): {L.piafs_last_error(None).decode()}")
     return int(n.value)

@@ -71,7 +71,7 @@ int piafs_batch_create(const char *const *dirs, int ncases, int device, piafs_ba
 
 int piafs_batch_destroy(piafs_batch *b) {
   if (!b) return 0;
-  b->eng.sync();
+  if (b->eng.stream) b->eng.sync();  // an engine that never started has nothing to wait for
   b->eng.cases.clear();
   for (Case *c : b->slots) delete c;
   delete b;
@@ -171,5 +171,69 @@ int piafs_batch_steps_done(piafs_batch *b, int k, int *steps) {
 }
 
 const char *piafs_batch_last_error(const piafs_batch *b) { return b ? b->err.c_str() : ""; }
+
+int piafs_ensemble_create(const char *dir, int device, piafs_batch **out) {
+  if (!out) return 1;
+  *out = nullptr;
+  const std::string d = dir ? dir : "";
+  int nsims = 0;
+  std::string err;
+  const int r = read_ensemble(d, nsims, err);
+  piafs_batch *b = new piafs_batch();
+  if (r) {
+    b->err = r == 1 ? "simulation.inp not found" : err;
+    *out = b;  // handle returned so that the error can be read; destroy it
+    return 1;
+  }
+  b->eng.owns_cases = false;
+  b->eng.use_graph = true;
+  int rc = b->eng.init(device);
+  if (rc) {
+    b->err = b->eng.err;
+    *out = b;
+    return rc;
+  }
+  for (int k = 0; k < nsims; k++) {  // members in index order: slot k is member k
+    Case *c = new Case();
+    c->cfg.member = k;
+    c->cfg.nsims = nsims;
+    rc = c->init(d, device, b->eng.stream);
+    if (rc) {
+      b->err = "domain " + std::to_string(k) + ": " + c->err;
+      delete c;
+      for (Case *x : b->slots) delete x;
+      b->slots.clear();
+      b->steps.clear();
+      *out = b;
+      return rc;
+    }
+    b->slots.push_back(c);
+    b->steps.push_back(0);
+  }
+  rc = b->rebuild();
+  *out = b;
+  return rc;
+}
+
+int piafs_ensemble_solve(piafs_batch *b, const char *dir, int verbose, double *seconds) {
+  if (!b) return 1;  // no batch
+  std::vector<Case *> cs;
+  for (Case *c : b->slots)
+    if (c) cs.push_back(c);
+  if (cs.empty()) {
+    b->err = "the batch has no members";
+    return 1;
+  }
+  for (size_t k = 0; k < cs.size(); k++)
+    if (cs[k]->cfg.nsims != (int)cs.size() || cs[k]->cfg.member != (int)k) {
+      b->err = "the batch is not the ensemble of one directory (members added or removed)";
+      return 1;
+    }
+  const int rc = run_solve(b->eng, cs, dir ? dir : "", verbose, seconds, b->err);
+  if (!rc)
+    for (size_t k = 0; k < b->slots.size(); k++)
+      if (b->slots[k]) b->steps[k] += cs[0]->cfg.n_iter - cs[0]->cfg.restart_iter;
+  return rc;
+}
 
 }  // extern "C"

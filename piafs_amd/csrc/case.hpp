@@ -118,4 +118,12 @@ struct piafs_case {
 
 namespace piafs {
 void fill_info(const Case *c, piafs_info *info);
+// One step of every member of `eng` (the members `cs`) with the largest CFL
+// number and the norm of the change over all of them; err on failure.
+int step_with_diagnostics(Engine &eng, const std::vector<Case *> &cs, double *cfl, double *norm,
+                          std::string &err);
+// The reference's run (n_iter steps, screen and file output, errors and the
+// run files) over the members `cs` of `eng`, in the directory d.
+int run_solve(Engine &eng, const std::vector<Case *> &cs, const std::string &d, int verbose, double *seconds,
+              std::string &err);
 }  // namespace piafs

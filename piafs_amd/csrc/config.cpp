@@ -109,7 +109,40 @@ static int read_array(const Config &c, const std::string &dir, const char *root,
 }
 
 int read_exact(const Config &c, const std::string &dir, std::vector<double> &u, bool &found, std::string &err) {
-  return read_array(c, dir, "exact", c.nvars, nullptr, u, found, err);
+  // exact[_<member>] (CalculateError.c:45-51)
+  return read_array(c, dir, ("exact" + c.tag()).c_str(), c.nvars, nullptr, u, found, err);
+}
+
+std::string Config::tag() const {
+  if (nsims <= 1) return std::string();
+  char b[32];
+  snprintf(b, sizeof b, "_%0*d", (int)log10((double)nsims) + 1, member);
+  return b;
+}
+
+int read_ensemble(const std::string &dir, int &nsims, std::string &err) {
+  nsims = -1;
+  const std::string fn = join(dir, "simulation.inp");
+  {
+    std::ifstream probe(fn);
+    if (!probe) return 1;
+  }
+  auto on_key = [&](const std::string &w, std::istream &s) -> bool {
+    if (w == "nsims") {
+      s >> nsims;
+      return true;
+    }
+    return false;
+  };
+  if (read_keywords(fn, on_key, err) != 2) {
+    if (err.empty()) err = "illegal format in simulation.inp";
+    return 2;
+  }
+  if (nsims < 1) {
+    err = "invalid value for nsims (" + std::to_string(nsims) + ") in simulation.inp";
+    return 2;
+  }
+  return 0;
 }
 
 int Config::parse(const std::string &dir, std::string &err) {
@@ -124,8 +157,15 @@ int Config::parse(const std::string &dir, std::string &err) {
     auto on_key = [&](const std::string &w, std::istream &s) -> bool {
       if (w == "ndims") s >> ndims;
       else if (w == "nvars") s >> nvars;
-      else if (w == "size") { for (int d = 0; d < ndims && d < 3; d++) s >> dim[d]; }
-      else if (w == "size_exact") { for (int d = 0; d < ndims && d < 3; d++) s >> dim_exact[d]; }
+      else if (w == "size" || w == "size_exact") {  // one tuple per member (ReadInputs.c:272-305)
+        int *to = (w == "size") ? dim : dim_exact;
+        for (int n = 0; n < nsims; n++)
+          for (int d = 0; d < ndims; d++) {
+            int v = 0;
+            s >> v;
+            if (n == member && d < 3) to[d] = v;
+          }
+      }
       else if (w == "ghost") s >> ghosts;
       else if (w == "n_iter") s >> n_iter;
       else if (w == "restart_iter") s >> restart_iter;
@@ -199,7 +239,7 @@ int Config::parse(const std::string &dir, std::string &err) {
   // ---- initial solution: InitialSolution.c:24-114 -------------------------
   {
     bool found = false;
-    if (read_array(*this, dir, "initial", nvars, &x, u0, found, err)) return 1;
+    if (read_array(*this, dir, ("initial" + tag()).c_str(), nvars, &x, u0, found, err)) return 1;
     if (!found) {
       err = "initial solution file not found";
       return 1;
@@ -217,9 +257,12 @@ int Config::parse(const std::string &dir, std::string &err) {
 
   // ---- boundary.inp: InitializeBoundaries.c:35-290 ------------------------
   {
-    std::ifstream in(join(dir, "boundary.inp"));
+    // boundary[_<member>].inp, else boundary.inp (InitializeBoundaries.c:51-75)
+    const std::string own = "boundary" + tag() + ".inp";
+    std::ifstream in(join(dir, own.c_str()));
+    if (!in && nsims > 1) in.open(join(dir, "boundary.inp"));
     if (!in) {
-      err = "boundary condition file boundary.inp not found";
+      err = "boundary condition file " + own + (nsims > 1 ? " or boundary.inp" : "") + " not found";
       return 1;
     }
     int nb = 0;

@@ -77,9 +77,21 @@ struct Config {
   // grid and data (ghosted x / dxinv, interior u point-major, ghosted imap)
   std::vector<double> x, dxinv, u0, imap;
   long npts_int = 0, npts_g = 0;
+  // member `member` of an ensemble of `nsims` simulations (simulation.inp,
+  // EnsembleSimulationsDefine.cpp:31-119); set before parse
+  int member = 0, nsims = 1;
 
   int parse(const std::string &dir, std::string &err);
+  // the member's file-name tag: "" for a single simulation, else "_" and the
+  // index zero-padded to floor(log10(nsims)) + 1 digits (GetStringFromInteger,
+  // InitialSolution.c:36-41)
+  std::string tag() const;
 };
+
+// simulation.inp of an ensemble directory: nsims (EnsembleSimulationsDefine.cpp:
+// 47-96).  Returns 0 and nsims >= 1, 1 when the file is absent (a single
+// simulation), 2 on a malformed file or nsims < 1 (err says why).
+int read_ensemble(const std::string &dir, int &nsims, std::string &err);
 
 // exact.inp (ExactSolution.c:17-60 with ReadArray): the interior solution,
 // point-major; found is false when the file is absent

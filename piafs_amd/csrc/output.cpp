@@ -143,7 +143,9 @@ int Case::write_species(const std::string &dir, const std::string &suffix, doubl
     }
     char idx[32];
     snprintf(idx, sizeof idx, "%0*d", width, iz);
-    std::string path = (dir.empty() ? std::string() : dir + "/") + "op_species_z" + idx + suffix +
+    // an ensemble member's root gains "_<member>_" (ChemistryWriteSpecies.c:44-50)
+    const std::string mem = cfg.nsims > 1 ? cfg.tag() + "_" : std::string();
+    std::string path = (dir.empty() ? std::string() : dir + "/") + "op_species_z" + idx + mem + suffix +
                        output_extension(cfg);
     std::string e;
     if ((rc = write_array(cfg, path, nsp, xi.data(), arr.data(), e))) return fail(rc, e);
@@ -233,8 +235,9 @@ int Case::conservation_errors(std::vector<double> &ce, hipStream_t st) {
 int Case::write_run_files(const std::string &dir, const double err3[3], long steps, double solver_rt,
                           double main_rt, hipStream_t st) {
   const std::string pre = dir.empty() ? std::string() : dir + "/";
-  FILE *f = fopen((pre + "errors.dat").c_str(), "w");
-  if (!f) return fail(2, "could not open errors.dat for writing");
+  const std::string t = cfg.tag();  // errors[_<member>].dat etc. (SimulationWriteErrors.c:39-61)
+  FILE *f = fopen((pre + "errors" + t + ".dat").c_str(), "w");
+  if (!f) return fail(2, "could not open errors" + t + ".dat for writing");
   for (int d = 0; d < cfg.ndims; d++) fprintf(f, "%4d ", cfg.dim[d]);
   for (int d = 0; d < cfg.ndims; d++) fprintf(f, "%4d ", 1);
   fprintf(f, "%1.16E  ", cfg.dt);
@@ -244,8 +247,8 @@ int Case::write_run_files(const std::string &dir, const double err3[3], long ste
   std::vector<double> ce;
   int rc = conservation_errors(ce, st);
   if (rc) return rc;
-  f = fopen((pre + "conservation.dat").c_str(), "w");
-  if (!f) return fail(2, "could not open conservation.dat for writing");
+  f = fopen((pre + "conservation" + t + ".dat").c_str(), "w");
+  if (!f) return fail(2, "could not open conservation" + t + ".dat for writing");
   for (int d = 0; d < cfg.ndims; d++) fprintf(f, "%4d ", cfg.dim[d]);
   for (int d = 0; d < cfg.ndims; d++) fprintf(f, "%4d ", 1);
   fprintf(f, "%1.16E  ", cfg.dt);
@@ -256,8 +259,8 @@ int Case::write_run_files(const std::string &dir, const double err3[3], long ste
   // the parabolic one counts only when viscous (NavierStokes2DParabolicFunction.c:64-65)
   const long evals = steps * cfg.nstages;
   const bool visc = (cfg.model_id == MODEL_NS2D && cfg.Re > 0);
-  f = fopen((pre + "function_counts.dat").c_str(), "w");
-  if (!f) return fail(2, "could not open function_counts.dat for writing");
+  f = fopen((pre + "function_counts" + t + ".dat").c_str(), "w");
+  if (!f) return fail(2, "could not open function_counts" + t + ".dat for writing");
   fprintf(f, "%d\n%ld\n%ld\n%ld\n", cfg.n_iter, evals, visc ? evals : 0L, evals);
   fclose(f);
   return 0;

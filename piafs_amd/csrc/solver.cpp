@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -604,7 +605,7 @@ int Engine::sync() {
 }
 
 Engine::~Engine() {
-  if (device >= 0) hipSetDevice(device);
+  if (device >= 0 && stream) hipSetDevice(device);
   if (stream) hipStreamSynchronize(stream);
   release_plan();
   for (auto &e : ev) {
@@ -618,6 +619,170 @@ Engine::~Engine() {
   if (fork.fork) hipEventDestroy(fork.fork);
   if (fork.join) hipEventDestroy(fork.join);
   if (stream) hipStreamDestroy(stream);
+}
+
+}  // namespace piafs
+
+namespace piafs {
+
+// One step of every member with the screen diagnostics of TimePreStep /
+// TimePostStep (TimePostStep.c:37-60): the largest CFL number of any member
+// and the norm of the change sqrt(sum over members of |u_new - u_old|^2 /
+// their total points).
+int step_with_diagnostics(Engine &eng, const std::vector<Case *> &cs, double *cfl, double *norm,
+                          std::string &err) {
+  hipStream_t st = eng.stream;
+  for (Case *c : cs)
+    if (hipMemcpyAsync(c->uprev, c->h.u, (size_t)c->cfg.nvars * c->h.G.plane * sizeof(double),
+                       hipMemcpyDeviceToDevice, st) != hipSuccess) {
+      err = "copy failed";
+      return -1;
+    }
+  int rc = eng.step(1);
+  if (rc) {
+    err = eng.err;
+    return rc;
+  }
+  double mx = 0, sm = 0, npts = 0;
+  for (Case *c : cs) {
+    std::vector<double> hbuf(2 * (size_t)c->nparts);
+    launch_cfl_norm(c->h.G, c->cfg.model_id, c->cfg.gamma, c->cfg.dt, c->h.u, c->uprev, c->h.dxinv, c->partials,
+                    c->nparts, st);
+    if (hipMemcpyAsync(hbuf.data(), c->partials, hbuf.size() * sizeof(double), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      err = "diagnostics failed";
+      return -1;
+    }
+    for (int b = 0; b < c->nparts; b++) {
+      mx = std::max(mx, hbuf[2 * b]);
+      sm += hbuf[2 * b + 1];
+    }
+    npts += (double)c->cfg.npts_int;
+  }
+  if (cfl) *cfl = mx;
+  if (norm) *norm = sqrt(sm / npts);
+  return 0;
+}
+
+// The reference's run (Solve.cpp:28-110 with TimePrintStep.c:19-75, then
+// SimWriteErrors) over the members of one engine: a single simulation, or the
+// nsims members of an ensemble directory, which share solver.inp's steps, dt
+// and output schedule (ReadInputs.c:259-445) and write their files under
+// their member tags.
+int run_solve(Engine &eng, const std::vector<Case *> &cs, const std::string &d, int verbose, double *seconds,
+              std::string &err) {
+  const auto t_main = std::chrono::steady_clock::now();
+  const Config &cf = cs[0]->cfg;
+  const int nsims = (int)cs.size();
+  const std::string ext = output_extension(cf);
+  const bool write = !ext.empty();
+  const std::string pre = d.empty() ? std::string() : d + "/";
+  auto fail_with = [&](Case *c, int rc) {
+    err = (c && !c->err.empty()) ? c->err : eng.err;
+    return rc;
+  };
+  // op[_<member>][_NNNNN]: the index advances after every write when
+  // op_overwrite is "no" (OutputSolution.cpp:42-62); a restart fast-forwards
+  // it (InitializeSolvers.c:240-245)
+  int index = 0;
+  if (cf.op_overwrite == "no" && cf.restart_iter)
+    for (int it = 0; it < cf.restart_iter; it++)
+      if ((it + 1) % cf.file_op_iter == 0) index++;
+  auto suffix = [&]() {
+    if (cf.op_overwrite != "no") return std::string();
+    char b[16];
+    snprintf(b, sizeof b, "_%05d", index);
+    return std::string(b);
+  };
+  // time of each member's latest right-hand-side evaluation (none yet)
+  std::vector<double> stage_time(nsims, -1.0);
+  auto output = [&]() -> int {  // PhysicsOutput of every member, then OutputSolution (Solve.cpp:95-107)
+    if (!write) return 0;
+    for (int k = 0; k < nsims; k++)
+      if (cs[k]->cfg.species_output && cs[k]->write_species(d, suffix(), stage_time[k], eng.stream))
+        return fail_with(cs[k], 2);
+    for (Case *c : cs) {
+      std::string fn = pre + "op" + c->cfg.tag() + suffix() + ext;
+      if (verbose) printf("Writing solution file %s.\n", fn.c_str());
+      if (c->write_solution(fn, eng.stream)) return fail_with(c, 2);
+    }
+    if (cf.op_overwrite == "no") index++;
+    return 0;
+  };
+  auto t0 = std::chrono::steady_clock::now();
+  long steps = 0;
+  for (int iter = cf.restart_iter; iter < cf.n_iter; iter++) {
+    if (iter == 0 && output()) return 2;
+    std::vector<double> t_start(nsims);
+    for (int k = 0; k < nsims; k++) t_start[k] = cs[k]->t;
+    const bool screen = ((iter + 1) % cf.screen_op_iter == 0);
+    const auto t_it = std::chrono::steady_clock::now();
+    double cfl = 0, norm = 0;
+    int rc = 0;
+    if (screen && verbose) rc = step_with_diagnostics(eng, cs, &cfl, &norm, err);
+    else if ((rc = eng.step(1))) err = eng.err;
+    if (rc) return rc;
+    steps++;
+    for (int k = 0; k < nsims; k++) stage_time[k] = t_start[k] + cs[k]->cfg.c[cs[k]->cfg.nstages - 1] * cs[k]->cfg.dt;
+    if (screen && verbose) {  // TimePrintStep; the wall time is the host's view of an asynchronous step
+      const double wc = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_it).count();
+      const double t = cs[0]->t;
+      if (nsims > 1) {
+        printf("--\n");
+        printf("iter=%7d,  t=%1.3e\n", iter + 1, t);
+        printf("  CFL=%1.3E\n", cfl);
+        printf("  norm=%1.4E\n", norm);
+        printf("  wctime=%1.1E (s)\n", wc);
+      } else {
+        printf("iter=%7d  t=%1.3E  CFL=%1.3E  norm=%1.4E  wctime: %1.1E (s)  ", iter + 1, t, cfl, norm, wc);
+      }
+      if (cs[0]->h.cons) {  // the conservation error over every member's variables
+        double e2 = 0;
+        for (Case *c : cs) {
+          std::vector<double> ce;
+          if (c->conservation_errors(ce, eng.stream)) return fail_with(c, -1);
+          for (double e : ce) e2 += e * e;
+        }
+        printf("  cons_err=%1.4E\n", sqrt(e2));
+      } else if (nsims == 1) {
+        printf("\n");
+      }
+      if (nsims > 1) printf("--\n\n");
+    }
+    if (((iter + 1) % cf.file_op_iter == 0) && ((iter + 1) < cf.n_iter) && output()) return 2;
+  }
+  if (eng.sync()) {
+    err = eng.err;
+    return -1;
+  }
+  double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (seconds) *seconds = secs;
+  // CalculateError of every member, then the final output (Solve.cpp:86-107),
+  // then SimWriteErrors (main.cpp)
+  std::vector<std::array<double, 3>> errs(nsims);
+  for (int k = 0; k < nsims; k++)
+    if (cs[k]->exact_errors(d, errs[k].data(), eng.stream)) return fail_with(cs[k], 1);
+  if (output()) return 2;
+  double main_secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_main).count();
+  for (int k = 0; k < nsims; k++)
+    if (cs[k]->write_run_files(d, errs[k].data(), steps, secs, main_secs, eng.stream)) return fail_with(cs[k], 2);
+  for (int k = 0; k < nsims; k++) {
+    Case *c = cs[k];
+    if (verbose && errs[k][0] >= 0) {
+      printf("Computed errors for domain %d:\n", k);
+      printf("  L1         Error           : %1.16E\n", errs[k][0]);
+      printf("  L2         Error           : %1.16E\n", errs[k][1]);
+      printf("  Linfinity  Error           : %1.16E\n", errs[k][2]);
+    }
+    if (verbose && c->h.cons) {
+      std::vector<double> ce;
+      if (c->conservation_errors(ce, eng.stream)) return fail_with(c, -1);
+      printf("Conservation Errors:\n");
+      for (double e : ce) printf("\t%1.16E\n", e);
+    }
+  }
+  return 0;
 }
 
 }  // namespace piafs
@@ -754,33 +919,10 @@ int piafs_sync(piafs_case *pc) {
 
 int piafs_step_diag(piafs_case *pc, int n, double *cfl, double *norm) {
   if (!pc || !pc->c) return 1;  // no case
-  Case *c = pc->c;
-  const Grid &G = c->h.G;
-  hipStream_t st = pc->eng.stream;
-  std::vector<double> hbuf(2 * (size_t)c->nparts);
+  std::vector<Case *> one{pc->c};
   for (int it = 0; it < n; it++) {
-    if (hipMemcpyAsync(c->uprev, c->h.u, (size_t)c->cfg.nvars * G.plane * sizeof(double), hipMemcpyDeviceToDevice,
-                       st) != hipSuccess) {
-      pc->err = "copy failed";
-      return -1;
-    }
-    int rc = pc->eng.step(1);
-    if (rc) return case_fail(pc, rc);
-    launch_cfl_norm(G, c->cfg.model_id, c->cfg.gamma, c->cfg.dt, c->h.u, c->uprev, c->h.dxinv, c->partials,
-                    c->nparts, st);
-    if (hipMemcpyAsync(hbuf.data(), c->partials, hbuf.size() * sizeof(double), hipMemcpyDeviceToHost, st) !=
-            hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
-      pc->err = "diagnostics failed";
-      return -1;
-    }
-    double mx = 0, sm = 0;
-    for (int b = 0; b < c->nparts; b++) {
-      mx = std::max(mx, hbuf[2 * b]);
-      sm += hbuf[2 * b + 1];
-    }
-    if (cfl) cfl[it] = mx;
-    if (norm) norm[it] = sqrt(sm / (double)c->cfg.npts_int);
+    int rc = step_with_diagnostics(pc->eng, one, cfl ? cfl + it : nullptr, norm ? norm + it : nullptr, pc->err);
+    if (rc) return rc;
   }
   return 0;
 }
@@ -1016,87 +1158,37 @@ int piafs_estimate_bytes(const char *dir, size_t *bytes) {
     g_create_error = "output pointer is NULL";
     return 1;
   }
-  Config cfg;
+  const std::string d = dir ? dir : "";
   std::string err;
-  if (cfg.parse(dir ? dir : "", err)) {
+  int nsims = 1;
+  const int ens = read_ensemble(d, nsims, err);  // 0: ensemble, 1: single simulation, 2: bad simulation.inp
+  if (ens == 2) {
     g_create_error = err;
     return 1;
   }
-  *bytes = Case::plan_bytes(cfg);
+  if (ens == 1) nsims = 1;
+  size_t total = 0;
+  for (int k = 0; k < nsims; k++) {  // an ensemble's members, each parsed as piafs_ensemble_create does
+    Config cfg;
+    if (ens == 0) {
+      cfg.member = k;
+      cfg.nsims = nsims;
+    }
+    if (cfg.parse(d, err)) {
+      g_create_error = ens == 0 ? "domain " + std::to_string(k) + ": " + err : err;
+      return 1;
+    }
+    total += Case::plan_bytes(cfg);
+  }
+  *bytes = total;
   return 0;
 }
 
 int piafs_solve(piafs_case *pc, const char *dir, int verbose, double *seconds) {
   if (!pc || !pc->c) return 1;  // no case
-  Case *c = pc->c;
-  const Config &cf = c->cfg;
-  const auto t_main = std::chrono::steady_clock::now();
-  std::string d = dir ? dir : "";
-  const std::string ext = output_extension(cf);
-  const bool write = !ext.empty();
-  // op[_NNNNN]: the index advances after every write when op_overwrite is "no"
-  // (OutputSolution.cpp:58-62); a restart fast-forwards it (InitializeSolvers.c:240-245)
-  int index = 0;
-  if (cf.op_overwrite == "no" && cf.restart_iter)
-    for (int it = 0; it < cf.restart_iter; it++)
-      if ((it + 1) % cf.file_op_iter == 0) index++;
-  auto suffix = [&]() {
-    if (cf.op_overwrite != "no") return std::string();
-    char b[16];
-    snprintf(b, sizeof b, "_%05d", index);
-    return std::string(b);
-  };
-  double stage_time = -1.0;  // time of the latest right-hand-side evaluation (none yet)
-  auto output = [&]() -> int {  // PhysicsOutput, then OutputSolution (Solve.cpp:95-107)
-    if (!write) return 0;
-    if (cf.species_output && c->write_species(d, suffix(), stage_time, pc->eng.stream)) return 2;
-    std::string fn = (d.empty() ? std::string() : d + "/") + "op" + suffix() + ext;
-    if (verbose) printf("Writing solution file %s.\n", fn.c_str());
-    if (c->write_solution(fn, pc->eng.stream)) return 2;
-    if (cf.op_overwrite == "no") index++;
-    return 0;
-  };
-  auto t0 = std::chrono::steady_clock::now();
-  std::vector<double> cfl(1), norm(1);
-  long steps = 0;
-  for (int iter = cf.restart_iter; iter < cf.n_iter; iter++) {
-    if (iter == 0 && output()) return case_fail(pc, 2);
-    const double t_start = c->t;
-    bool screen = ((iter + 1) % cf.screen_op_iter == 0);
-    const auto t_it = std::chrono::steady_clock::now();
-    int rc = screen && verbose ? piafs_step_diag(pc, 1, cfl.data(), norm.data()) : piafs_step(pc, 1);
-    if (rc) return rc;
-    steps++;
-    stage_time = t_start + cf.c[cf.nstages - 1] * cf.dt;
-    if (screen && verbose) {  // TimePrintStep's line; the wall time is the host's view of an asynchronous step
-      const double wc = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_it).count();
-      printf("iter=%7d  t=%1.3E  CFL=%1.3E  norm=%1.4E  wctime: %1.1E (s)  \n", iter + 1, c->t, cfl[0], norm[0], wc);
-    }
-    if (((iter + 1) % cf.file_op_iter == 0) && ((iter + 1) < cf.n_iter) && output()) return case_fail(pc, 2);
-  }
-  int rc = piafs_sync(pc);
-  if (rc) return rc;
-  double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  if (seconds) *seconds = secs;
-  // CalculateError, then the final output (Solve.cpp:86-107), then SimWriteErrors (main.cpp)
-  double err3[3];
-  if (c->exact_errors(d, err3, pc->eng.stream)) return case_fail(pc, 1);
-  if (output()) return case_fail(pc, 2);
-  double main_secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_main).count();
-  if (c->write_run_files(d, err3, steps, secs, main_secs, pc->eng.stream)) return case_fail(pc, 2);
-  if (verbose && err3[0] >= 0) {
-    printf("Computed errors for domain 0:\n");
-    printf("  L1         Error           : %1.16E\n", err3[0]);
-    printf("  L2         Error           : %1.16E\n", err3[1]);
-    printf("  Linfinity  Error           : %1.16E\n", err3[2]);
-  }
-  if (verbose && c->h.cons) {
-    std::vector<double> ce;
-    if (c->conservation_errors(ce, pc->eng.stream)) return case_fail(pc, -1);
-    printf("Conservation Errors:\n");
-    for (double e : ce) printf("\t%1.16E\n", e);
-  }
-  return 0;
+  std::vector<Case *> one{pc->c};
+  int rc = run_solve(pc->eng, one, dir ? dir : "", verbose, seconds, pc->err);
+  return rc;
 }
 
 }  // extern "C"

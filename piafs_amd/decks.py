@@ -852,6 +852,65 @@ class Member:
     steps: int
 
 
+def member_tag(k: int, nsims: int) -> str:
+    """File-name tag of member k of an ensemble of nsims ("" for one
+    simulation): "_" and k zero-padded to floor(log10 nsims) + 1 digits
+    (GetStringFromInteger as InitialSolution.c:36-41 calls it)."""
+    if nsims <= 1:
+        return ""
+    return "_" + str(k).zfill(int(math.log10(nsims)) + 1)
+
+
+def write_ensemble(d: str, members: list, shared_boundary: bool = True) -> str:
+    """The reference's ensemble directory (EnsembleSimulationsDefine.cpp:31-119)
+    for decks that differ only in grid size, initial solution and boundary
+    zones: simulation.inp (nsims), one solver.inp whose `size` lists every
+    member's sizes in turn (ReadInputs.c:272-290), the shared physics, weno
+    and chemistry files, initial_<i>.inp, and boundary_<i>.inp for each member
+    whose zones differ from member 0's (member 0's zones go to the shared
+    boundary.inp when `shared_boundary`, read as the fallback by
+    InitializeBoundaries.c:51-75)."""
+    if not members:
+        raise ValueError("an ensemble needs at least one member")
+    base = members[0]
+    n = len(members)
+    for m in members[1:]:
+        a = {k: v for k, v in m.solver.items() if k not in ("size", "size_exact")}
+        b = {k: v for k, v in base.solver.items() if k not in ("size", "size_exact")}
+        if a != b or m.physics != base.physics or m.weno != base.weno or m.chemistry != base.chemistry:
+            raise ValueError("ensemble members share solver.inp (but for size), physics, weno and chemistry")
+        if m.imap is not None:
+            raise ValueError("the reference reads one imap.inp for every member; build members without it")
+    os.makedirs(d, exist_ok=True)
+    _kw_file(os.path.join(d, "simulation.inp"), {"nsims": n})
+    solver = dict(base.solver)
+    solver["size"] = [v for m in members for v in m.dims]
+    _kw_file(os.path.join(d, "solver.inp"), solver)
+    _kw_file(os.path.join(d, "physics.inp"), base.physics)
+    if base.weno:
+        _kw_file(os.path.join(d, "weno.inp"), base.weno)
+    if base.chemistry is not None:
+        _kw_file(os.path.join(d, "chemistry.inp"), base.chemistry)
+
+    def zones(path, bl):
+        with open(path, "w") as f:
+            f.write(f"{len(bl)}\n")
+            for b in bl:
+                f.write(" ".join(str(t) for t in b) + "\n")
+
+    if shared_boundary:
+        zones(os.path.join(d, "boundary.inp"), base.boundary)
+    for k, m in enumerate(members):
+        t = member_tag(k, n)
+        if not shared_boundary or m.boundary != base.boundary:
+            zones(os.path.join(d, f"boundary{t}.inp"), m.boundary)
+        with open(os.path.join(d, f"initial{t}.inp"), "wb") as f:
+            for xd in m.x:
+                np.ascontiguousarray(xd, dtype=np.float64).tofile(f)
+            np.ascontiguousarray(m.u, dtype=np.float64).tofile(f)
+    return d
+
+
 def ensemble(n: int = 256, seed: int = 20261202, n_iter: int = 200, side=(128, 512),
              Re_range=(1e2, 1e4), amp_range=(0.5, 2.0)) -> list:
     """Independent 2D NS + photochemistry cases (case k drawn from SplitMix64

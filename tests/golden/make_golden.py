@@ -14,7 +14,8 @@ Two kinds of fixtures:
    LaxShockTube} and the error norms of
    Examples/2D_NavierStokes/InviscidVortexConvection/errors.dat.
 
-Usage: python tests/golden/make_golden.py [--ns2d-1024-full RUN_DIR | --ns2d-4096-nochem RUN_DIR]
+Usage: python tests/golden/make_golden.py [--ns2d-1024-full RUN_DIR | --ns2d-4096-nochem RUN_DIR |
+       --ensembles | ...]
 """
 from __future__ import annotations
 
@@ -371,6 +372,60 @@ def outputs():
         json.dump(meta, f, indent=1)
 
 
+# Ensemble directories (simulation.inp): name -> (member builders with their
+# arguments, changes to the shared solver.inp).  Members share every solver.inp
+# keyword but size; member 1 of the 1D ensemble has zones of its own
+# (boundary_1.inp), the others read the shared boundary.inp.
+ENSEMBLE_CASES = {
+    "ens_sod_crweno_comp_3": (
+        [("sod_1d", dict(N=64, n_iter=40, t_final=0.08, scheme="crweno5", interp="components")),
+         ("sod_1d", dict(N=96, n_iter=40, t_final=0.08, scheme="crweno5", interp="components",
+                         boundary=[("periodic", 0, 1, 0, 0), ("periodic", 0, -1, 0, 0)])),
+         ("sod_1d", dict(N=80, n_iter=40, t_final=0.08, scheme="crweno5", interp="components"))],
+        dict(op_file_format="binary", op_overwrite="no", file_op_iter=20, screen_op_iter=10)),
+    "ens_ns2d_chem_2": (
+        [("ns2d_photochem", dict(nx=24, ny=20, n_iter=6, with_imap=False, seed=11)),
+         ("ns2d_photochem", dict(nx=16, ny=24, n_iter=6, with_imap=False, seed=12))],
+        dict(op_file_format="binary", op_overwrite="yes", file_op_iter=60)),
+}
+
+
+def ensemble_members(name):
+    builders, over = ENSEMBLE_CASES[name]
+    out = []
+    for fn, kw in builders:
+        dk = getattr(decks, fn)(**kw)
+        dk.solver.update(over)
+        out.append(dk)
+    return out
+
+
+def ensembles():
+    """Run each ensemble directory through the reference and keep every file it writes."""
+    base = os.path.join(HERE, "ensembles")
+    meta = {}
+    for name in ENSEMBLE_CASES:
+        members = ensemble_members(name)
+        dst = os.path.join(base, name)
+        if os.path.isdir(dst):
+            shutil.rmtree(dst)
+        os.makedirs(dst)
+        with tempfile.TemporaryDirectory() as d:
+            decks.write_ensemble(d, members)
+            before = set(os.listdir(d))
+            r = subprocess.run([REF_BIN], cwd=d, capture_output=True, text=True)
+            if r.returncode:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            made = sorted(set(os.listdir(d)) - before)
+            for fn in made:
+                shutil.copy(os.path.join(d, fn), os.path.join(dst, fn))
+        builders, over = ENSEMBLE_CASES[name]
+        meta[name] = {"members": [[fn, kw] for fn, kw in builders], "solver": over, "files": made}
+        print(name, made)
+    with open(os.path.join(base, "ensembles.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
 # Shipped 2D examples: source directory, grid size, the input files, and the
 # changes made to solver.inp (a shorter n_iter so that the reference's serial
 # run takes seconds; the flat plate's `iproc 2 4`, which the serial reader
@@ -484,6 +539,8 @@ if __name__ == "__main__":
         ns2d_4096_nochem(sys.argv[2])
     elif len(sys.argv) > 1 and sys.argv[1] == "--ns2d-4096-chem-oracle":
         ns2d_4096_chem_oracle(int(sys.argv[2]) if len(sys.argv) > 2 else 2)
+    elif len(sys.argv) > 1 and sys.argv[1] == "--ensembles":
+        ensembles()
     elif len(sys.argv) > 1 and sys.argv[1] == "--outputs":
         outputs()
     elif len(sys.argv) > 1 and sys.argv[1] == "--add":

@@ -193,3 +193,73 @@ int main(void) {
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and r.stdout.strip() == "OK", (r.stdout, r.stderr)
+
+
+# --------------------------------------------------------------------------
+# ensemble directories (simulation.inp), parsed on the host
+# --------------------------------------------------------------------------
+def _ens_members():
+    return [decks.sod_1d(N=64, n_iter=10, t_final=0.02, scheme="crweno5", interp="components"),
+            decks.sod_1d(N=96, n_iter=10, t_final=0.02, scheme="crweno5", interp="components",
+                         boundary=[("periodic", 0, 1, 0, 0), ("periodic", 0, -1, 0, 0)]),
+            decks.sod_1d(N=80, n_iter=10, t_final=0.02, scheme="crweno5", interp="components")]
+
+
+def test_ensemble_directory_layout():
+    """write_ensemble lays out the reference's ensemble files
+    (EnsembleSimulationsDefine.cpp:31-119, ReadInputs.c:272-290,
+    InitialSolution.c:32-41, InitializeBoundaries.c:51-75)."""
+    ms = _ens_members()
+    with tempfile.TemporaryDirectory() as d:
+        decks.write_ensemble(d, ms)
+        assert sorted(os.listdir(d)) == ["boundary.inp", "boundary_1.inp", "initial_0.inp", "initial_1.inp",
+                                         "initial_2.inp", "physics.inp", "simulation.inp", "solver.inp",
+                                         "weno.inp"]
+        sim = open(os.path.join(d, "simulation.inp")).read().split()
+        assert sim == ["begin", "nsims", "3", "end"]
+        size = [ln.split()[1:] for ln in open(os.path.join(d, "solver.inp")) if ln.split()[:1] == ["size"]]
+        assert size == [["64", "96", "80"]]
+    assert [decks.member_tag(k, 12) for k in (0, 7, 11)] == ["_00", "_07", "_11"]
+    assert decks.member_tag(0, 1) == ""
+
+
+def test_ensemble_members_parse_as_their_own_decks():
+    """Each member of an ensemble directory reads its own size, initial
+    solution and zones: the ensemble's device memory is the sum of the
+    members' own decks' (piafs_estimate_bytes parses both on the host)."""
+    ms = _ens_members()
+    with tempfile.TemporaryDirectory() as d:
+        decks.write_ensemble(d, ms)
+        total = piafs_amd.estimate_bytes(d)
+    own = 0
+    for dk in ms:
+        with tempfile.TemporaryDirectory() as d:
+            dk.write(d)
+            own += piafs_amd.estimate_bytes(d)
+    assert total == own
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda d: os.remove(os.path.join(d, "initial_1.inp")), "domain 1: initial solution file not found"),
+    (lambda d: os.remove(os.path.join(d, "boundary.inp")), "domain 0: boundary condition file boundary_0.inp or boundary.inp not found"),
+    (lambda d: open(os.path.join(d, "simulation.inp"), "w").write("begin\n nsims 0\nend\n"), "invalid value for nsims (0)"),
+    (lambda d: open(os.path.join(d, "simulation.inp"), "w").write("nsims 3\n"), "illegal format in"),
+])
+def test_ensemble_errors_are_reported(mutate, msg):
+    with tempfile.TemporaryDirectory() as d:
+        decks.write_ensemble(d, _ens_members())
+        mutate(d)
+        with pytest.raises(piafs_amd.PiafsError) as e:
+            piafs_amd.estimate_bytes(d)
+        assert msg in str(e.value)
+
+
+def test_ensemble_create_without_simulation_inp_is_an_error():
+    """piafs_ensemble_create reads simulation.inp before touching a device."""
+    L = piafs_amd.lib()
+    with tempfile.TemporaryDirectory() as d:
+        decks.sod_1d(N=64, n_iter=2).write(d)
+        h = ctypes.c_void_p()
+        assert L.piafs_ensemble_create(d.encode(), 0, ctypes.byref(h)) == 1
+        assert L.piafs_batch_last_error(h).decode() == "simulation.inp not found"
+        assert L.piafs_batch_destroy(h) == 0
