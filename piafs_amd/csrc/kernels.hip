@@ -858,34 +858,21 @@ __device__ __forceinline__ double rcp_est(double b) {
   return r;
 }
 
-// Solve one team's system.  On entry the n rows of this lane hold
-// (A, B, Cc, X) = (sub-, main, super-diagonal, right-hand side); on exit X
-// holds the solution and B the pivots.  The row before the team's first row
-// is an identity row carrying xfirst; the row after the team's last row (held
-// by the lane with is_last) is one carrying xlast.  Every lane of every team
-// of the block must call this (it may synchronise the block).
-// The multipliers of the
-// pivot chain (fac = a_k / b'_{k-1}, one division each) are kept per row, so
-// the x' chain of a pass costs no division: every pass takes the segments'
-// entry states, runs the x' chain along the kept multipliers, and relays the
-// segment ends; only when an entry pivot changes are the multipliers derived
-// again.  Once no entry changes, one pass without divisions writes b' and x'
-// (fac * c and fac * x' exactly as the reference computes them).  The back
-// substitution iterates the same way from the top.  If `plain_start` is set
-// the scanned estimates are replaced by the trivial ones (b' = 1, x' = 0), so
-// that the relay runs through many passes (testing).  FULL: every lane holds
-// MR rows (n is not read), rows past the system's end being identity rows.
+// Forward elimination of team_solve: on exit B holds b' and X holds x' of
+// every row, Fk the multipliers.  The row before the team's first row has
+// pivot bfirst, super-diagonal cfirst and x' xfirst (an identity row: 1, 0,
+// its value).  zp is set when a pivot is zero.
 template <int MR, bool FULL = false>
-__device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], double (&Cc)[MR], double (&X)[MR],
-                                                int n, int lane, int T, bool is_last, double xfirst, double xlast,
-                                                double *red, int *status, bool plain_start) {
+__device__ __forceinline__ void team_forward(double (&A)[MR], double (&B)[MR], double (&Cc)[MR], double (&X)[MR],
+                                             double (&Fk)[MR], int n, int lane, int T, double bfirst, double cfirst,
+                                             double xfirst, double *red, bool plain_start, bool &zp) {
   const bool has = FULL || n > 0;
   double cin = 0.0, unused = 0.0;
 #pragma unroll
   for (int i = 0; i < MR; i++)
     if (FULL ? i == MR - 1 : i == n - 1) cin = Cc[i];
   team_shift2(cin, unused, lane, T, +1, red);
-  if (lane == 0) cin = 0.0;
+  if (lane == 0) cin = cfirst;
   // pivot entering the segment: scan of M_k = [[b_k, -a_k c_{k-1}], [1, 0]] on (b'_{k-1}, 1)
   Mob P = {1.0, 0.0, 0.0, 1.0};
 #pragma unroll
@@ -898,9 +885,8 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
   }
   P = mob_norm(P);
   const Mob E = team_scan_mob(P, lane, T, red);
-  double bin = lane == 0 ? 1.0 : (plain_start ? 1.0 : (E.a + E.b) / (E.c + E.d));
+  double bin = lane == 0 ? bfirst : (plain_start ? 1.0 : (E.a + E.b) / (E.c + E.d));
   // multipliers along the estimated pivots; affine maps of x'
-  double Fk[MR];
   double bend = bin;
   Aff F = {lane == 0 ? 0.0 : 1.0, lane == 0 ? xfirst : 0.0};
   {
@@ -956,7 +942,7 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
     }
   }
   // b' and x' of every row, in the reference's arithmetic
-  bool zp = (bin == 0.0) && has;
+  zp = zp || ((bin == 0.0) && has);
   {
     double b = bin, x = xin, c = cin;
 #pragma unroll
@@ -971,7 +957,17 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
       }
     }
   }
-  // back substitution: estimate of x entering from above, then the relay
+}
+
+// Back substitution of team_solve: B holds b', X holds x' of every row; on
+// exit X holds the solution.  The row after the team's last row (held by the
+// lane with is_last) has solution xlast.  W is scratch.
+template <int MR, bool FULL = false>
+__device__ __forceinline__ void team_backward(double (&B)[MR], double (&Cc)[MR], double (&X)[MR], double (&W)[MR],
+                                              int n, int lane, int T, bool is_last, double xlast, double *red,
+                                              bool plain_start) {
+  const bool has = FULL || n > 0;
+  // estimate of x entering from above, then the relay
   Aff Gm = {is_last ? 0.0 : 1.0, is_last ? xlast : 0.0};
 #pragma unroll
   for (int i = MR - 1; i >= 0; i--) {
@@ -990,12 +986,12 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
       if ((FULL || i < n) && live) {
         const double nx = (X[i] - Cc[i] * xv) / B[i];
         // from the second pass on: a row that repeats its value leaves every row above unchanged
-        if (it > 0 && __double_as_longlong(nx) == __double_as_longlong(Fk[i])) live = false;
-        Fk[i] = nx;
+        if (it > 0 && __double_as_longlong(nx) == __double_as_longlong(W[i])) live = false;
+        W[i] = nx;
         xv = nx;
       }
     }
-    if (FULL || n > 0) xv = Fk[0];
+    if (FULL || n > 0) xv = W[0];
     team_shift2(xv, dummy, lane, T, -1, red);
     const bool changed = has && !is_last && !(xv == yin);
     if (!is_last) yin = xv;
@@ -1003,7 +999,34 @@ __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], dou
   }
 #pragma unroll
   for (int i = 0; i < MR; i++)
-    if (FULL || i < n) X[i] = Fk[i];
+    if (FULL || i < n) X[i] = W[i];
+}
+
+// Solve one team's system.  On entry the n rows of this lane hold
+// (A, B, Cc, X) = (sub-, main, super-diagonal, right-hand side); on exit X
+// holds the solution and B the pivots.  The row before the team's first row
+// is an identity row carrying xfirst; the row after the team's last row (held
+// by the lane with is_last) is one carrying xlast.  Every lane of every team
+// of the block must call this (it may synchronise the block).
+// The multipliers of the
+// pivot chain (fac = a_k / b'_{k-1}, one division each) are kept per row, so
+// the x' chain of a pass costs no division: every pass takes the segments'
+// entry states, runs the x' chain along the kept multipliers, and relays the
+// segment ends; only when an entry pivot changes are the multipliers derived
+// again.  Once no entry changes, one pass without divisions writes b' and x'
+// (fac * c and fac * x' exactly as the reference computes them).  The back
+// substitution iterates the same way from the top.  If `plain_start` is set
+// the scanned estimates are replaced by the trivial ones (b' = 1, x' = 0), so
+// that the relay runs through many passes (testing).  FULL: every lane holds
+// MR rows (n is not read), rows past the system's end being identity rows.
+template <int MR, bool FULL = false>
+__device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], double (&Cc)[MR], double (&X)[MR],
+                                                int n, int lane, int T, bool is_last, double xfirst, double xlast,
+                                                double *red, int *status, bool plain_start) {
+  double Fk[MR];
+  bool zp = false;
+  team_forward<MR, FULL>(A, B, Cc, X, Fk, n, lane, T, 1.0, 0.0, xfirst, red, plain_start, zp);
+  team_backward<MR, FULL>(B, Cc, X, Fk, n, lane, T, is_last, xlast, red, plain_start);
   if (zp) atomicOr(status, 1);
 }
 
@@ -1901,13 +1924,19 @@ __host__ __device__ inline int recon_sweep_blocks(const SweepGeom &S, int nv, in
 
 // ---------------------------------------------------------------------------
 // Component-wise reconstruction of lines longer than one team covers
-// (SweepGeom::longl): k_lrows assembles every row of every system, end rows
-// included, one thread per (interface, system), into the sweep's scratch;
-// k_lsolve then solves the two systems of a pair (uL and uR, or fL and fR of
-// one line and variable) one thread each pair, by the sequential Thomas sweep
-// of tridiagLU.c:133-150 and :243-255 over rows 0..N (the reference solves the
-// same N+1 rows, end rows being identity rows), and writes the interface
-// arrays as the team kernels do.
+// (SweepGeom::longl).  k_lrows assembles every row of every system, end rows
+// included, one thread per (system, interface), into the sweep's scratch,
+// system by system ([A|B|C|X][system][interface]).  k_lsolve gives each pair
+// of systems (uL and uR, or fL and fR of one line and variable) a block, whose
+// 256 lanes walk the line window by window (kMaxRows rows per lane, 4096 rows
+// a window): the forward elimination of each window (team_forward) starts
+// from the exact pivot, super-diagonal and x' of the previous window's last
+// row and stores b' and x'; the back substitution (team_backward) then runs
+// the windows from the last, each from the exact solution of the row after
+// it.  Every row goes through the reference's arithmetic (tridiagLU.c:133-150,
+// :243-255, over rows 0..N with identity end rows), so the result is the
+// sequential sweep's bit for bit.  The block then writes the interface arrays
+// as the team kernels do.
 // ---------------------------------------------------------------------------
 __host__ __device__ inline long long_rows(const SweepGeom &S, int nv) {
   return S.longl ? (long)(S.N + 1) * S.nlines * nv * 4 : 0;
@@ -1918,8 +1947,8 @@ __device__ __forceinline__ void lrow_at(const CaseDev &C, int dir, long t) {
   const SweepGeom &S = dir ? C.Sy : C.Sx;
   const int nv = C.G.nv, N = S.N;
   const long ns4 = (long)S.nlines * nv * 4;
-  const long sys = t % ns4;
-  const int k = (int)(t / ns4);
+  const long sys = t / (N + 1);
+  const int k = (int)(t % (N + 1));
   const int r = (int)(sys % 4), v = (int)((sys / 4) % nv), l = (int)(sys / 4 / nv);
   const double *src = (r < 2 ? (dir ? C.UT : C.U) : (dir ? C.flux_y : C.flux_x)) + (long)v * S.plane + S.base +
                       (long)l * S.line_stride;
@@ -1927,7 +1956,7 @@ __device__ __forceinline__ void lrow_at(const CaseDev &C, int dir, long t) {
   row_coeffs<CR, KIND>(k, N, !(r & 1), C.R, src[k - 3], src[k - 2], src[k - 1], src[k], src[k + 1], src[k + 2], a,
                        b, c, x);
   double *L = C.lrow[dir];
-  const long f = (long)(N + 1) * ns4, o = (long)k * ns4 + sys;
+  const long f = (long)(N + 1) * ns4, o = sys * (N + 1) + k;
   L[o] = a;
   L[f + o] = b;
   L[2 * f + o] = c;
@@ -1951,15 +1980,99 @@ __global__ void __launch_bounds__(256) k_lrows(CASE_ARGS) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_lsolve(CASE_ARGS) {
+// pairs of systems of the long sweeps, one block each
+__host__ __device__ inline long long_pairs(const SweepGeom &S, int nv) {
+  return S.longl ? (long)S.nlines * nv * 2 : 0;
+}
+
+// One system of a long line, rows 0..N of the scratch (A, B, C, X at
+// interface k: L[arr*f + k]): rows 1..N-1 solved in windows of kLongWin
+// rows, the identity rows 0 and N holding the end values.  On exit X holds
+// the solution at every row.
+constexpr int kLongTeam = 256, kLongWin = kLongTeam * kMaxRows;
+__device__ void long_system(double *A, double *B, double *Cc, double *X, int N, double *red, double *carry,
+                            bool plain_start, bool &zp) {
+  const int lane = threadIdx.x;
+  const int nwin = (N - 1 + kLongWin - 1) / kLongWin;  // interior rows 1..N-1
+  auto window_rows = [&](int w, int &lo, int &n) {
+    lo = 1 + w * kLongWin + lane * kMaxRows;
+    n = N - lo;
+    n = n < 0 ? 0 : (n > kMaxRows ? kMaxRows : n);
+  };
+  // forward: window w enters with the exact b', c and x' of the row before it
+  double bfirst = 1.0, cfirst = 0.0, xfirst = X[0];  // row 0: identity, x' = its value
+  for (int w = 0; w < nwin; w++) {
+    int lo, n;
+    window_rows(w, lo, n);
+    double a[kMaxRows], b[kMaxRows], c[kMaxRows], x[kMaxRows], fk[kMaxRows];
+#pragma unroll
+    for (int i = 0; i < kMaxRows; i++) {
+      const bool in = i < n;
+      a[i] = in ? A[lo + i] : 0.0;
+      b[i] = in ? B[lo + i] : 1.0;
+      c[i] = in ? Cc[lo + i] : 0.0;
+      x[i] = in ? X[lo + i] : 0.0;
+    }
+    team_forward<kMaxRows>(a, b, c, x, fk, n, lane, kLongTeam, bfirst, cfirst, xfirst, red, plain_start, zp);
+#pragma unroll
+    for (int i = 0; i < kMaxRows; i++)
+      if (i < n) {
+        B[lo + i] = b[i];
+        X[lo + i] = x[i];
+      }
+    // the window's last row, from the lane that holds it, for the next window
+    const int last = (w + 1) * kLongWin < N - 1 ? (w + 1) * kLongWin : N - 1;  // row index
+    if (n > 0 && lo + n - 1 == last) {
+      carry[0] = b[n - 1];
+      carry[1] = c[n - 1];
+      carry[2] = x[n - 1];
+    }
+    __syncthreads();
+    bfirst = carry[0];
+    cfirst = carry[1];
+    xfirst = carry[2];
+    __syncthreads();
+  }
+  // back substitution: window w ends at the row whose solution the window
+  // after it (or row N, an identity row) has fixed
+  double xlast = X[N];
+  for (int w = nwin - 1; w >= 0; w--) {
+    int lo, n;
+    window_rows(w, lo, n);
+    double b[kMaxRows], c[kMaxRows], x[kMaxRows], wk[kMaxRows];
+#pragma unroll
+    for (int i = 0; i < kMaxRows; i++) {
+      const bool in = i < n;
+      b[i] = in ? B[lo + i] : 1.0;
+      c[i] = in ? Cc[lo + i] : 0.0;
+      x[i] = in ? X[lo + i] : 0.0;
+    }
+    const int last = (w + 1) * kLongWin < N - 1 ? (w + 1) * kLongWin : N - 1;
+    const bool is_last = n > 0 && lo + n - 1 == last;
+    team_backward<kMaxRows>(b, c, x, wk, n, lane, kLongTeam, is_last, xlast, red, plain_start);
+#pragma unroll
+    for (int i = 0; i < kMaxRows; i++)
+      if (i < n) X[lo + i] = x[i];
+    if (lane == 0) carry[0] = x[0];  // the window's first row
+    __syncthreads();
+    xlast = carry[0];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kLongTeam) k_lsolve(CASE_ARGS) {
   CASE_PREAMBLE
+  (void)t;
   if (stage >= C.nstages) return;
+  __shared__ double red[16];
+  __shared__ double carry[3];
   const int nv = C.G.nv;
-  const long nX = C.Sx.longl ? (long)C.Sx.nlines * nv * 2 : 0;
-  const long nY = (C.G.ndims == 2 && C.Sy.longl) ? (long)C.Sy.nlines * nv * 2 : 0;
-  if (t >= nX + nY) return;
-  const int dir = t < nX ? 0 : 1;
-  const long p = dir ? t - nX : t;  // pair (line*nv + var)*2 + q
+  const long nX = long_pairs(C.Sx, nv);
+  const long nY = C.G.ndims == 2 ? long_pairs(C.Sy, nv) : 0;
+  const long pair = blockIdx.x - firstblock[cidx];
+  if (pair >= nX + nY) return;  // uniform over the block
+  const int dir = pair < nX ? 0 : 1;
+  const long p = dir ? pair - nX : pair;  // (line*nv + var)*2 + q
   const SweepGeom &S = dir ? C.Sy : C.Sx;
   const int N = S.N, q = (int)(p % 2);
   const long lv = p / 2;  // line*nv + var
@@ -1967,118 +2080,25 @@ __global__ void __launch_bounds__(256) k_lsolve(CASE_ARGS) {
   const long ns4 = (long)S.nlines * nv * 4, f = (long)(N + 1) * ns4;
   double *L = C.lrow[dir];
   const long s0 = lv * 4 + 2 * q;  // the left-biased system; s0 + 1 the right-biased one
-  double *A = L, *B = L + f, *Cc = L + 2 * f, *X = L + 3 * f;
+  if (C.R.crweno) {
+    bool zp = false;
+    for (int j = 0; j < 2; j++) {
+      const long o = (s0 + j) * (N + 1);
+      long_system(L + o, L + f + o, L + 2 * f + o, L + 3 * f + o, N, red, carry, C.R.plain_start != 0, zp);
+    }
+    if (zp) atomicOr(C.status, 1);
+  }
+  // interface arrays: split mode uL, uR (q 0) or fL, fR (q 1); paired mode
+  // 0.5*(uR - uL) or 0.5*(fL + fR)
   double *const *io = dir ? C.iy : C.ix;
   const long line = ((long)v * S.nlines + l) * (N + 1);
-  // the interface values of row k: split mode uL, uR (q 0) or fL, fR (q 1);
-  // paired mode 0.5*(uR - uL) or 0.5*(fL + fR)
-  auto put = [&](int k, double xl, double xr) {
-    if (C.paired) io[q][line + k] = q ? 0.5 * (xl + xr) : 0.5 * (xr - xl);
+  const double *xl = L + 3 * f + s0 * (N + 1), *xr = xl + (N + 1);
+  for (int k = threadIdx.x; k <= N; k += kLongTeam) {
+    if (C.paired) io[q][line + k] = q ? 0.5 * (xl[k] + xr[k]) : 0.5 * (xr[k] - xl[k]);
     else {
-      io[2 * q][line + k] = xl;
-      io[2 * q + 1][line + k] = xr;
+      io[2 * q][line + k] = xl[k];
+      io[2 * q + 1][line + k] = xr[k];
     }
-  };
-  // Rows go through in chunks of kLChunk, the loads of the next chunk issued
-  // before the elimination chain of the current one, so that memory latency
-  // hides behind the chain.
-  constexpr int kLChunk = 8;
-  if (!C.R.crweno) {
-    for (int k0 = 0; k0 <= N; k0 += kLChunk) {
-      double x[kLChunk][2];
-#pragma unroll
-      for (int i = 0; i < kLChunk; i++)
-        if (k0 + i <= N)
-#pragma unroll
-          for (int j = 0; j < 2; j++) x[i][j] = X[(long)(k0 + i) * ns4 + s0 + j];
-#pragma unroll
-      for (int i = 0; i < kLChunk; i++)
-        if (k0 + i <= N) put(k0 + i, x[i][0], x[i][1]);
-    }
-    return;
-  }
-  // forward: b'_k = b_k - (a_k / b'_{k-1}) c_{k-1}, x'_k = x_k - (a_k / b'_{k-1}) x'_{k-1}
-  double bp[2], xp[2], cp[2];
-  bool zp = false;
-  for (int j = 0; j < 2; j++) {
-    bp[j] = B[s0 + j];
-    xp[j] = X[s0 + j];
-    cp[j] = Cc[s0 + j];
-    zp = zp || bp[j] == 0.0;
-  }
-  double ra[2][kLChunk][2], rb[2][kLChunk][2], rc[2][kLChunk][2], rx[2][kLChunk][2];
-  auto load_fwd = [&](int k0, int h) {
-#pragma unroll
-    for (int i = 0; i < kLChunk; i++)
-      if (k0 + i <= N)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          const long o = (long)(k0 + i) * ns4 + s0 + j;
-          ra[h][i][j] = A[o];
-          rb[h][i][j] = B[o];
-          rc[h][i][j] = Cc[o];
-          rx[h][i][j] = X[o];
-        }
-  };
-  auto run_fwd = [&](int k0, int h) {
-#pragma unroll
-    for (int i = 0; i < kLChunk; i++)
-      if (k0 + i <= N)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          const long o = (long)(k0 + i) * ns4 + s0 + j;
-          const double fac = ra[h][i][j] / bp[j];
-          bp[j] = rb[h][i][j] - fac * cp[j];
-          xp[j] = rx[h][i][j] - fac * xp[j];
-          cp[j] = rc[h][i][j];
-          zp = zp || bp[j] == 0.0;
-          B[o] = bp[j];
-          X[o] = xp[j];
-        }
-  };
-  // two register buffers, h = 0 and 1 in turn (constant indices after unrolling)
-  load_fwd(1, 0);
-  for (int k0 = 1; k0 <= N; k0 += 2 * kLChunk) {
-    if (k0 + kLChunk <= N) load_fwd(k0 + kLChunk, 1);
-    run_fwd(k0, 0);
-    if (k0 + kLChunk > N) break;
-    if (k0 + 2 * kLChunk <= N) load_fwd(k0 + 2 * kLChunk, 0);
-    run_fwd(k0 + kLChunk, 1);
-  }
-  if (zp) atomicOr(C.status, 1);
-  // back substitution: x_N = x'_N / b'_N, x_k = (x'_k - c_k x_{k+1}) / b'_k,
-  // each value written to the interface arrays as it is found
-  double xn[2];
-  for (int j = 0; j < 2; j++) xn[j] = xp[j] / bp[j];
-  put(N, xn[0], xn[1]);
-  auto load_bwd = [&](int k1, int h) {
-#pragma unroll
-    for (int i = 0; i < kLChunk; i++)
-      if (k1 - i >= 0)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-          const long o = (long)(k1 - i) * ns4 + s0 + j;
-          rb[h][i][j] = B[o];
-          rc[h][i][j] = Cc[o];
-          rx[h][i][j] = X[o];
-        }
-  };
-  auto run_bwd = [&](int k1, int h) {
-#pragma unroll
-    for (int i = 0; i < kLChunk; i++)
-      if (k1 - i >= 0) {
-#pragma unroll
-        for (int j = 0; j < 2; j++) xn[j] = (rx[h][i][j] - rc[h][i][j] * xn[j]) / rb[h][i][j];
-        put(k1 - i, xn[0], xn[1]);
-      }
-  };
-  load_bwd(N - 1, 0);
-  for (int k1 = N - 1; k1 >= 0; k1 -= 2 * kLChunk) {
-    if (k1 - kLChunk >= 0) load_bwd(k1 - kLChunk, 1);
-    run_bwd(k1, 0);
-    if (k1 - kLChunk < 0) break;
-    if (k1 - 2 * kLChunk >= 0) load_bwd(k1 - 2 * kLChunk, 0);
-    run_bwd(k1 - kLChunk, 1);
   }
 }
 
@@ -3139,8 +3159,7 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
     blocks[K_RECON] = full ? 0 : nrec;
     blocks[K_RECONF] = full ? nrec : 0;
     blocks[K_LROWS] = nb(long_rows(c.Sx, G.nv) + (two ? long_rows(c.Sy, G.nv) : 0));
-    blocks[K_LSOLVE] = nb((c.Sx.longl ? (long)c.Sx.nlines * G.nv * 2 : 0) +
-                          (two && c.Sy.longl ? (long)c.Sy.nlines * G.nv * 2 : 0));
+    blocks[K_LSOLVE] = (int)(long_pairs(c.Sx, G.nv) + (two ? long_pairs(c.Sy, G.nv) : 0));  // a block per pair
   }
   blocks[K_UPWIND] = nb((long)(c.Sx.N + 1) * c.Sx.nlines + (two ? (long)(c.Sy.N + 1) * c.Sy.nlines : 0));
   blocks[K_BFLUX] = c.cons ? 1 : 0;
