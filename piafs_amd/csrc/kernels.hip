@@ -1145,6 +1145,14 @@ __host__ __device__ inline int line_region(int T, int m) {
 // plus the overrun of the last lane's window when its segment is short
 constexpr int kLineLds = 256 * (16 + 5 + 2) + 32;
 
+// Block barrier that orders LDS accesses only: global loads and stores issued
+// before it stay in flight (a __syncthreads also waits for those).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // all lanes of the team have finished their LDS accesses before any lane goes on
 __device__ __forceinline__ void team_lds_sync(int T) {
   if (T > 64) {
@@ -2744,6 +2752,35 @@ __global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
     if (visc && i < G.nx && j < G.ny) viscous_par(C, i, j, (long)(j + G.gy) * G.nxg + (i + G.g), par[q]);
   }
   const double dxi = i < G.nx ? C.dxinv[G.g + i] : 0.0;
+#if PIAFS_ASM_PIPELINE
+  // y-sweep interfaces of lines i0..i0+31, interfaces j0..j0+32, of variable
+  // v: this thread's share (rows ty + 8q, interfaces tx and tx + 32), loaded
+  // one variable ahead so that the loads are in flight during the previous
+  // variable's work; the barriers order the LDS tile only, so that global
+  // loads and stores stay in flight across them
+  double fy[4][2];
+  auto load_y = [&](int v) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int ii = i0 + ty + 8 * q;
+      const double *row = C.fIy + ((long)v * G.nx + ii) * (G.ny + 1);
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        const int a = tx + 32 * k, j = j0 + a;
+        fy[q][k] = (ii < G.nx && a < 33 && j <= G.ny) ? row[j] : 0.0;
+      }
+    }
+  };
+  load_y(0);
+  for (int v = 0; v < nv; v++) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      tl[ty + 8 * q][tx] = fy[q][0];
+      if (tx == 0) tl[ty + 8 * q][32] = fy[q][1];
+    }
+    lds_barrier();
+    if (v + 1 < nv) load_y(v + 1);
+#else
   for (int v = 0; v < nv; v++) {
     // y-sweep interfaces of lines i0..i0+31, interfaces j0..j0+32
     for (int b = ty; b < 32; b += 8) {
@@ -2757,6 +2794,7 @@ __global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
       }
     }
     __syncthreads();
+#endif
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int jl = ty + 8 * q, j = j0 + jl;
@@ -2769,7 +2807,11 @@ __global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
         udot[(long)v * G.plane + (long)(j + G.gy) * G.nxg + (i + G.g)] = r;
       }
     }
+#if PIAFS_ASM_PIPELINE
+    lds_barrier();
+#else
     __syncthreads();
+#endif
   }
   if ((C.P.chem || C.sponge) && (C.terms & 4)) {
 #pragma unroll

@@ -69,6 +69,12 @@ constexpr int kMaxRows = PIAFS_MAX_ROWS;
 #ifndef PIAFS_SCAN_DPP
 #define PIAFS_SCAN_DPP 1
 #endif
+// k_assemble loads each variable's y-interface tile one variable ahead and
+// synchronises on the LDS tile only (1), or loads and waits variable by
+// variable (0).
+#ifndef PIAFS_ASM_PIPELINE
+#define PIAFS_ASM_PIPELINE 1
+#endif
 // blocks per CU that the register allocation of k_recon_full aims at (2: two
 // waves per SIMD, which the DPP exchanges need a few registers' spill for)
 #ifndef PIAFS_RECONF_MINB
