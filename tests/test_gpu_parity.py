@@ -392,6 +392,24 @@ def test_line_solve_relay_from_plain_start(ref_runs, name, monkeypatch):
     assert np.all(_rel(slow, data[name]) <= TOL_STEPS)
 
 
+@pytest.mark.parametrize("N", [4098, 4099, 8194])
+def test_long_line_window_edges_match_oracle(N):
+    """Lines just past one 4096-row window (4097 and 4098 interior rows: a
+    last window of one and of two rows, held by lane 0) and one row past two
+    windows, solved by the windowed team relay, against the oracle's
+    sequential sweep."""
+    dk = decks.sod_1d(N=N, n_iter=3, t_final=3 * 0.25 / N, scheme="crweno5", interp="components")
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        oc = orc.OCase(d)
+        oc.step(3)
+        ref = oc.solution()
+        with piafs_amd.Case(d) as c:
+            c.step(3)
+            got = c.solution()
+    assert np.all(_rel(got, ref) <= 1e-13), _rel(got, ref)
+
+
 def test_derivative_exchange_translation_invariant_and_matches_oracle():
     """The derivative-ghost exchange of a run decomposed along the periodic
     dimensions (piafs_set_derivative_exchange): the right-hand side of a
