@@ -1638,8 +1638,35 @@ __global__ void __launch_bounds__(256) k_cw_rows(CASE_ARGS) {
 
 // A species chain (chain >= 1): the scalar chain of the same arithmetic as the
 // flow block's, for species NF + chain - 1.
+// Segments of a block chain: rows 1..N of the forward elimination split into
+// `segs` runs of cw_seg_rows rows, each run owned by a lane (species) or a
+// quad of lanes (flow) of the same wave.  A run starts from an estimate of
+// the state entering it (its first row taken as if it were the system's
+// first), runs the reference's row steps, and hands its exit state on; a run
+// whose entry differs from its predecessor's exit runs again from that exit
+// until a row reproduces, bit for bit, what the previous pass stored there
+// (the chain depends on the past only through that row's stored values, so
+// every later row would repeat too).  Run 0 starts from row 0 exactly, so
+// the fixed point is the sequential chain's, bit for bit; the recurrences
+// contract, so a run of a few dozen rows forgets its estimate and one
+// correcting pass usually suffices.  The back substitution is relayed the
+// same way from the top.  segs = 1 is the sequential chain.
+#ifndef PIAFS_CW_RMIN
+#define PIAFS_CW_RMIN 128
+#endif
+#ifndef PIAFS_CW_LANES
+#define PIAFS_CW_LANES 65536
+#endif
+__host__ __device__ inline int cw_segs(const SweepGeom &S, int nv, int nf) {
+  const long lanes = 4L * S.nlines * (4 + nv - nf);  // lanes of the sweep's chains unsegmented
+  int s = 1;
+  while (s < 16 && 2 * s * PIAFS_CW_RMIN <= S.N + 1 && 2 * s * lanes <= PIAFS_CW_LANES) s <<= 1;
+  return s;
+}
+__host__ __device__ inline int cw_seg_rows(int N, int segs) { return (N + segs - 1) / segs; }
+
 __device__ __forceinline__ void cwc_species(const CaseDev &C, const SweepGeom &S, int dir, int l, int sys,
-                                           int chain, int NF) {
+                                           int chain, int NF, int seg, int segs) {
   const int nv = C.G.nv, nl = S.nlines, N = S.N;
   const long vs = (long)nl * (N + 1);
   const long bs = 4L * nl * (N + 1), bb = (long)sys * (N + 1) * nl + l;
@@ -1650,43 +1677,121 @@ __device__ __forceinline__ void cwc_species(const CaseDev &C, const SweepGeom &S
   auto bk = [&](int f, int k) -> double & { return back[(long)f * bs + bb + (long)k * nl]; };
   auto coef = [&](int v, int which, int k) { return rf(NF * NF + (sys * nv + v) * 4 + which, k); };
   const int m = NF + chain - 1, f0 = 2 * NF * NF + NF + 3 * (chain - 1);
-  double bp = 0.0, xp = 0.0, cprev = 0.0;
-  // each row's coefficients are loaded one row ahead, so that their latency overlaps the chain
-  double na = coef(m, 0, 0), nb = coef(m, 1, 0), nc = coef(m, 2, 0), nF = coef(m, 3, 0);
-  for (int k = 0; k <= N; k++) {
-    const double a = na, b = nb, c = nc, F = nF;
-    if (k < N) {
-      na = coef(m, 0, k + 1);
-      nb = coef(m, 1, k + 1);
-      nc = coef(m, 2, k + 1);
-      nF = coef(m, 3, k + 1);
-    }
-    if (k == 0) {
-      bp = b;
-      xp = F;
-    } else {
+  const int R = cw_seg_rows(N, segs);
+  const int k0 = 1 + seg * R, k1 = min(N + 1, k0 + R);  // forward rows of this run
+  const bool has = k0 < k1;
+  const int lane = threadIdx.x & 63;
+  // forward: rows k0..k1-1, row k storing (1/b'_{k-1}, c_{k-1}, x'_{k-1}) at k-1
+  double bp, xp, cprev;
+  {
+    const int ks = seg == 0 ? 0 : (has ? k0 - 1 : 0);
+    bp = coef(m, 1, ks);
+    xp = coef(m, 3, ks);
+    cprev = coef(m, 2, ks);
+  }
+  // one pass from the entry (bp, xp); early: stop where a row repeats its stored values
+  auto forward = [&](bool early) {
+    double a = has ? coef(m, 0, k0) : 0.0, b = has ? coef(m, 1, k0) : 0.0, c = has ? coef(m, 2, k0) : 0.0,
+           F = has ? coef(m, 3, k0) : 0.0;
+    // what the previous pass stored at k-1, loaded one row ahead like the coefficients
+    double srb = early && has ? bk(f0, k0 - 1) : 0.0, sxp = early && has ? bk(f0 + 2, k0 - 1) : 0.0;
+    for (int k = k0; k < k1; k++) {
+      const double na = k + 1 < k1 ? coef(m, 0, k + 1) : 0.0, nb = k + 1 < k1 ? coef(m, 1, k + 1) : 0.0,
+                   nc = k + 1 < k1 ? coef(m, 2, k + 1) : 0.0, nF = k + 1 < k1 ? coef(m, 3, k + 1) : 0.0;
+      const double crb = srb, cxp = sxp;
+      if (early && k + 1 < k1) {
+        srb = bk(f0, k);
+        sxp = bk(f0 + 2, k);
+      }
       const double rb = 1.0 / bp, fac = a * rb;
+      if (early && __double_as_longlong(rb) == __double_as_longlong(crb) &&
+          __double_as_longlong(xp) == __double_as_longlong(cxp))
+        return false;  // this row and every later one as stored
       bk(f0, k - 1) = rb;
       bk(f0 + 1, k - 1) = cprev;
       bk(f0 + 2, k - 1) = xp;
       bp = b - fac * cprev;
       xp = F - fac * xp;
+      cprev = c;
+      a = na; b = nb; c = nc; F = nF;
     }
-    cprev = c;
+    return true;
+  };
+  forward(false);
+  double ebp = bp, exp_ = xp;  // exit state
+  double nbp = bp, nxp = xp;   // the entry this run last started from (set below)
+  {
+    // the entry of the first pass, for the comparison with the predecessor's exit
+    const int ks = seg == 0 ? 0 : (has ? k0 - 1 : 0);
+    nbp = coef(m, 1, ks);
+    nxp = coef(m, 3, ks);
+    if (seg == 0) {  // run 0 starts at row 0: b'_0 = b_0, x'_0 = F_0
+      nbp = coef(m, 1, 0);
+      nxp = coef(m, 3, 0);
+    }
   }
-  double x = (1.0 / bp) * xp;
-  out[((long)m * nl + l) * (N + 1) + N] = x;
-  double nrb = N > 0 ? bk(f0, N - 1) : 0.0, ncp = N > 0 ? bk(f0 + 1, N - 1) : 0.0,
-         nxq = N > 0 ? bk(f0 + 2, N - 1) : 0.0;
-  for (int k = N - 1; k >= 0; k--) {
-    const double rb = nrb, c = ncp, xq = nxq;
-    if (k > 0) {
-      nrb = bk(f0, k - 1);
-      ncp = bk(f0 + 1, k - 1);
-      nxq = bk(f0 + 2, k - 1);
+  for (int pass = 0;; pass++) {
+    const double pbp = __shfl(ebp, lane - 1), pxp = __shfl(exp_, lane - 1);
+    const bool changed = seg > 0 && (__double_as_longlong(pbp) != __double_as_longlong(nbp) ||
+                                     __double_as_longlong(pxp) != __double_as_longlong(nxp));
+    if (!__any(changed) || pass > segs) break;
+    if (changed) {
+      nbp = pbp;
+      nxp = pxp;
+      if (!has) {
+        ebp = pbp;
+        exp_ = pxp;
+      } else {
+        bp = pbp;
+        xp = pxp;
+        cprev = coef(m, 2, k0 - 1);
+        if (forward(true)) {
+          ebp = bp;
+          exp_ = xp;
+        }
+      }
     }
-    x = rb * (xq - c * x);
-    out[((long)m * nl + l) * (N + 1) + k] = x;
+  }
+  // back substitution: rows k1-2 .. k0-1 of this run (the top run also row N),
+  // entering from the run above
+  const long ob = ((long)m * nl + l) * (N + 1);
+  const bool top = k1 == N + 1 && (has || N == 0);
+  double xin = 0.0;  // x of the row above this run's rows
+  if (top) {
+    xin = (1.0 / ebp) * exp_;
+    out[ob + N] = xin;
+  }
+  auto backward = [&](double x, bool early) {
+    int k = k1 - 2;
+    if (k < k0 - 1) return;
+    // each row's values loaded one row ahead
+    double nrb = bk(f0, k), ncp = bk(f0 + 1, k), nxq = bk(f0 + 2, k), nso = early ? out[ob + k] : 0.0;
+    for (; k >= k0 - 1; k--) {
+      const double rb = nrb, c = ncp, xq = nxq, so = nso;
+      if (k > k0 - 1) {
+        nrb = bk(f0, k - 1);
+        ncp = bk(f0 + 1, k - 1);
+        nxq = bk(f0 + 2, k - 1);
+        if (early) nso = out[ob + k - 1];
+      }
+      x = rb * (xq - c * x);
+      if (early && __double_as_longlong(x) == __double_as_longlong(so)) return;
+      out[ob + k] = x;
+    }
+  };
+  double xs = xin;  // the entry the last pass started from
+  if (has) backward(xin, false);
+  for (int pass = 0;; pass++) {
+    // the exit of a run is the x of its lowest row; an empty run passes its entry on
+    const double mine = has ? out[ob + k0 - 1] : xs;
+    const double above = __shfl(mine, lane + 1);
+    const bool last = seg == segs - 1 || top;
+    const bool changed = !last && __double_as_longlong(above) != __double_as_longlong(xs);
+    if (!__any(changed) || pass > segs) break;
+    if (changed) {
+      xs = above;
+      if (has) backward(above, true);
+    }
   }
 }
 
@@ -1759,8 +1864,15 @@ __device__ __forceinline__ void cw_inv_quad(const double (&bp)[NF], int ri, doub
   }
 }
 
+// all four lanes of the quad hold p
+__device__ __forceinline__ bool quad_all(bool p) {
+  const double d = p ? 1.0 : 0.0;
+  return quad_get<0>(d) * quad_get<1>(d) * quad_get<2>(d) * quad_get<3>(d) == 1.0;
+}
+
 template <int NF>
-__device__ __forceinline__ void cwc_flow_quad(const CaseDev &C, const SweepGeom &S, int dir, int l, int sys, int e) {
+__device__ __forceinline__ void cwc_flow_quad(const CaseDev &C, const SweepGeom &S, int dir, int l, int sys, int e,
+                                              int seg, int segs) {
   const int nv = C.G.nv, nl = S.nlines, N = S.N;
   const bool act = e < NF;
   const int ri = act ? e : NF - 1;  // the spare lane of a 1D quad reads row NF-1 and stores nothing
@@ -1772,37 +1884,78 @@ __device__ __forceinline__ void cwc_flow_quad(const CaseDev &C, const SweepGeom 
   auto rf = [&](int f, int k) { return rows[(long)f * vs + (long)k * nl + l]; };
   auto bk = [&](int f, int k) -> double & { return back[(long)f * bs + bb + (long)k * nl]; };
   auto coef = [&](int which, int k) { return rf(NF * NF + (sys * nv + ri) * 4 + which, k); };
-  // this lane's row of L and its coefficients, loaded one row ahead
-  double nL[NF], na, nb, nc, nF;
-  auto load_row = [&](int k) {
-#pragma unroll
-    for (int j = 0; j < NF; j++) nL[j] = rf(NF * ri + j, k);
-    na = coef(0, k);
-    nb = coef(1, k);
-    nc = coef(2, k);
-    nF = coef(3, k);
-  };
-  load_row(0);
-  double bp[NF], cprev[NF], xp = 0.0;
-#pragma unroll
-  for (int j = 0; j < NF; j++) bp[j] = cprev[j] = 0.0;
-  for (int k = 0; k <= N; k++) {
-    double A[NF], Bm[NF], Cm[NF];
-    const double a = na, b = nb, c = nc, F = nF;
+  const int R = cw_seg_rows(N, segs);
+  const int k0 = 1 + seg * R, k1 = min(N + 1, k0 + R);  // forward rows of this run
+  const bool has = k0 < k1;
+  const int lane = threadIdx.x & 63;
+  // this lane's row of the blocks of row k: A = a L, B = b L, C = c L, and F
+  auto row = [&](int k, double (&A)[NF], double (&Bm)[NF], double (&Cm)[NF], double &F) {
+    const double a = coef(0, k), b = coef(1, k), c = coef(2, k);
+    F = coef(3, k);
 #pragma unroll
     for (int j = 0; j < NF; j++) {
-      A[j] = a * nL[j];
-      Bm[j] = b * nL[j];
-      Cm[j] = c * nL[j];
+      const double Lj = rf(NF * ri + j, k);
+      A[j] = a * Lj;
+      Bm[j] = b * Lj;
+      Cm[j] = c * Lj;
     }
-    if (k < N) load_row(k + 1);
-    if (k == 0) {
-      xp = F;
+  };
+  // state after row k-1: row ri of B', row ri of C, component ri of x'
+  double bp[NF], cprev[NF], xp;
+  auto first = [&](int k) {  // row k as the first row of the chain
+    double A[NF], Bm[NF], Cm[NF], F;
+    row(k, A, Bm, Cm, F);
+    xp = F;
 #pragma unroll
-      for (int j = 0; j < NF; j++) bp[j] = Bm[j];
-    } else {
+    for (int j = 0; j < NF; j++) {
+      bp[j] = Bm[j];
+      cprev[j] = Cm[j];
+    }
+  };
+  // one pass over rows k0..k1-1 (row k stores the inverse of B'_{k-1}, C_{k-1}
+  // and x'_{k-1} at k-1); early: stop where a row repeats what is stored
+  auto forward = [&](bool early) {
+    // row k+1's coefficients and what the previous pass stored at k are loaded one row ahead
+    double na = 0.0, nb = 0.0, nc = 0.0, nF = 0.0, nL[NF], sBi[NF], sxp = 0.0;
+#pragma unroll
+    for (int j = 0; j < NF; j++) nL[j] = sBi[j] = 0.0;
+    auto load = [&](int k) {
+      na = coef(0, k);
+      nb = coef(1, k);
+      nc = coef(2, k);
+      nF = coef(3, k);
+#pragma unroll
+      for (int j = 0; j < NF; j++) nL[j] = rf(NF * ri + j, k);
+    };
+    auto load_stored = [&](int k) {
+#pragma unroll
+      for (int j = 0; j < NF; j++) sBi[j] = bk(NF * ri + j, k);
+      sxp = bk(2 * NF * NF + ri, k);
+    };
+    if (has) load(k0);
+    if (early && has) load_stored(k0 - 1);
+    for (int k = k0; k < k1; k++) {
+      double A[NF], Bm[NF], Cm[NF], cBi[NF];
+      const double F = nF, cxp = sxp;
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        A[j] = na * nL[j];
+        Bm[j] = nb * nL[j];
+        Cm[j] = nc * nL[j];
+        cBi[j] = sBi[j];
+      }
+      if (k + 1 < k1) {
+        load(k + 1);
+        if (early) load_stored(k);
+      }
       double Bi[NF][NF];
       cw_inv_quad<NF>(bp, ri, Bi);
+      if (early) {
+        bool same = __double_as_longlong(xp) == __double_as_longlong(cxp);
+#pragma unroll
+        for (int j = 0; j < NF; j++) same = same && __double_as_longlong(Bi[ri][j]) == __double_as_longlong(cBi[j]);
+        if (quad_all(same || !act)) return false;  // this row and every later one as stored
+      }
       double fac[NF];
 #pragma unroll
       for (int j = 0; j < NF; j++) {
@@ -1811,7 +1964,6 @@ __device__ __forceinline__ void cwc_flow_quad(const CaseDev &C, const SweepGeom 
         for (int q = 0; q < NF; q++) acc += (A[q] * Bi[q][j]);
         fac[j] = acc;
       }
-      // row k-1 for the back substitution
       if (act) {
 #pragma unroll
         for (int j = 0; j < NF; j++) {
@@ -1838,69 +1990,144 @@ __device__ __forceinline__ void cwc_flow_quad(const CaseDev &C, const SweepGeom 
 #pragma unroll
       for (int q = 0; q < NF; q++) xn -= (fac[q] * xpf[q]);
       xp = xn;
+#pragma unroll
+      for (int j = 0; j < NF; j++) cprev[j] = Cm[j];
     }
-#pragma unroll
-    for (int j = 0; j < NF; j++) cprev[j] = Cm[j];
-  }
-  // last row, then back substitution x_k = binv_k (x'_k - C_k x_{k+1})
-  double Bi[NF][NF], xpf[NF];
-  cw_inv_quad<NF>(bp, ri, Bi);
-#pragma unroll
-  for (int q = 0; q < NF; q++) xpf[q] = qget(xp, q);
-  double x = 0;
-#pragma unroll
-  for (int j = 0; j < NF; j++) x += (Bi[ri][j] * xpf[j]);
-  if (act) out[((long)ri * nl + l) * (N + 1) + N] = x;
-  double nbi[NF], ncp[NF], nxq = 0.0;
-  auto load_back = [&](int k) {
-#pragma unroll
-    for (int j = 0; j < NF; j++) {
-      nbi[j] = bk(NF * ri + j, k);
-      ncp[j] = bk(NF * NF + NF * ri + j, k);
-    }
-    nxq = bk(2 * NF * NF + ri, k);
+    return true;
   };
+  // run 0 starts at row 0 exactly; the others from their first row's predecessor taken as a first row
+  first(seg == 0 || !has ? 0 : k0 - 1);
+  double nbp[NF], nxp = xp;  // the entry this run last started from
 #pragma unroll
-  for (int j = 0; j < NF; j++) nbi[j] = ncp[j] = 0.0;
-  if (N > 0) load_back(N - 1);
-  for (int k = N - 1; k >= 0; k--) {
-    double bi[NF], cp[NF], xf[NF], yf[NF];
-    const double xq = nxq;
+  for (int j = 0; j < NF; j++) nbp[j] = bp[j];
+  if (has) forward(false);
+  double ebp[NF], exq = xp;  // exit state
+#pragma unroll
+  for (int j = 0; j < NF; j++) ebp[j] = bp[j];
+  for (int pass = 0;; pass++) {
+    double pbp[NF];
+    bool changed = false;
 #pragma unroll
     for (int j = 0; j < NF; j++) {
-      bi[j] = nbi[j];
-      cp[j] = ncp[j];
-      xf[j] = qget(x, j);
+      pbp[j] = __shfl(ebp[j], lane - 4);
+      changed = changed || __double_as_longlong(pbp[j]) != __double_as_longlong(nbp[j]);
     }
-    if (k > 0) load_back(k - 1);
-    double y = xq;
+    const double pxp = __shfl(exq, lane - 4);
+    changed = seg > 0 && (changed || __double_as_longlong(pxp) != __double_as_longlong(nxp));
+    changed = !quad_all(!changed);  // a quad moves as one
+    if (!__any(changed) || pass > segs) break;
+    if (changed) {
 #pragma unroll
-    for (int j = 0; j < NF; j++) y -= (cp[j] * xf[j]);
+      for (int j = 0; j < NF; j++) nbp[j] = bp[j] = pbp[j];
+      nxp = xp = pxp;
+      if (!has) {
 #pragma unroll
-    for (int j = 0; j < NF; j++) yf[j] = qget(y, j);
-    double xn = 0;
+        for (int j = 0; j < NF; j++) ebp[j] = pbp[j];
+        exq = pxp;
+      } else {
+        double A[NF], Bm[NF], Cm[NF], F;
+        row(k0 - 1, A, Bm, Cm, F);
 #pragma unroll
-    for (int j = 0; j < NF; j++) xn += (bi[j] * yf[j]);
-    x = xn;
-    if (act) out[((long)ri * nl + l) * (N + 1) + k] = x;
+        for (int j = 0; j < NF; j++) cprev[j] = Cm[j];
+        if (forward(true)) {
+#pragma unroll
+          for (int j = 0; j < NF; j++) ebp[j] = bp[j];
+          exq = xp;
+        }
+      }
+    }
+  }
+  // back substitution x_k = binv_k (x'_k - C_k x_{k+1}): rows k1-2 .. k0-1 of
+  // this run, the top run also row N, entering from the run above
+  const long ob = ((long)ri * nl + l) * (N + 1);
+  const bool top = k1 == N + 1 && (has || N == 0);
+  double xin = 0.0;
+  if (top) {
+    double Bi[NF][NF], xpf[NF];
+#pragma unroll
+    for (int j = 0; j < NF; j++) bp[j] = ebp[j];
+    cw_inv_quad<NF>(bp, ri, Bi);
+#pragma unroll
+    for (int q = 0; q < NF; q++) xpf[q] = qget(exq, q);
+    double x = 0;
+#pragma unroll
+    for (int j = 0; j < NF; j++) x += (Bi[ri][j] * xpf[j]);
+    if (act) out[ob + N] = x;
+    xin = x;
+  }
+  auto backward = [&](double x, bool early) {
+    int k = k1 - 2;
+    if (k < k0 - 1) return;
+    // each row's values loaded one row ahead
+    double nbi[NF], ncp[NF], nxq = 0.0, nso = 0.0;
+    auto load_back = [&](int k) {
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        nbi[j] = bk(NF * ri + j, k);
+        ncp[j] = bk(NF * NF + NF * ri + j, k);
+      }
+      nxq = bk(2 * NF * NF + ri, k);
+      if (early) nso = out[ob + k];
+    };
+    load_back(k);
+    for (; k >= k0 - 1; k--) {
+      double bi[NF], cp[NF], xf[NF], yf[NF];
+      const double so = nso;
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        bi[j] = nbi[j];
+        cp[j] = ncp[j];
+        xf[j] = qget(x, j);
+      }
+      double y = nxq;
+      if (k > k0 - 1) load_back(k - 1);
+#pragma unroll
+      for (int j = 0; j < NF; j++) y -= (cp[j] * xf[j]);
+#pragma unroll
+      for (int j = 0; j < NF; j++) yf[j] = qget(y, j);
+      double xn = 0;
+#pragma unroll
+      for (int j = 0; j < NF; j++) xn += (bi[j] * yf[j]);
+      x = xn;
+      if (early && quad_all(!act || __double_as_longlong(x) == __double_as_longlong(so))) return;
+      if (act) out[ob + k] = x;
+    }
+  };
+  double xs = xin;  // the entry the last pass started from
+  if (has) backward(xin, false);
+  for (int pass = 0;; pass++) {
+    const double mine = has ? out[ob + k0 - 1] : xs;  // this run's exit: the x of its lowest row
+    const double above = __shfl(mine, lane + 4);
+    const bool last = seg == segs - 1 || top;
+    bool changed = !last && __double_as_longlong(above) != __double_as_longlong(xs);
+    changed = !quad_all(!changed);
+    if (!__any(changed) || pass > segs) break;
+    if (changed) {
+      xs = above;
+      if (has) backward(above, true);
+    }
   }
 }
 
-// Work items of a sweep: the flow chains first (a quad of lanes per system;
-// (system, line) with the line fastest), then one item per species
-// chain ((chain, system, line), the line fastest), so that a wave does one
-// kind of chain.  The y sweep starts on a 64-item boundary, which keeps every
-// quad of a flow chain on one quad of a wave.
+// Work items of a sweep: the flow chains first (a quad of lanes per run of a
+// system; (run, system, line) with the run fastest), then one item per run of
+// each species chain ((run, chain, system, line), the run fastest), so that
+// the runs of a chain are neighbouring lanes of one wave.  The y sweep starts
+// on a 64-item boundary, which keeps every quad of a flow chain on one quad
+// of a wave.
 __host__ __device__ inline int cw_flow_lanes(int) { return 4; }
 __host__ __device__ inline long cw_items(const SweepGeom &S, int nv, int nf) {
-  return 4L * S.nlines * (cw_flow_lanes(nf) + nv - nf);
+  return 4L * S.nlines * cw_segs(S, nv, nf) * (cw_flow_lanes(nf) + nv - nf);
 }
 __host__ __device__ inline long cw_x_span(const CaseDev &c, int nf) {
   const long nX = cw_items(c.Sx, c.G.nv, nf);
   return c.G.ndims == 2 ? (nX + 63) & ~63L : nX;
 }
 
-__global__ void __launch_bounds__(256) k_cw_solve(CASE_ARGS) {
+#ifndef PIAFS_CWSOLVE_MINB
+#define PIAFS_CWSOLVE_MINB 1
+#endif
+__global__ void __launch_bounds__(256, PIAFS_CWSOLVE_MINB) k_cw_solve(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
   const int nv = C.G.nv, nf = C.model == 1 ? 3 : 4;
@@ -1910,16 +2137,20 @@ __global__ void __launch_bounds__(256) k_cw_solve(CASE_ARGS) {
   const int dir = t < nXs ? 0 : 1;
   const SweepGeom &S = dir ? C.Sy : C.Sx;
   const long u = dir ? t - nXs : t;
-  const long nl = S.nlines, nflow = 4L * nl * cw_flow_lanes(nf);
+  const int segs = cw_segs(S, nv, nf);
+  const long nl = S.nlines, nflow = 4L * nl * cw_flow_lanes(nf) * segs;
   if (u < nflow) {
-    const long grp = u >> 2;
-    if (nf == 4) cwc_flow_quad<4>(C, S, dir, (int)(grp % nl), (int)(grp / nl), (int)(u & 3));
-    else cwc_flow_quad<3>(C, S, dir, (int)(grp % nl), (int)(grp / nl), (int)(u & 3));
+    const long quad = u >> 2, grp = quad / segs;
+    const int seg = (int)(quad % segs);
+    if (nf == 4) cwc_flow_quad<4>(C, S, dir, (int)(grp % nl), (int)(grp / nl), (int)(u & 3), seg, segs);
+    else cwc_flow_quad<3>(C, S, dir, (int)(grp % nl), (int)(grp / nl), (int)(u & 3), seg, segs);
     return;
   }
   const long v = u - nflow, per = 4L * nl;
-  const int chain = 1 + (int)(v / per), sys = (int)((v % per) / nl), l = (int)(v % nl);
-  cwc_species(C, S, dir, l, sys, chain, nf);
+  const long ch = v / segs;
+  const int seg = (int)(v % segs);
+  const int chain = 1 + (int)(ch / per), sys = (int)((ch % per) / nl), l = (int)(ch % nl);
+  cwc_species(C, S, dir, l, sys, chain, nf, seg, segs);
 }
 
 // Blocks of a sweep for the team-based reconstruction (each block holds

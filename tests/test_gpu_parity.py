@@ -410,6 +410,43 @@ def test_long_line_window_edges_match_oracle(N):
     assert np.all(_rel(got, ref) <= 1e-13), _rel(got, ref)
 
 
+_PERIODIC_2D = [("periodic", 0, 1, 0, 0, -1e99, 1e99), ("periodic", 0, -1, 0, 0, -1e99, 1e99),
+                ("periodic", 1, 1, -1e99, 1e99, 0, 0), ("periodic", 1, -1, -1e99, 1e99, 0, 0)]
+BLOCK_RUN_DECKS = {
+    # 1D Sod: 4 runs of 256 rows; 2 ragged runs of 150; the shock and plateaus make reruns
+    "sod_crweno_char_N1024": lambda: decks.sod_1d(N=1024, n_iter=12, t_final=12 * 0.2 / 901, scheme="crweno5",
+                                                  interp="characteristic"),
+    "sod_crweno_char_rf_N300": lambda: decks.sod_1d(N=300, n_iter=12, t_final=12 * 0.2 / 264, scheme="crweno5",
+                                                    interp="characteristic", upwinding="rf-char"),
+    # x lines of 641 rows in 4 runs with chemistry; 2049 rows in 16 runs of 128
+    "ns2d_chem_crweno_char_640x6": lambda: decks.ns2d_open_box(
+        nx=640, ny=6, n_iter=3, dt=1e-4, flow=(1.0, 0.3, 0.2, None), scheme="crweno5", interp="characteristic",
+        upwinding="roe", include_chemistry=True, Re=1000.0, boundary=_PERIODIC_2D),
+    "ns2d_crweno_char_2048x4": lambda: decks.ns2d_open_box(
+        nx=2048, ny=4, n_iter=3, dt=2e-5, flow=(1.0, 0.3, 0.2, None), scheme="crweno5", interp="characteristic",
+        upwinding="roe", Re=1000.0, boundary=_PERIODIC_2D),
+}
+
+
+@pytest.mark.parametrize("name", sorted(BLOCK_RUN_DECKS))
+def test_block_chain_runs_match_oracle(name):
+    """Characteristic CRWENO5 lines long enough that k_cw_solve splits each
+    block chain into runs relayed bit for bit (runs of at least 128 rows),
+    against the oracle's sequential block sweep (blocktridiagLU.c:159-301)."""
+    dk = BLOCK_RUN_DECKS[name]()
+    n = int(dk.solver["n_iter"])
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        oc = orc.OCase(d)
+        oc.step(n)
+        ref = oc.solution()
+        with piafs_amd.Case(d) as c:
+            c.step(n)
+            got = c.solution()
+    assert np.all(np.isfinite(ref))
+    assert np.all(_rel(got, ref) <= 1e-11), _rel(got, ref)
+
+
 def test_derivative_exchange_translation_invariant_and_matches_oracle():
     """The derivative-ghost exchange of a run decomposed along the periodic
     dimensions (piafs_set_derivative_exchange): the right-hand side of a
