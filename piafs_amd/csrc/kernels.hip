@@ -19,6 +19,35 @@ constexpr int TB = 256;  // threads per block
 inline unsigned nblk(long n, int tb = TB) { return (unsigned)((n + tb - 1) / tb); }
 
 // ---------------------------------------------------------------------------
+// Quotient n / d for the WENO weights: n >= +0, d > 0, both normal or n zero,
+// and n / d far from the overflow and underflow limits (d >= eps^2 with the
+// smoothness-indicator epsilon, n a constant or a nonnegative weight).  It is
+// the target's IEEE double division (a reciprocal estimate, two Newton-Raphson
+// refinements and one residual correction, Markstein's method) without the
+// range scaling and special-case fix-up steps, which leave such operands and
+// their quotient unchanged: the result is the correctly rounded quotient, the
+// same bits as n / d, in three fewer instructions and with no condition
+// register between them.
+// ---------------------------------------------------------------------------
+#ifndef PIAFS_FAST_WDIV
+#define PIAFS_FAST_WDIV 1
+#endif
+__device__ __forceinline__ double wdiv(double n, double d) {
+#if PIAFS_FAST_WDIV
+  const double r0 = __builtin_amdgcn_rcp(d);
+  const double e0 = __builtin_fma(-d, r0, 1.0);
+  const double r1 = __builtin_fma(r0, e0, r0);
+  const double e1 = __builtin_fma(-d, r1, 1.0);
+  const double r2 = __builtin_fma(r1, e1, r1);
+  const double q = n * r2;
+  const double t = __builtin_fma(-d, q, n);
+  return __builtin_fma(t, r2, q);
+#else
+  return n / d;
+#endif
+}
+
+// ---------------------------------------------------------------------------
 // WENO5 nonlinear weights, include/interpolation.h:255-490
 // (JS :255-281, mapped :309-341, Z (Borges) :420-450, YC :478-508)
 // ---------------------------------------------------------------------------
@@ -41,24 +70,24 @@ __device__ __forceinline__ void weno_w(double c1, double c2, double c3, double m
       const double t = m3 - 4 * m2 + 6 * m1 - 4 * p1 + p2;
       tau = t * t;
     }
-    const double r1 = tau / (b1 + eps), r2 = tau / (b2 + eps), r3 = tau / (b3 + eps);
+    const double r1 = wdiv(tau, b1 + eps), r2 = wdiv(tau, b2 + eps), r3 = wdiv(tau, b3 + eps);
     a1 = c1 * (1.0 + r1 * r1);
     a2 = c2 * (1.0 + r2 * r2);
     a3 = c3 * (1.0 + r3 * r3);
   } else {
-    a1 = c1 / ((b1 + eps) * (b1 + eps));
-    a2 = c2 / ((b2 + eps) * (b2 + eps));
-    a3 = c3 / ((b3 + eps) * (b3 + eps));
+    a1 = wdiv(c1, (b1 + eps) * (b1 + eps));
+    a2 = wdiv(c2, (b2 + eps) * (b2 + eps));
+    a3 = wdiv(c3, (b3 + eps) * (b3 + eps));
   }
-  double s = 1.0 / (a1 + a2 + a3);
+  double s = wdiv(1.0, a1 + a2 + a3);
   w1 = a1 * s;
   w2 = a2 * s;
   w3 = a3 * s;
   if (KIND == 1) {
-    a1 = w1 * (c1 + c1 * c1 - 3 * c1 * w1 + w1 * w1) / (c1 * c1 + w1 * (1.0 - 2.0 * c1));
-    a2 = w2 * (c2 + c2 * c2 - 3 * c2 * w2 + w2 * w2) / (c2 * c2 + w2 * (1.0 - 2.0 * c2));
-    a3 = w3 * (c3 + c3 * c3 - 3 * c3 * w3 + w3 * w3) / (c3 * c3 + w3 * (1.0 - 2.0 * c3));
-    s = 1.0 / (a1 + a2 + a3);
+    a1 = wdiv(w1 * (c1 + c1 * c1 - 3 * c1 * w1 + w1 * w1), c1 * c1 + w1 * (1.0 - 2.0 * c1));
+    a2 = wdiv(w2 * (c2 + c2 * c2 - 3 * c2 * w2 + w2 * w2), c2 * c2 + w2 * (1.0 - 2.0 * c2));
+    a3 = wdiv(w3 * (c3 + c3 * c3 - 3 * c3 * w3 + w3 * w3), c3 * c3 + w3 * (1.0 - 2.0 * c3));
+    s = wdiv(1.0, a1 + a2 + a3);
     w1 = a1 * s;
     w2 = a2 * s;
     w3 = a3 * s;
