@@ -890,8 +890,11 @@ __device__ __forceinline__ double rcp_est(double b) {
 // Forward elimination of team_solve: on exit B holds b' and X holds x' of
 // every row, Fk the multipliers.  The row before the team's first row has
 // pivot bfirst, super-diagonal cfirst and x' xfirst (an identity row: 1, 0,
-// its value).  zp is set when a pivot is zero.
-template <int MR, bool FULL = false>
+// its value).  zp is set when a pivot is zero.  QD: the system is a CRWENO5
+// coefficient system (a_k >= +0, true pivots positive and of order one), whose
+// multipliers a_k / b'_{k-1} divide with wdiv; the relay ends on the true
+// pivots, where wdiv gives the IEEE quotient's bits.
+template <int MR, bool FULL = false, bool QD = false>
 __device__ __forceinline__ void team_forward(double (&A)[MR], double (&B)[MR], double (&Cc)[MR], double (&X)[MR],
                                              double (&Fk)[MR], int n, int lane, int T, double bfirst, double cfirst,
                                              double xfirst, double *red, bool plain_start, bool &zp) {
@@ -923,7 +926,7 @@ __device__ __forceinline__ void team_forward(double (&A)[MR], double (&B)[MR], d
 #pragma unroll
     for (int i = 0; i < MR; i++) {
       if (FULL || i < n) {
-        const double fac = A[i] / bend;
+        const double fac = QD ? wdiv(A[i], bend) : A[i] / bend;
         bend = B[i] - fac * c;
         c = Cc[i];
         Fk[i] = fac;
@@ -957,7 +960,7 @@ __device__ __forceinline__ void team_forward(double (&A)[MR], double (&B)[MR], d
 #pragma unroll
       for (int i = 0; i < MR; i++) {
         if ((FULL || i < n) && live) {
-          const double fac = A[i] / bb;
+          const double fac = QD ? wdiv(A[i], bb) : A[i] / bb;
           if (__double_as_longlong(fac) == __double_as_longlong(Fk[i])) {
             live = false;  // same multiplier, so the same pivot here and in every later row
           } else {
@@ -1048,13 +1051,13 @@ __device__ __forceinline__ void team_backward(double (&B)[MR], double (&Cc)[MR],
 // the scanned estimates are replaced by the trivial ones (b' = 1, x' = 0), so
 // that the relay runs through many passes (testing).  FULL: every lane holds
 // MR rows (n is not read), rows past the system's end being identity rows.
-template <int MR, bool FULL = false>
+template <int MR, bool FULL = false, bool QD = false>
 __device__ __forceinline__ void team_solve(double (&A)[MR], double (&B)[MR], double (&Cc)[MR], double (&X)[MR],
                                                 int n, int lane, int T, bool is_last, double xfirst, double xlast,
                                                 double *red, int *status, bool plain_start) {
   double Fk[MR];
   bool zp = false;
-  team_forward<MR, FULL>(A, B, Cc, X, Fk, n, lane, T, 1.0, 0.0, xfirst, red, plain_start, zp);
+  team_forward<MR, FULL, QD>(A, B, Cc, X, Fk, n, lane, T, 1.0, 0.0, xfirst, red, plain_start, zp);
   team_backward<MR, FULL>(B, Cc, X, Fk, n, lane, T, is_last, xlast, red, plain_start);
   if (zp) atomicOr(status, 1);
 }
@@ -1130,7 +1133,7 @@ __device__ __forceinline__ void recon_team(const CaseDev &C, int dir, long sys, 
       }
     }
     if (CR)
-      team_solve<kMaxRows>(A, B, Cc, X, n, lane, S.team, lane == lastlane, x0, xN, red, C.status,
+      team_solve<kMaxRows, false, PIAFS_FAST_WDIV>(A, B, Cc, X, n, lane, S.team, lane == lastlane, x0, xN, red, C.status,
                            C.R.plain_start != 0);
   };
   solve(r0, 0);
@@ -1262,10 +1265,10 @@ __device__ __forceinline__ void recon_pair_lds(const CaseDev &C, int dir, long s
     }
     if (CR) {
       if (FULL)
-        team_solve<kMaxRows, true>(A, B, Cc, X, kMaxRows, lane, T, lane == T - 1, x0, nlane < T ? 0.0 : xN, red,
+        team_solve<kMaxRows, true, PIAFS_FAST_WDIV>(A, B, Cc, X, kMaxRows, lane, T, lane == T - 1, x0, nlane < T ? 0.0 : xN, red,
                                    C.status, C.R.plain_start != 0);
       else
-        team_solve<kMaxRows>(A, B, Cc, X, n, lane, T, lane == lastlane, x0, xN, red, C.status,
+        team_solve<kMaxRows, false, PIAFS_FAST_WDIV>(A, B, Cc, X, n, lane, T, lane == lastlane, x0, xN, red, C.status,
                              C.R.plain_start != 0);
     }
   };
