@@ -1682,7 +1682,10 @@ __global__ void __launch_bounds__(256) k_cw_rows(CASE_ARGS) {
 // the fixed point is the sequential chain's, bit for bit; the recurrences
 // contract, so a run of a few dozen rows forgets its estimate and one
 // correcting pass usually suffices.  The back substitution is relayed the
-// same way from the top.  segs = 1 is the sequential chain.
+// same way from the top.  segs = 1 is the sequential chain.  After relay pass
+// p, runs 0 .. p+1 hold the sequential chain (run p+1 has started from run
+// p's exact exit), so segs - 1 passes always suffice and the loops' bound of
+// segs + 1 passes is never what ends them.
 #ifndef PIAFS_CW_RMIN
 #define PIAFS_CW_RMIN 128
 #endif
