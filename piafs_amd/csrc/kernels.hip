@@ -477,7 +477,8 @@ __device__ double ghost_value(const CaseDev &C, int s, int v, int i, int j) {
   return igm1 * pg + 0.5 * rg * (ug * ug + vg * vg);
 }
 
-__global__ void __launch_bounds__(256) k_stage(CASE_ARGS) {
+template <int ST>
+__device__ __forceinline__ void stage_body(CASE_ARGS) {
   CASE_PREAMBLE
   if (stage >= C.nstages) return;
   const Grid &G = C.G;
@@ -489,16 +490,17 @@ __global__ void __launch_bounds__(256) k_stage(CASE_ARGS) {
     C.U[p] = stage_value(C, s, p);
     return;
   }
-  // 2D interior: 32x32 tiles, 256 threads; natural write, then transposed
+  // 2D interior: ST x ST tiles, 256 threads; natural write, then transposed
   // write via LDS.  The ghost layer is k_ghost's.
-  __shared__ double tile[32][33];
-  const int ntx = (G.nx + 31) / 32, nty = (G.ny + 31) / 32;
+  constexpr int SR = 256 / ST;
+  __shared__ double tile[ST][ST + 1];
+  const int ntx = (G.nx + ST - 1) / ST, nty = (G.ny + ST - 1) / ST;
   const int id = (int)(blockIdx.x - firstblock[cidx]);
   const int v = id / (ntx * nty), rem = id % (ntx * nty);
-  const int i0 = (rem % ntx) * 32, j0 = (rem / ntx) * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int i0 = (rem % ntx) * ST, j0 = (rem / ntx) * ST;
+  const int tx = threadIdx.x & (ST - 1), ty = threadIdx.x / ST;
   const long pl = (long)v * G.plane;
-  for (int r = ty; r < 32; r += 8) {
+  for (int r = ty; r < ST; r += SR) {
     const int i = i0 + tx, j = j0 + r;
     double val = 0.0;
     if (i < G.nx && j < G.ny) {
@@ -509,11 +511,17 @@ __global__ void __launch_bounds__(256) k_stage(CASE_ARGS) {
     tile[r][tx] = val;
   }
   __syncthreads();
-  for (int r = ty; r < 32; r += 8) {
+  for (int r = ty; r < ST; r += SR) {
     const int i = i0 + r, j = j0 + tx;
     if (i < G.nx && j < G.ny) C.UT[pl + (long)(i + G.g) * G.nyg + (j + G.gy)] = tile[tx][r];
   }
 }
+
+// 32 x 32 tiles, or 64 x 64 when every case of the plan has at least
+// 2048 x 2048 points (rows of 512 B in both layouts; 16 % faster there, 12 %
+// slower at 1024 x 1024, where the smaller tiles give twice the blocks)
+__global__ void __launch_bounds__(256) k_stage(CASE_ARGS) { stage_body<32>(cases, blockcase, firstblock, stage); }
+__global__ void __launch_bounds__(256) k_stage64(CASE_ARGS) { stage_body<64>(cases, blockcase, firstblock, stage); }
 
 // Ghost points of the stage state, one thread per (variable, ghost point):
 // the g rows below and above the interior (full ghosted width), then the g
@@ -3440,11 +3448,14 @@ bool team_shape(int N, int max_rows, int &team, int &rows) {
   return team <= 256;
 }
 
+bool stage_tile_64(const CaseDev &c) { return c.G.ndims == 2 && (long)c.G.nx * c.G.ny >= 2048L * 2048L; }
+
 void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
   const Grid &G = c.G;
   const bool two = (G.ndims == 2);
   auto nb = [](long n) { return (int)((n + TB - 1) / TB); };
-  blocks[K_STAGE] = two ? ((G.nx + 31) / 32) * ((G.ny + 31) / 32) * G.nv : nb((long)G.nx * G.nv);
+  const int st = c.stage_tile == 64 ? 64 : 32;
+  blocks[K_STAGE] = two ? ((G.nx + st - 1) / st) * ((G.ny + st - 1) / st) * G.nv : nb((long)G.nx * G.nv);
   blocks[K_GHOST] = nb(ghost_points(G) * G.nv);
   blocks[K_FLUX] = nb((long)(c.Sx.N + 2 * c.Sx.g) * c.Sx.nlines + (two ? (long)(c.Sy.N + 2 * c.Sy.g) * c.Sy.nlines : 0));
   blocks[K_CWROWS] = blocks[K_CWSOLVE] = blocks[K_LROWS] = blocks[K_LSOLVE] = 0;
@@ -3496,7 +3507,7 @@ void launch_stage(const CaseDev *d, const Plan &pl, int s, hipStream_t st, MarkF
   auto R = [&](KernFn k, int cls, hipStream_t on) {
     if (cls <= last) run(k, cls, d, pl, s, on, mark, ctx);
   };
-  R(k_stage, K_STAGE, st);
+  R(pl.stage64 ? k_stage64 : k_stage, K_STAGE, st);
   R(k_ghost, K_GHOST, st);
   R(k_flux, K_FLUX, st);
   R(k_recon_ends, K_ENDS, st);
@@ -3589,7 +3600,7 @@ void launch_class(const CaseDev *d, const Plan &pl, int cls, int s, hipStream_t 
                                          k_lsolve, k_recon_char,
                                          k_cw_rows, k_cw_solve, k_upwind, k_bflux, k_qderiv, k_viscflux,
                                          k_assemble, k_update};
-  run(table[cls], cls, d, pl, s, st, nullptr, nullptr);
+  run(cls == K_STAGE && pl.stage64 ? k_stage64 : table[cls], cls, d, pl, s, st, nullptr, nullptr);
 }
 
 void launch_weno_weights(int kind, const double *copt, const double *st, int count, double eps, double *w,

@@ -136,6 +136,7 @@ struct CaseDev {
   ReconParams R;
   int model, upwind, nstages;  // upwind: 0 Roe, 1 rf-char, 2 llf-char, 3 rusanov
   int paired;    // reconstruction writes 0.5*(uR-uL), 0.5*(fL+fR) only (Roe with component-wise interpolation)
+  int stage_tile; // side of k_stage's transpose tiles, 32 or 64 (set by the plan)
   int terms;     // terms k_assemble adds: bit 0 hyperbolic, bit 1 parabolic, bit 2 source (7 in a step)
   double dt, A[16], b[4], c[4];
   double *u, *U, *UT, *Udot[4];
@@ -172,6 +173,7 @@ const char *kclass_name(int k);
 // Block -> case tables of every kernel class for one ensemble.
 struct Plan {
   int ncases = 0;
+  bool stage64 = false;  // k_stage with 64 x 64 tiles (every case at least 2048 x 2048)
   int nblocks[K_NCLASS] = {0};
   int *blockcase[K_NCLASS] = {nullptr};   // device arrays, one int per block
   int *firstblock[K_NCLASS] = {nullptr};  // device arrays, one int per case
@@ -179,6 +181,8 @@ struct Plan {
 
 // Host side: blocks each case needs for each kernel class (0 = class unused).
 void case_blocks(const CaseDev &c, int blocks[K_NCLASS]);
+// Whether the case is large enough for k_stage's 64 x 64 tiles.
+bool stage_tile_64(const CaseDev &c);
 // doubles per interface of the characteristic CRWENO5 scratch of one sweep:
 // block rows (the flow block of L, then a, b, c coefficients and the right-hand
 // side of every variable of the four systems), back-substitution data (four

@@ -424,6 +424,9 @@ int Engine::build() {
       hd[k].status = d_status + k;
     }
   }
+  plan.stage64 = true;
+  for (int k = 0; k < nc; k++) plan.stage64 = plan.stage64 && stage_tile_64(hd[k]);
+  for (int k = 0; k < nc; k++) hd[k].stage_tile = plan.stage64 ? 64 : 32;
   HIPCHK(hipMalloc(&p, nc * sizeof(CaseDev)));
   plan_allocs.push_back(p);
   d_cases = (CaseDev *)p;
