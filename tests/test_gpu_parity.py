@@ -447,6 +447,47 @@ def test_block_chain_runs_match_oracle(name):
     assert np.all(_rel(got, ref) <= 1e-11), _rel(got, ref)
 
 
+def _eps(dk, eps):
+    dk.weno["epsilon"] = eps
+    return dk
+
+
+SMALL_EPS_DECKS = {
+    # the weights' quotients c_k / (beta_k + eps)^2 reach 1e80 where the data are smooth
+    "sod_crweno_comp_eps1e-40": lambda: _eps(decks.sod_1d(N=256, n_iter=20, t_final=20 * 0.2 / 225,
+                                                          scheme="crweno5", interp="components"), 1e-40),
+    "sod_weno_char_mapped_eps1e-40": lambda: _eps(decks.sod_1d(N=200, n_iter=20, t_final=20 * 0.2 / 176,
+                                                               scheme="weno5", interp="characteristic", mapped=1),
+                                                  1e-40),
+    "ns2d_chem_crweno_mapped_eps1e-40": lambda: _eps(decks.ns2d_open_box(
+        nx=48, ny=40, n_iter=4, dt=5e-4, flow=(1.0, 0.3, 0.2, None), scheme="crweno5", weights="mapped",
+        include_chemistry=True, Re=1000.0, boundary=_PERIODIC_2D), 1e-40),
+    "ns2d_crweno_z_eps1e-30": lambda: _eps(decks.ns2d_open_box(
+        nx=40, ny=48, n_iter=4, dt=5e-4, flow=(1.0, 0.3, 0.2, None), scheme="crweno5", weights="z",
+        Re=1000.0, boundary=_PERIODIC_2D), 1e-30),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SMALL_EPS_DECKS))
+def test_small_weno_epsilon_matches_oracle(name):
+    """WENO weights with a small epsilon (weno.inp `epsilon`, WENOInitialize.c),
+    where the weights' divisions take operands far from the default's, against
+    the oracle after the deck's steps: 1e-13 for Euler, 1e-11 with chemistry."""
+    dk = SMALL_EPS_DECKS[name]()
+    n = int(dk.solver["n_iter"])
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        oc = orc.OCase(d)
+        oc.step(n)
+        ref = oc.solution()
+        with piafs_amd.Case(d) as c:
+            c.step(n)
+            got = c.solution()
+    assert np.all(np.isfinite(ref))
+    tol = 1e-13 if name.startswith("sod") else 1e-11
+    assert np.all(_rel(got, ref) <= tol), _rel(got, ref)
+
+
 def test_derivative_exchange_translation_invariant_and_matches_oracle():
     """The derivative-ghost exchange of a run decomposed along the periodic
     dimensions (piafs_set_derivative_exchange): the right-hand side of a
