@@ -19,15 +19,20 @@ constexpr int TB = 256;  // threads per block
 inline unsigned nblk(long n, int tb = TB) { return (unsigned)((n + tb - 1) / tb); }
 
 // ---------------------------------------------------------------------------
-// Quotient n / d for the WENO weights: n >= +0, d > 0, both normal or n zero,
-// and n / d far from the overflow and underflow limits (d >= eps^2 with the
-// smoothness-indicator epsilon, n a constant or a nonnegative weight).  It is
-// the target's IEEE double division (a reciprocal estimate, two Newton-Raphson
-// refinements and one residual correction, Markstein's method) without the
-// range scaling and special-case fix-up steps, which leave such operands and
-// their quotient unchanged: the result is the correctly rounded quotient, the
-// same bits as n / d, in three fewer instructions and with no condition
-// register between them.
+// Quotient n / d for the WENO weights and the CRWENO5 multipliers: the
+// target's IEEE double division (a reciprocal estimate, two Newton-Raphson
+// refinements and one residual correction, Markstein's method) without its
+// range scaling and special-case fix-up steps, in three fewer instructions
+// and with no condition register between them.  Those steps are identities,
+// and the result is the IEEE quotient's bits, while n is +0 or of magnitude at
+// least 2^-969, d is normal and positive, and the quotient lies within
+// 2^-969 .. 2^1022: for the weights, whenever the smoothness indicators stay
+// within about 2^400 of epsilon (data of ordinary magnitude with the usual
+// epsilons).  Far outside (epsilon 1e-100 with data of 1e70), a weight below
+// about 1e-290 can differ in its last bits, and where a Z or YC ratio
+// overflows the row's weights are all NaN rather than partly NaN: the
+// interpolated value is still NaN as the reference's is, and a tiny weight's
+// term lies far below the last bit of a sum led by the large weights.
 // ---------------------------------------------------------------------------
 #ifndef PIAFS_FAST_WDIV
 #define PIAFS_FAST_WDIV 1
@@ -901,7 +906,8 @@ __device__ __forceinline__ double rcp_est(double b) {
 // its value).  zp is set when a pivot is zero.  QD: the system is a CRWENO5
 // coefficient system (a_k >= +0, true pivots positive and of order one), whose
 // multipliers a_k / b'_{k-1} divide with wdiv; the relay ends on the true
-// pivots, where wdiv gives the IEEE quotient's bits.
+// pivots, where wdiv gives the IEEE quotient's bits within its envelope (the
+// weights in a_k not tiny).
 template <int MR, bool FULL = false, bool QD = false>
 __device__ __forceinline__ void team_forward(double (&A)[MR], double (&B)[MR], double (&Cc)[MR], double (&X)[MR],
                                              double (&Fk)[MR], int n, int lane, int T, double bfirst, double cfirst,

@@ -53,6 +53,50 @@ def test_weno_weights_match_oracle(kind):
         np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-15)
 
 
+def _wide_stencils(rng, n, lo, hi):
+    mag = 10.0 ** rng.uniform(lo, hi, (n, 1))
+    st = rng.uniform(-1, 1, (n, 5)) * mag
+    q = n // 4
+    st[:q] = np.cumsum(rng.uniform(0, 1e-3, (q, 5)), axis=1) * mag[:q]  # smooth
+    st[q:q + q // 3] = np.repeat(mag[q:q + q // 3], 5, axis=1)            # constant: every beta zero
+    st[2 * q:2 * q + q // 3, 2:] *= 1e6                                    # jumps
+    return st
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_weno_weights_bitwise_over_wide_ranges(kind):
+    """The weights (piafs_weno_weights_host, include/interpolation.h:255-508),
+    whose divisions run the Newton-Raphson core of the target's IEEE double
+    division without its range scaling and fix-up steps, against the
+    oracle's C divisions.  Inside the envelope where those steps are
+    identities -- stencils of magnitude 1e-30 .. 1e30 of both signs (random,
+    smooth, constant, with jumps of 1e6), epsilon 1e-6 and 1e-40, both sets of
+    optimal weights -- every weight has the same bits.  Far outside it
+    (magnitudes up to 1e70 with epsilon 1e-100, where weights fall within
+    2^53 of the underflow limit and Z / YC ratios overflow) a weight may
+    differ in its last bits below 1e-290, and a row whose reference weights
+    are not finite is not finite here either: the interpolated value is then
+    the same (NaN, or the same sum)."""
+    rng = np.random.default_rng(100 + kind)
+    st = _wide_stencils(rng, 6000, -30, 30)
+    for eps in (1e-6, 1e-40):
+        for copt in ((0.1, 0.6, 0.3), (0.2, 0.5, 0.3)):
+            got = piafs_amd.weno_weights(kind, copt, st, eps)
+            ref = np.array([orc.weno_weights(kind, copt, s, eps) for s in st])
+            assert np.all(np.isfinite(ref))
+            same = got.view(np.int64) == ref.view(np.int64)
+            assert same.all(), (eps, copt, np.argwhere(~same)[:5], got[~same][:5], ref[~same][:5])
+    st = _wide_stencils(rng, 6000, -70, 70)
+    for copt in ((0.1, 0.6, 0.3), (0.2, 0.5, 0.3)):
+        got = piafs_amd.weno_weights(kind, copt, st, 1e-100)
+        ref = np.array([orc.weno_weights(kind, copt, s, 1e-100) for s in st])
+        row_fin = np.isfinite(ref).all(axis=1)
+        assert not np.isfinite(got[~row_fin]).all(axis=1).any()
+        g, r = got[row_fin], ref[row_fin]
+        close = (np.abs(g - r) <= 4 * np.spacing(np.abs(r))) | ((np.abs(r) < 1e-290) & (np.abs(g) < 1e-290))
+        assert close.all(), (copt, g[~close][:5], r[~close][:5])
+
+
 RHS_DECKS = {
     "sod_char": lambda: decks.sod_1d(N=64, n_iter=1),
     "sod_rfchar": lambda: decks.sod_1d(N=64, n_iter=1, upwinding="rf-char", mapped=1),
