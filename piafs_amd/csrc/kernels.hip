@@ -252,6 +252,19 @@ __device__ __forceinline__ void ns2d_eigvals(double g, const double u[4], double
   D[3] = vn;
 }
 
+// |v_n| of the Roe average of two cells with the Harten entropy fix
+// (delta = 1e-6, NavierStokes2DUpwind.c:75-78): the species' dissipation
+// coefficient of the Roe flux, as upwind_ns2d_roe computes it
+template <int DIR>
+__device__ __forceinline__ double ns2d_roe_dsp(double g, const double uLc[4], const double uRc[4]) {
+  double ua[4], D[4];
+  ns2d_roeavg(g, uLc, uRc, ua);
+  ns2d_eigvals<DIR>(g, ua, D);
+  const double vn = D[3];
+  const double delta = 0.000001, delta2 = delta * delta;
+  return fabs(vn) < delta ? (vn * vn + delta2) / (2 * delta) : fabs(vn);
+}
+
 // Left (rows) and right (columns) flow eigenvectors (navierstokes2d.h:186-311)
 template <int DIR>
 __device__ __forceinline__ void ns2d_eigvecs(double g, const double u[4], double L[4][4], double R[4][4]) {
@@ -619,6 +632,12 @@ __global__ void __launch_bounds__(256) k_flux(CASE_ARGS) {
   const long pl = S.plane;
   point_flux(C.model, dir, C.P.gamma, C.G.nv, [&](int m) { return q[m * pl + b]; },
              [&](int m, double x) { f[m * pl + b] = x; });
+  if (C.fuse_sp && pos >= 0 && pos <= S.N) {  // |v_n| at interface pos, between cells pos-1 and pos
+    const double uLc[4] = {q[b - 1], q[pl + b - 1], q[2 * pl + b - 1], q[3 * pl + b - 1]};
+    const double uRc[4] = {q[b], q[pl + b], q[2 * pl + b], q[3 * pl + b]};
+    C.dsp[dir][(long)l * (S.N + 1) + pos] =
+        dir == 0 ? ns2d_roe_dsp<0>(C.P.gamma, uLc, uRc) : ns2d_roe_dsp<1>(C.P.gamma, uLc, uRc);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1228,6 +1247,13 @@ __device__ __forceinline__ void recon_pair_lds(const CaseDev &C, int dir, long s
   const int hi_pos = N + g - 1;
   const double *ends = C.ends + 4 * ((dir ? recon_systems(C.Sx, nv, 1) : 0) + sy);
   double *L = lines + (threadIdx.x / T) * line_region(T, m);
+  // species pairs whose two teams are different waves (T = 64) meet through
+  // an arrival counter per pair, cleared before any wave can reach it
+  __shared__ unsigned pair_arrived[2];
+  if (C.fuse_sp && T == 64) {
+    if (threadIdx.x < 2) pair_arrived[threadIdx.x] = 0u;
+    __syncthreads();
+  }
   // the line, positions -2 .. T*m+2 (zero past its last ghost)
   const int tot = T * m + 5;
   for (int b = 0; b < tot; b += T) {
@@ -1290,6 +1316,23 @@ __device__ __forceinline__ void recon_pair_lds(const CaseDev &C, int dir, long s
 #pragma unroll
   for (int i = 0; i < kMaxRows; i++) stash[i * 256 + threadIdx.x] = X[i];  // this lane's own rows
   solve(std::integral_constant<bool, false>(), 1);
+  // a species of a case whose reconstruction writes the species fluxes: the
+  // |v_n| of this lane's interfaces, loaded now so that the combination step
+  // below hides their latency (the line's N + 1 interfaces take at most
+  // kMaxRows + 1 per lane)
+  constexpr int KO = kMaxRows + 1;
+  const bool fsp = C.fuse_sp && v >= 4;
+  double dvv[KO];
+#pragma unroll
+  for (int it = 0; it < KO; it++) dvv[it] = 0.0;
+  if (fsp) {
+    const double *dv = C.dsp[dir] + (long)l * (N + 1);
+#pragma unroll
+    for (int it = 0; it < KO; it++) {
+      const int k = it * T + lane;
+      if (active && k <= N) dvv[it] = dv[k];
+    }
+  }
   team_lds_sync(T);  // every window of the team is read
 #pragma unroll
   for (int i = 0; i < kMaxRows; i++) {
@@ -1299,6 +1342,42 @@ __device__ __forceinline__ void recon_pair_lds(const CaseDev &C, int dir, long s
     }
   }
   team_lds_sync(T);
+  if (fsp) {
+    // A species: its Roe flux f_I = 0.5 (fL + fR) - |v_n| 0.5 (uR - uL), the
+    // upwinder's arithmetic, is written here from the combinations of the
+    // pair's two teams (u and f sources of one line and species), which sit
+    // side by side in the block.  A team of at most 32 lanes shares its wave
+    // with its partner, so the wave's own LDS sync has made both visible and
+    // the flux team writes.  Teams of 64 lanes are two waves: each publishes
+    // its combination with a release increment of the pair's counter, and the
+    // one that arrives second (acquire) writes, so neither waits.  The two end
+    // interfaces take the combinations k_recon_ends wrote.
+    bool writer = q == 1;
+    if (T == 64) {
+      unsigned prev = 0u;
+      if (lane == 0)
+        prev = __hip_atomic_fetch_add(&pair_arrived[(threadIdx.x >> 6) >> 1], 1u, __ATOMIC_ACQ_REL,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+      writer = __shfl(prev, 0, 64) == 1u;
+    }
+    if (writer && active) {
+      const double *Lp = lines + ((threadIdx.x / T) ^ 1) * line_region(T, m);
+      const double *Lf = q ? L : Lp, *Lu = q ? Lp : L;
+      double *const *io = dir ? C.iy : C.ix;
+      double *fo = (dir ? C.fIy : C.fIx) + line;
+#pragma unroll
+      for (int it = 0; it < KO; it++) {
+        const int k = it * T + lane;
+        if (k <= N) {
+          const bool end = k == 0 || k == N;
+          const double fs = end ? io[1][line + k] : Lf[lds_slot(k)];
+          const double ud = end ? io[0][line + k] : Lu[lds_slot(k)];
+          fo[k] = fs - dvv[it] * ud;
+        }
+      }
+    }
+    return;
+  }
   double *out = (dir ? C.iy : C.ix)[q] + line;  // [0]: 0.5*(uR - uL), [1]: 0.5*(fL + fR)
   for (int b = 1; b < N; b += T) {
     const int k = b + lane;
@@ -2469,7 +2548,7 @@ __global__ void __launch_bounds__(256, PIAFS_RECONF_MINB) k_recon_full(CASE_ARGS
 template <int DIR, int PAIR>
 __device__ __forceinline__ void upwind_ns2d_roe(const SweepGeom &S, int nv, double g, long t,
                                                 const double *__restrict__ q, double *const *io,
-                                                double *__restrict__ fI) {
+                                                double *__restrict__ fI, bool species = true) {
   const int nl = S.nlines, N = S.N;
   const int k = (int)(t % (N + 1)), l = (int)(t / (N + 1));
   const long pl = S.plane;
@@ -2508,7 +2587,8 @@ __device__ __forceinline__ void upwind_ns2d_roe(const SweepGeom &S, int nv, doub
   }
 #pragma unroll
   for (int m = 0; m < 4; m++) fI[m * vstride + ii] = fo[m];
-  for (int m = 4; m < nv; m++) fI[m * vstride + ii] = FS(m) - Dsp * UD(m);
+  if (species)  // else the reconstruction has written them
+    for (int m = 4; m < nv; m++) fI[m * vstride + ii] = FS(m) - Dsp * UD(m);
 }
 
 // mode: 0 Roe (Euler1DUpwind.c:27-93), 1 rf-char (:107-190), 2 llf-char
@@ -2673,10 +2753,10 @@ __global__ void __launch_bounds__(256) k_upwind(CASE_ARGS) {
       else upwind_ns2d_rusanov<1, 0>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.UT, C.iy, C.fIy);
     }
   } else if (t < nX) {
-    if (C.paired) upwind_ns2d_roe<0, 1>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
+    if (C.paired) upwind_ns2d_roe<0, 1>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx, !C.fuse_sp);
     else upwind_ns2d_roe<0, 0>(C.Sx, C.G.nv, C.P.gamma, t, C.U, C.ix, C.fIx);
   } else {
-    if (C.paired) upwind_ns2d_roe<1, 1>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.UT, C.iy, C.fIy);
+    if (C.paired) upwind_ns2d_roe<1, 1>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.UT, C.iy, C.fIy, !C.fuse_sp);
     else upwind_ns2d_roe<1, 0>(C.Sy, C.G.nv, C.P.gamma, t - nX, C.UT, C.iy, C.fIy);
   }
 }
@@ -3455,6 +3535,13 @@ bool team_shape(int N, int max_rows, int &team, int &rows) {
 }
 
 bool stage_tile_64(const CaseDev &c) { return c.G.ndims == 2 && (long)c.G.nx * c.G.ny >= 2048L * 2048L; }
+
+bool species_in_recon(const CaseDev &c) {
+  const bool two = (c.G.ndims == 2);
+  return PIAFS_RECON_SPECIES && PIAFS_RECON_LDS && two && c.model == 2 && c.paired && c.upwind == 0 &&
+         !c.R.characteristic && c.G.nv > 4 && c.dsp[0] && c.dsp[1] && c.Sx.team == 64 && c.Sy.team == 64 &&
+         long_rows(c.Sx, c.G.nv) == 0 && long_rows(c.Sy, c.G.nv) == 0;
+}
 
 void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
   const Grid &G = c.G;

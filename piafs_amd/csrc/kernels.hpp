@@ -64,6 +64,13 @@ constexpr int kMaxRows = PIAFS_MAX_ROWS;
 #ifndef PIAFS_RECON_FULL
 #define PIAFS_RECON_FULL 1
 #endif
+// NS2D paired Roe: the reconstruction writes the species' interface fluxes
+// f_I = 0.5 (fL + fR) - |v_n| 0.5 (uR - uL) itself, with |v_n| per interface
+// from k_flux, and k_upwind does the four flow variables only (1); or every
+// variable goes through k_upwind (0).
+#ifndef PIAFS_RECON_SPECIES
+#define PIAFS_RECON_SPECIES 1
+#endif
 // Lane exchanges of the line-solve scans and relays by DPP (1) or by LDS
 // permutes (0).
 #ifndef PIAFS_SCAN_DPP
@@ -136,6 +143,8 @@ struct CaseDev {
   ReconParams R;
   int model, upwind, nstages;  // upwind: 0 Roe, 1 rf-char, 2 llf-char, 3 rusanov
   int paired;    // reconstruction writes 0.5*(uR-uL), 0.5*(fL+fR) only (Roe with component-wise interpolation)
+  int fuse_sp;   // species interface fluxes written by the reconstruction (set by the plan; see species_in_recon)
+  double *dsp[2];  // per sweep and interface: Roe-average |v_n| with the entropy fix (k_flux, when fuse_sp)
   int stage_tile; // side of k_stage's transpose tiles, 32 or 64 (set by the plan)
   int terms;     // terms k_assemble adds: bit 0 hyperbolic, bit 1 parabolic, bit 2 source (7 in a step)
   double dt, A[16], b[4], c[4];
@@ -181,6 +190,10 @@ struct Plan {
 
 // Host side: blocks each case needs for each kernel class (0 = class unused).
 void case_blocks(const CaseDev &c, int blocks[K_NCLASS]);
+// Whether the reconstruction can write the species' Roe interface fluxes
+// itself (NS2D, paired Roe, teams of at most 64 lanes, no line longer than a
+// team, |v_n| buffers present).
+bool species_in_recon(const CaseDev &c);
 // Whether the case is large enough for k_stage's 64 x 64 tiles.
 bool stage_tile_64(const CaseDev &c);
 // doubles per interface of the characteristic CRWENO5 scratch of one sweep:

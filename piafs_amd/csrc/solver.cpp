@@ -70,6 +70,13 @@ static bool paired_mode(const Config &c) {
   return !c.characteristic && (c.upwind == UPW_ROE || c.upwind == UPW_RUSANOV);
 }
 
+// Cases whose species Roe fluxes the reconstruction can write (NS2D with
+// species, paired Roe, component-wise): they keep |v_n| per interface.
+static bool species_dsp(const Config &c) {
+  return PIAFS_RECON_SPECIES && c.model_id == MODEL_NS2D && c.ndims == 2 && c.nvars > 4 && paired_mode(c) &&
+         c.upwind == UPW_ROE;
+}
+
 size_t Case::plan_bytes(const Config &c) {
   Grid g = make_grid(c);
   size_t plane = g.plane, nv = c.nvars;
@@ -77,6 +84,7 @@ size_t Case::plan_bytes(const Config &c) {
   size_t ix = nv * (size_t)(c.dim[0] + 1) * (c.ndims == 2 ? c.dim[1] : 1);
   size_t iy = c.ndims == 2 ? nv * (size_t)(c.dim[1] + 1) * c.dim[0] : 0;
   n += (paired_mode(c) ? 3 : 5) * (ix + iy);
+  if (species_dsp(c)) n += (ix + iy) / nv;
   if (c.characteristic && c.crweno) {
     const int nf = c.model_id == MODEL_EULER1D ? 3 : 4;
     n += (size_t)(cw_row_fields(nf, (int)nv) + 4 * cw_back_fields(nf, (int)nv)) * (ix + iy) / nv;
@@ -211,6 +219,12 @@ int Case::setup(int dev, hipStream_t st) {
     D.flux_y = dalloc(st_sz, st);
     for (int k = 0; k < narr; k++) D.iy[k] = dalloc(iy, st);
     D.fIy = dalloc(iy, st);
+  }
+  D.dsp[0] = D.dsp[1] = nullptr;
+  if (species_dsp(cfg)) {  // |v_n| per interface, for the species fluxes written by the reconstruction
+    D.dsp[0] = dalloc((size_t)(D.Sx.N + 1) * D.Sx.nlines, st);
+    D.dsp[1] = dalloc((size_t)(D.Sy.N + 1) * D.Sy.nlines, st);
+    if (!D.dsp[0] || !D.dsp[1]) return fail(-2, "device allocation failed");
   }
   if (P.viscous) {
     D.Qd = dalloc(6 * (size_t)G.nxg * G.nyg, st);
@@ -423,6 +437,7 @@ int Engine::build() {
       hd[k] = cases[k]->h;
       hd[k].status = d_status + k;
     }
+    hd[k].fuse_sp = species_in_recon(hd[k]) ? 1 : 0;
   }
   plan.stage64 = true;
   for (int k = 0; k < nc; k++) plan.stage64 = plan.stage64 && stage_tile_64(hd[k]);
