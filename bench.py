@@ -36,6 +36,13 @@ HBM_PEAK_GBS = 8000.0
 STEP_BYTES_PER_CELL = lambda nv, ns: 40 * nv * ns  # noqa: E731
 
 
+def species_fused(nx, ny, nv, paired):
+    """Whether the library's reconstruction writes the species' Roe fluxes
+    itself: NS2D with species, paired Roe, and 64-lane line teams in both
+    sweeps (lines of 513 .. 1025 rows)."""
+    return paired and nv > 4 and 513 <= nx <= 1025 and 513 <= ny <= 1025
+
+
 def kernel_algorithmic_bytes(name, nx, ny, nv, g, nstages, crweno, paired=True):
     """Compulsory bytes of one launch of a kernel class (its inputs read once
     and outputs written once).  One launch covers both sweeps (x and y) of an
@@ -48,12 +55,17 @@ def kernel_algorithmic_bytes(name, nx, ny, nv, g, nstages, crweno, paired=True):
     pos = N + 2 * g
     sweeps = 2
     nio = 2 if paired else 4
+    fused = species_fused(nx, ny, nv, paired)
     if name in ("recon", "recon_full"):  # u and f at every position of every line; the interface arrays
+        if fused:  # flow: the two combinations; species: their flux, from |v_n| (one value per interface)
+            return sweeps * 8 * (2 * nv * pos * lines + (2 * 4 + (nv - 4) + 1) * (N + 1) * lines)
         return sweeps * 8 * (2 * nv * pos * lines + nio * nv * (N + 1) * lines)
     if name == "upwind":  # the interface arrays + 4 flow vars at the two cells; fI
+        if fused:  # the flow variables only
+            return sweeps * 8 * (nio * 4 * (N + 1) * lines + 2 * 4 * (N + 1) * lines + 4 * (N + 1) * lines)
         return sweeps * 8 * (nio * nv * (N + 1) * lines + 2 * 4 * (N + 1) * lines + nv * (N + 1) * lines)
-    if name == "flux":
-        return sweeps * 8 * 2 * nv * pos * lines
+    if name == "flux":  # the state and the flux at every position; |v_n| per interface when fused
+        return sweeps * 8 * (2 * nv * pos * lines + ((N + 1) * lines if fused else 0))
     if name == "stage_bc":  # u + up to one Udot read, U and UT written (ghosted)
         return 8 * nv * (nx + 2 * g) * (ny + 2 * g) * 4
     return None

@@ -491,6 +491,38 @@ def test_block_chain_runs_match_oracle(name):
     assert np.all(_rel(got, ref) <= 1e-11), _rel(got, ref)
 
 
+def _rel_max(got, ref):
+    scale = np.abs(ref).max(axis=tuple(range(ref.ndim - 1))) + 1e-300
+    return np.abs(got - ref).max(axis=tuple(range(ref.ndim - 1))) / scale
+
+
+def test_species_fluxes_from_the_reconstruction_match_oracle():
+    """NS2D with chemistry and paired Roe on lines of 513 .. 1025 rows in both
+    sweeps (64-lane teams), where the reconstruction writes the species' Roe
+    interface fluxes itself (|v_n| per interface from k_flux) and k_upwind the
+    flow variables: the interface flux of every line (piafs_interface_flux)
+    against the oracle's upwind output (orc_rhs_parts, NavierStokes2DUpwind.c:
+    26-92) to 1e-12, and one step against the oracle's to 1e-11."""
+    dk = decks.ns2d_open_box(nx=520, ny=530, n_iter=1, dt=5e-4, flow=(1.0, 0.3, 0.2, None), scheme="crweno5",
+                             upwinding="roe", include_chemistry=True, Re=1000.0, boundary=_PERIODIC_2D)
+    rng = np.random.default_rng(23)
+    u = dk.u * (1.0 + 1e-3 * rng.uniform(-1, 1, dk.u.shape))
+    with tempfile.TemporaryDirectory() as d:
+        dk.write(d)
+        oc = orc.OCase(d)
+        P = oc.rhs_parts(u, 0.45)
+        with piafs_amd.Case(d) as c:
+            fi = [c.interface_flux(u, 0.45, k) for k in range(2)]
+            c.step(1)
+            got = c.solution()
+        oc.step(1)
+        ref = oc.solution()
+    for k in range(2):
+        assert fi[k].shape == P["fi%d" % k].shape
+        assert np.all(_rel_max(fi[k], P["fi%d" % k]) <= 1e-12), (k, _rel_max(fi[k], P["fi%d" % k]))
+    assert np.all(_rel(got, ref) <= 1e-11), _rel(got, ref)
+
+
 def _eps(dk, eps):
     dk.weno["epsilon"] = eps
     return dk
