@@ -145,3 +145,29 @@ def test_autotune_keeps_results_bitwise():
         with piafs_amd.Case(dirs[0]) as c:
             c.step(8)
             assert np.array_equal(tuned, c.solution())
+
+
+def test_batch_with_species_fluxes_in_the_reconstruction():
+    """A member whose lines take 64-lane teams in both sweeps (600 x 560, NS2D
+    with chemistry, paired Roe), where the reconstruction writes the species'
+    fluxes itself, batched with members that do not (a 48 x 40 case of 16-lane
+    teams and a WENO5 case): every member equals its solo run bit for bit."""
+    periodic = [("periodic", 0, 1, 0, 0, -1e99, 1e99), ("periodic", 0, -1, 0, 0, -1e99, 1e99),
+                ("periodic", 1, 1, -1e99, 1e99, 0, 0), ("periodic", 1, -1, -1e99, 1e99, 0, 0)]
+    mem = [decks.ns2d_open_box(nx=600, ny=560, n_iter=3, dt=5e-4, flow=(1.0, 0.3, 0.2, None), scheme="crweno5",
+                               include_chemistry=True, Re=1000.0, boundary=periodic),
+           decks.ns2d_photochem(nx=48, ny=40, n_iter=3),
+           decks.ns2d_open_box(nx=64, ny=600, n_iter=3, dt=5e-4, flow=(1.0, 0.3, 0.2, None), scheme="weno5",
+                               include_chemistry=True, Re=1000.0, boundary=periodic)]
+    with tempfile.TemporaryDirectory() as tmp:
+        dirs = []
+        for k, dk in enumerate(mem):
+            dirs.append(os.path.join(tmp, f"m{k}"))
+            dk.write(dirs[-1])
+        with piafs_amd.Batch(dirs) as b:
+            b.step(3)
+            got = [b.solution(k) for k in range(len(dirs))]
+        for k, d in enumerate(dirs):
+            with piafs_amd.Case(d) as c:
+                c.step(3)
+                assert np.array_equal(got[k], c.solution()), k
