@@ -38,9 +38,9 @@ STEP_BYTES_PER_CELL = lambda nv, ns: 40 * nv * ns  # noqa: E731
 
 def species_fused(nx, ny, nv, paired):
     """Whether the library's reconstruction writes the species' Roe fluxes
-    itself: NS2D with species, paired Roe, and 64-lane line teams in both
-    sweeps (lines of 513 .. 1025 rows)."""
-    return paired and nv > 4 and 513 <= nx <= 1025 and 513 <= ny <= 1025
+    itself: NS2D with species, paired Roe, and 32- or 64-lane line teams in
+    both sweeps (lines of 258 .. 1025 rows)."""
+    return paired and nv > 4 and 258 <= nx <= 1025 and 258 <= ny <= 1025
 
 
 def kernel_algorithmic_bytes(name, nx, ny, nv, g, nstages, crweno, paired=True):

@@ -1318,14 +1318,14 @@ __device__ __forceinline__ void recon_pair_lds(const CaseDev &C, int dir, long s
   solve(std::integral_constant<bool, false>(), 1);
   // a species of a case whose reconstruction writes the species fluxes: the
   // |v_n| of this lane's interfaces, loaded now so that the combination step
-  // below hides their latency (the line's N + 1 interfaces take at most
-  // kMaxRows + 1 per lane)
-  constexpr int KO = kMaxRows + 1;
+  // below hides their latency (the line's N + 1 <= T kMaxRows + 2 interfaces
+  // take at most kMaxRows + 2 per lane)
+  constexpr int KO = kMaxRows + 2;
   const bool fsp = C.fuse_sp && v >= 4;
   double dvv[KO];
 #pragma unroll
   for (int it = 0; it < KO; it++) dvv[it] = 0.0;
-  if (fsp) {
+  if (fsp && (T == 64 || q == 1)) {  // a team of at most 32 lanes writes when it is the flux team
     const double *dv = C.dsp[dir] + (long)l * (N + 1);
 #pragma unroll
     for (int it = 0; it < KO; it++) {
@@ -3536,10 +3536,17 @@ bool team_shape(int N, int max_rows, int &team, int &rows) {
 
 bool stage_tile_64(const CaseDev &c) { return c.G.ndims == 2 && (long)c.G.nx * c.G.ny >= 2048L * 2048L; }
 
+#ifndef PIAFS_SP_TEAM_MIN
+#define PIAFS_SP_TEAM_MIN 32
+#endif
+#ifndef PIAFS_SP_TEAM_MAX
+#define PIAFS_SP_TEAM_MAX 64
+#endif
 bool species_in_recon(const CaseDev &c) {
   const bool two = (c.G.ndims == 2);
   return PIAFS_RECON_SPECIES && PIAFS_RECON_LDS && two && c.model == 2 && c.paired && c.upwind == 0 &&
-         !c.R.characteristic && c.G.nv > 4 && c.dsp[0] && c.dsp[1] && c.Sx.team == 64 && c.Sy.team == 64 &&
+         !c.R.characteristic && c.G.nv > 4 && c.dsp[0] && c.dsp[1] && c.Sx.team <= PIAFS_SP_TEAM_MAX &&
+         c.Sy.team <= PIAFS_SP_TEAM_MAX && c.Sx.team >= PIAFS_SP_TEAM_MIN && c.Sy.team >= PIAFS_SP_TEAM_MIN &&
          long_rows(c.Sx, c.G.nv) == 0 && long_rows(c.Sy, c.G.nv) == 0;
 }
 

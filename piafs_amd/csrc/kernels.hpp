@@ -191,8 +191,8 @@ struct Plan {
 // Host side: blocks each case needs for each kernel class (0 = class unused).
 void case_blocks(const CaseDev &c, int blocks[K_NCLASS]);
 // Whether the reconstruction can write the species' Roe interface fluxes
-// itself (NS2D, paired Roe, teams of at most 64 lanes, no line longer than a
-// team, |v_n| buffers present).
+// itself (NS2D, paired Roe, teams of 32 or 64 lanes -- lines of 258 to 1025
+// rows -- no line longer than a team, |v_n| buffers present).
 bool species_in_recon(const CaseDev &c);
 // Whether the case is large enough for k_stage's 64 x 64 tiles.
 bool stage_tile_64(const CaseDev &c);
