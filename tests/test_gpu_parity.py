@@ -496,14 +496,17 @@ def _rel_max(got, ref):
     return np.abs(got - ref).max(axis=tuple(range(ref.ndim - 1))) / scale
 
 
-def test_species_fluxes_from_the_reconstruction_match_oracle():
-    """NS2D with chemistry and paired Roe on lines of 513 .. 1025 rows in both
-    sweeps (64-lane teams), where the reconstruction writes the species' Roe
-    interface fluxes itself (|v_n| per interface from k_flux) and k_upwind the
-    flow variables: the interface flux of every line (piafs_interface_flux)
-    against the oracle's upwind output (orc_rhs_parts, NavierStokes2DUpwind.c:
-    26-92) to 1e-12, and one step against the oracle's to 1e-11."""
-    dk = decks.ns2d_open_box(nx=520, ny=530, n_iter=1, dt=5e-4, flow=(1.0, 0.3, 0.2, None), scheme="crweno5",
+@pytest.mark.parametrize("nx,ny", [(520, 530), (300, 400)])
+def test_species_fluxes_from_the_reconstruction_match_oracle(nx, ny):
+    """NS2D with chemistry and paired Roe on lines whose teams have 64 lanes
+    (513 .. 1025 rows: the pair's two teams are two waves and meet through an
+    LDS counter) or 32 lanes (258 .. 512 rows: one wave) in both sweeps, where
+    the reconstruction writes the species' Roe interface fluxes itself (|v_n|
+    per interface from k_flux) and k_upwind the flow variables: the interface
+    flux of every line (piafs_interface_flux) against the oracle's upwind
+    output (orc_rhs_parts, NavierStokes2DUpwind.c:26-92) to 1e-12, and one step
+    against the oracle's to 1e-11."""
+    dk = decks.ns2d_open_box(nx=nx, ny=ny, n_iter=1, dt=5e-4, flow=(1.0, 0.3, 0.2, None), scheme="crweno5",
                              upwinding="roe", include_chemistry=True, Re=1000.0, boundary=_PERIODIC_2D)
     rng = np.random.default_rng(23)
     u = dk.u * (1.0 + 1e-3 * rng.uniform(-1, 1, dk.u.shape))
