@@ -1950,7 +1950,19 @@ __device__ __forceinline__ double qget(double v, int src) {  // src is a constan
   }
 }
 
-// inverse of the block whose row ri this lane holds in bp; every lane gets the whole inverse
+// inverse of the block whose row ri this lane holds in bp; every lane gets the
+// whole inverse.  Its divisions (elimination factors and back substitution)
+// take wdiv: the IEEE quotient's bits for the blocks' operands, except that an
+// exactly zero numerator -0 over a positive pivot gives +0; the inverse's
+// entries are never divisors, so no nonzero value downstream changes.
+#ifndef PIAFS_CW_FAST_INV
+#define PIAFS_CW_FAST_INV 1
+#endif
+#if PIAFS_CW_FAST_INV
+#define CW_DIV(n, d) wdiv(n, d)
+#else
+#define CW_DIV(n, d) ((n) / (d))
+#endif
 template <int NF>
 __device__ __forceinline__ void cw_inv_quad(const double (&bp)[NF], int ri, double (&Bi)[NF][NF]) {
   double ac[NF], bb[NF];
@@ -1968,7 +1980,7 @@ __device__ __forceinline__ void cw_inv_quad(const double (&bp)[NF], int ri, doub
       pb[k] = qget(bb[k], i);
     }
     if (ri > i) {
-      const double f = ac[i] / pa[i];
+      const double f = CW_DIV(ac[i], pa[i]);
 #pragma unroll
       for (int k = i + 1; k < NF; k++) ac[k] -= (f * pa[k]);
 #pragma unroll
@@ -1984,7 +1996,7 @@ __device__ __forceinline__ void cw_inv_quad(const double (&bp)[NF], int ri, doub
         double sum = 0.0;
 #pragma unroll
         for (int j = i + 1; j < NF; j++) sum += (ac[j] * Bi[j][k]);
-        bb[k] = (bb[k] - sum) / ac[i];
+        bb[k] = CW_DIV(bb[k] - sum, ac[i]);
       }
     }
 #pragma unroll
