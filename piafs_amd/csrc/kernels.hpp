@@ -82,6 +82,11 @@ constexpr int kMaxRows = PIAFS_MAX_ROWS;
 #ifndef PIAFS_ASM_PIPELINE
 #define PIAFS_ASM_PIPELINE 1
 #endif
+// the tile kernels (k_assemble, k_stage with 32-point tiles) give each XCD a
+// contiguous band of tiles (1) or take tiles in workgroup order (0)
+#ifndef PIAFS_XCD_ORDER
+#define PIAFS_XCD_ORDER 1
+#endif
 // blocks per CU that the register allocation of k_recon_full aims at (2: two
 // waves per SIMD, which the DPP exchanges need a few registers' spill for)
 #ifndef PIAFS_RECONF_MINB
