@@ -524,9 +524,12 @@ __device__ __forceinline__ void stage_body(CASE_ARGS) {
   constexpr int SR = 256 / ST;
   __shared__ double tile[ST][ST + 1];
   const int ntx = (G.nx + ST - 1) / ST, nty = (G.ny + ST - 1) / ST;
-  // (64-point tiles: workgroup order, which timed faster for them at 4096^2)
+  // (64-point tiles: workgroup order, which timed faster for them at 4096^2);
+  // the case's block count is padded to a multiple of eight
+  const int nb = G.nv * ntx * nty;
   const int lid = (int)(blockIdx.x - firstblock[cidx]);
-  const int id = ST == 32 ? xcd_order(lid, G.nv * ntx * nty, firstblock[cidx]) : lid;
+  const int id = ST == 32 ? xcd_order(lid, (nb + 7) & ~7, firstblock[cidx]) : lid;
+  if (id >= nb) return;
   const int v = id / (ntx * nty), rem = id % (ntx * nty);
   const int i0 = (rem % ntx) * ST, j0 = (rem / ntx) * ST;
   const int tx = threadIdx.x & (ST - 1), ty = threadIdx.x / ST;
@@ -3124,7 +3127,9 @@ __global__ void __launch_bounds__(256) k_assemble(CASE_ARGS) {
   // 2D: a 32 x 32 tile of points, four per thread
   __shared__ double tl[32][33];
   const int ntx = (G.nx + 31) / 32;
-  const int id = xcd_order((int)(blockIdx.x - firstblock[cidx]), ntx * ((G.ny + 31) / 32), firstblock[cidx]);
+  const int nb = ntx * ((G.ny + 31) / 32);  // tiles; the case's blocks are padded to a multiple of eight
+  const int id = xcd_order((int)(blockIdx.x - firstblock[cidx]), (nb + 7) & ~7, firstblock[cidx]);
+  if (id >= nb) return;
   const int i0 = (id % ntx) * 32, j0 = (id / ntx) * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int i = i0 + tx;
@@ -3580,7 +3585,10 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
   const bool two = (G.ndims == 2);
   auto nb = [](long n) { return (int)((n + TB - 1) / TB); };
   const int st = c.stage_tile == 64 ? 64 : 32;
-  blocks[K_STAGE] = two ? ((G.nx + st - 1) / st) * ((G.ny + st - 1) / st) * G.nv : nb((long)G.nx * G.nv);
+  // the tile kernels' counts are padded to a multiple of eight, so that every
+  // case's tiles start on one and can be taken in xcd_order
+  auto pad8 = [](int n) { return (n + 7) & ~7; };
+  blocks[K_STAGE] = pad8(two ? ((G.nx + st - 1) / st) * ((G.ny + st - 1) / st) * G.nv : nb((long)G.nx * G.nv));
   blocks[K_GHOST] = nb(ghost_points(G) * G.nv);
   blocks[K_FLUX] = nb((long)(c.Sx.N + 2 * c.Sx.g) * c.Sx.nlines + (two ? (long)(c.Sy.N + 2 * c.Sy.g) * c.Sy.nlines : 0));
   blocks[K_CWROWS] = blocks[K_CWSOLVE] = blocks[K_LROWS] = blocks[K_LSOLVE] = 0;
@@ -3610,7 +3618,7 @@ void case_blocks(const CaseDev &c, int blocks[K_NCLASS]) {
   const bool visc = two && c.P.viscous;
   blocks[K_QDERIV] = visc ? nb((long)G.nxg * G.nyg) : 0;
   blocks[K_VISCFLUX] = visc ? nb((long)G.nxg * G.nyg) : 0;
-  blocks[K_ASSEMBLE] = two ? ((G.nx + 31) / 32) * ((G.ny + 31) / 32) : nb(G.nx);
+  blocks[K_ASSEMBLE] = pad8(two ? ((G.nx + 31) / 32) * ((G.ny + 31) / 32) : nb(G.nx));
   blocks[K_UPDATE] = nb((long)G.nx * G.ny * G.nv);
 }
 
